@@ -1,0 +1,247 @@
+/*
+ * ufomi.h -- C ABI of the MI355X-native FBP hot path of ufo-filters.
+ *
+ * Plain C: no GLib, no torch types, no HIP types in the signatures.  Device
+ * pointers are caller-owned (hipMalloc'd or any framework's device memory on
+ * the context's device); streams are passed as opaque `void *` hipStream_t.
+ * Every entry point returns UFOMI_OK (0) or a negative ufomi_status; the
+ * message of the last failure on the calling thread is ufomi_last_error().
+ *
+ * Each entry point cites the reference interface (ufo-filters 0.17.0) whose
+ * computation it replaces.  The GObject task shims (include/ufo_tasks.h)
+ * translate task properties into these calls.
+ */
+#ifndef UFOMI_H
+#define UFOMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    UFOMI_OK = 0,
+    UFOMI_ERR_ARG = -1,          /* invalid argument (null pointer, size out of range) */
+    UFOMI_ERR_HIP = -2,          /* HIP runtime error */
+    UFOMI_ERR_NOT_POW2 = -3,     /* filter width not a power of two (ufo-filter-task.c:295-300) */
+    UFOMI_ERR_UNSUPPORTED = -4,  /* option the reference accepts but this build rejects */
+    UFOMI_ERR_PROJECTIONS = -5,  /* sinogram taller than num-projections (ufo-backproject-task.c:231-237) */
+    UFOMI_ERR_SIZE = -6,         /* fft size-x smaller than input (ufo-fft-task.c:139-143), crop larger than input (ufo-ifft-task.c:130-134) */
+    UFOMI_ERR_NO_DEVICE = -7,    /* no HIP device / extension not usable */
+    UFOMI_ERR_GRAY_MAP = -8,     /* gray-map-min >= gray-map-max for integer store (ufo-general-backproject-task.c:1448-1453) */
+    UFOMI_ERR_MEMORY = -9,       /* volume does not fit (ufo-general-backproject-task.c:1542-1546) */
+    UFOMI_ERR_NUM_PROJECTIONS = -10, /* general-backproject without num-projections (:1429-1433) */
+    UFOMI_ERR_LAYOUT = -11,      /* ifft input is not complex (ufo-ifft-task.c:121-125) */
+    UFOMI_ERR_INCOMPLETE = -12   /* fewer projections than declared (ufo-general-backproject-task.c:1733-1738) */
+} ufomi_status;
+
+/* Storage type of a filtered sinogram / projection handed to a backprojector. */
+typedef enum { UFOMI_F32 = 0, UFOMI_F16 = 1, UFOMI_BF16 = 2 } ufomi_dtype;
+
+/* Filter enum of ufo-filter-task.c:51-68, same order and nicks. */
+typedef enum {
+    UFOMI_FILTER_RAMP = 0,
+    UFOMI_FILTER_RAMP_FROMREAL = 1,
+    UFOMI_FILTER_BUTTERWORTH = 2,
+    UFOMI_FILTER_FARIS_BYER = 3,
+    UFOMI_FILTER_HAMMING = 4,
+    UFOMI_FILTER_BH3 = 5,
+    /* Extension without a reference counterpart: Shepp-Logan window on the ramp,
+       c_k = |f| sinc(f / 2), f = k/N.  No reference parity exists for it. */
+    UFOMI_FILTER_SHEPP_LOGAN = 6
+} ufomi_filter_type;
+
+typedef struct ufomi_ctx ufomi_ctx;
+
+/* ---------------------------------------------------------------- context */
+
+/* One context per task thread and device; owns a twiddle table and scratch. */
+int ufomi_create(int device, ufomi_ctx **out);
+int ufomi_destroy(ufomi_ctx *ctx);
+/* Launch subsequent work on `hip_stream` (NULL = the legacy default stream). */
+int ufomi_set_stream(ufomi_ctx *ctx, void *hip_stream);
+int ufomi_synchronize(ufomi_ctx *ctx);
+const char *ufomi_last_error(void);
+const char *ufomi_version(void);
+
+/* --------------------------------------------------------- ramp filtering */
+
+/*
+ * Host-side coefficients of the `filter` task, float32[width], width = 2N,
+ * exactly the buffer ufo_filter_task_get_requisition uploads
+ * (src/ufo-filter-task.c:290-335), including the FFT of the real-space Ram-Lak
+ * kernel for ramp-fromreal (:317-334), done here in FP64 and rounded once.
+ * Parameters are the task properties (:481-521): cutoff, order, tau, theta, scale.
+ */
+int ufomi_filter_coefficients(int filter_type, unsigned width, float scale, float cutoff,
+                              float order, float tau, float theta, float *coefficients);
+
+/*
+ * `fft dimensions=1 size-x=N`: P real rows of W floats (row pitch W) ->
+ * P rows of N interleaved complex (2N floats), zero padded, forward
+ * unnormalised.  Replaces fft_spread (src/kernels/fft.cl:20-48) + oclfft
+ * (src/ufo-fft-task.c:226-352 -> src/common/ufo-fft.c:123-139).  N: power of
+ * two, W <= N <= 16384.
+ */
+int ufomi_fft_rows(ufomi_ctx *ctx, const float *d_in, int W, int P, int N, float *d_out);
+
+/*
+ * `filter`: out[r][i] = in[r][i] * coefficients[i] over the 2N floats of each
+ * of P rows (src/kernels/filter.cl:20-29, launched by ufo-filter-task.c:117-144).
+ * d_coefficients: device float[2N].  d_in may equal d_out.
+ */
+int ufomi_filter_apply(ufomi_ctx *ctx, const float *d_in, float *d_out, const float *d_coefficients,
+                       int width, int P);
+
+/*
+ * `ifft dimensions=1 crop-width=crop`: P rows of N interleaved complex ->
+ * P real rows of `crop` floats = Re(IFFT(row))[0:crop] / N
+ * (src/ufo-ifft-task.c:208-311 + fft_pack, src/kernels/fft.cl:50-82).
+ * crop = 0 means N.  The input is left unchanged (the reference transforms it
+ * in place as a side effect).
+ */
+int ufomi_ifft_rows(ufomi_ctx *ctx, const float *d_in, int N, int P, int crop, float *d_out);
+
+/*
+ * The fused chain `fft size-x=N ! filter ! ifft crop-width=crop` in one
+ * launch: rows [P][W] (row pitch W) -> [P][crop] in out_dtype (f32, f16 or
+ * bf16 storage).  d_coefficients: device float[2N] as produced by
+ * ufomi_filter_coefficients.  When c[2k] == c[2k+1] and c is even in k (every
+ * filter except faris-byer) two rows share one complex FFT; the test runs on
+ * the device in stream order, so the call never blocks the host.
+ */
+int ufomi_ramp_filter(ufomi_ctx *ctx, const float *d_in, int W, int P, int N,
+                      const float *d_coefficients, int crop, void *d_out, int out_dtype);
+
+/* ------------------------------------------------ parallel backprojection */
+
+typedef enum { UFOMI_BP_NEAREST = 0, UFOMI_BP_TEXTURE = 1 } ufomi_bp_mode;
+
+/* Properties of the `backproject` task (src/ufo-backproject-task.c:442-510, 520-541). */
+typedef struct {
+    unsigned num_projections;   /* 0 -> sinogram height (:225-227) */
+    unsigned offset;            /* first LUT entry used (kernel arg 6) */
+    double axis_pos;            /* <= 0 -> W / 2 (:125-134) */
+    double angle_step;          /* <= 0 -> pi / num_projections (:246-251) */
+    double angle_offset;        /* radians */
+    int mode;                   /* ufomi_bp_mode, default texture */
+    unsigned roi_x, roi_y, roi_width, roi_height;  /* width/height 0 -> W */
+} ufomi_bp_params;
+
+void ufomi_bp_params_init(ufomi_bp_params *p);
+
+/*
+ * `backproject`: n_slices sinograms [P_in][W] (slice pitch in elements, 0 =
+ * P_in * W) -> n_slices slices [roi_h][roi_w] (out slice pitch 0 = roi_w *
+ * roi_h).  Replaces backproject_tex / backproject_nearest
+ * (src/kernels/backproject.cl:24-94) and the LUT set-up of
+ * src/ufo-backproject-task.c:174-267.  Storage dtype: f32, f16 or bf16
+ * (accumulation always f32; nearest mode requires f32).
+ */
+int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in,
+                      int n_slices, size_t sino_slice_pitch, const ufomi_bp_params *params,
+                      float *d_out, size_t out_slice_pitch);
+
+/* ---------------------------------------------------- cone-beam weighting */
+
+/*
+ * `cone-beam-projection-weight` for one projection [H][W]
+ * (src/kernels/conebeam.cl:20-36, src/ufo-cone-beam-projection-weight-task.c:116-158).
+ * n_proj consecutive projections may be weighted in one call; per-projection
+ * values are given as arrays of length n_param (1 broadcasts, else n_proj).
+ */
+int ufomi_cone_weight(ufomi_ctx *ctx, const float *d_in, float *d_out, int W, int H, int n_proj,
+                      const double *center_x, const double *center_z, const double *source_distance,
+                      const double *detector_distance, const double *axis_angle_x, int n_param);
+
+/* ------------------------------------------------- general backprojection */
+
+/* Per-projection geometry, the arrays of the general-backproject task
+ * (src/ufo-general-backproject-task.c:2087-2311); one value broadcasts. */
+typedef struct {
+    const double *axis_angle_x, *axis_angle_y, *axis_angle_z;
+    const double *volume_angle_x, *volume_angle_y, *volume_angle_z;
+    const double *detector_angle_x, *detector_angle_y, *detector_angle_z;
+    const double *center_position_x, *center_position_z;
+    const double *source_position_x, *source_position_y, *source_position_z;
+    const double *detector_position_x, *detector_position_y, *detector_position_z;
+    int n_axis_angle_x, n_axis_angle_y, n_axis_angle_z;
+    int n_volume_angle_x, n_volume_angle_y, n_volume_angle_z;
+    int n_detector_angle_x, n_detector_angle_y, n_detector_angle_z;
+    int n_center_position_x, n_center_position_z;
+    int n_source_position_x, n_source_position_y, n_source_position_z;
+    int n_detector_position_x, n_detector_position_y, n_detector_position_z;
+} ufomi_gbp_geometry;
+
+typedef enum { UFOMI_ST_HALF = 0, UFOMI_ST_FLOAT, UFOMI_ST_DOUBLE, UFOMI_ST_UCHAR, UFOMI_ST_USHORT, UFOMI_ST_UINT } ufomi_store_type;
+typedef enum { UFOMI_ADDR_NONE = 0, UFOMI_ADDR_CLAMP_TO_EDGE = 1, UFOMI_ADDR_CLAMP = 2,
+               UFOMI_ADDR_REPEAT = 3, UFOMI_ADDR_MIRRORED_REPEAT = 4 } ufomi_addressing;
+
+typedef struct {
+    unsigned num_projections;
+    double overall_angle;        /* default 2 pi */
+    double z;                    /* slice z when region step is 0 */
+    double region[3];            /* z region (from, to, step); step 0 -> one slice at 0 */
+    double x_region[3], y_region[3];  /* step 0 -> (-W/2, 1) over W voxels */
+    int store_type;              /* ufomi_store_type, default float */
+    int addressing_mode;         /* default clamp */
+    double gray_map_min, gray_map_max;
+} ufomi_gbp_params;
+
+void ufomi_gbp_params_init(ufomi_gbp_params *p);
+void ufomi_gbp_geometry_init(ufomi_gbp_geometry *g);
+
+/* Volume dimensions the task's get_requisition would produce (:1464-1587). */
+int ufomi_gbp_volume_shape(const ufomi_gbp_params *p, int W, int *nx, int *ny, int *nz);
+
+/*
+ * `general-backproject` (parameter z): P projections [H][W] float32 (stacked,
+ * projection pitch H*W) -> volume [nz][ny][nx] in store_type.  Each
+ * projection's transformation chain (ufo-general-backproject-task.c:807-1048)
+ * is composed in FP64 into a projective 4x4 map on the host; the kernel
+ * accumulates all P projections per voxel in registers (no per-burst volume
+ * read-modify-write).
+ */
+int ufomi_gbp(ufomi_ctx *ctx, const float *d_projections, int W, int H, int P,
+              const ufomi_gbp_geometry *geometry, const ufomi_gbp_params *params, void *d_volume);
+
+/* ------------------------------------------------- synthetic phantom data */
+
+/*
+ * Seeded ellipse phantoms and their analytic parallel-beam sinograms, for
+ * benchmarks and tests (not part of the reference).  Ellipses: 6 doubles each
+ * (x0, y0, a, b, phi, density); ellipsoids (rotated about z): 8 doubles
+ * (x0, y0, z0, a, b, c, phi, density); pixel units relative to the rotation
+ * axis.  Centres uniform in the unit disk (ball) x 0.7, semi-axes U[0.05, 0.4],
+ * rotation U[0, pi), density U[-0.5, 1], all scaled by radius_px.
+ */
+int ufomi_phantom_ellipses(uint64_t seed, int n, double radius_px, double *ellipses);
+int ufomi_phantom_ellipsoids(uint64_t seed, int n, double radius_px, double *ellipsoids);
+/* Plane section z of n ellipsoids -> n ellipses (missed ones get density 0). */
+int ufomi_slice_ellipsoids(const double *ellipsoids, int n, double z, double *ellipses);
+/* n_slices sinograms [P][W] of n_ellipses ellipses each (host array
+ * [n_slices][n_ellipses][6]); detector pixel j at t = j + 0.5 - axis_pos,
+ * angle p * angle_step; FP64 line integrals rounded to float. */
+int ufomi_sinogram_ellipses(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, int n_slices,
+                            int W, int P, double axis_pos, double angle_step, float *d_sino);
+/* Adds N(0, (rel_sigma * max|d|)^2) noise from a counter-based generator;
+ * returns max|d| before the noise in *max_out (may be NULL). */
+int ufomi_add_noise(ufomi_ctx *ctx, float *d, size_t n, double rel_sigma, uint64_t seed, float *max_out);
+
+/* ------------------------------------------------ roofline microbenchmarks */
+
+/* Measurement helpers (no reference counterpart): a conflict-free LDS read
+ * loop (ds_read_b128) over `blocks` 256-thread workgroups, `iters` iterations
+ * (*bytes = LDS bytes read), and a streaming float4 copy of n floats; time
+ * them with events on the context's stream to get the device's sustained
+ * LDS and HBM bandwidth. */
+int ufomi_bench_lds_read(ufomi_ctx *ctx, int blocks, int iters, float *d_sink, double *bytes);
+int ufomi_bench_copy(ufomi_ctx *ctx, const float *d_in, float *d_out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,193 @@
+"""GPU parity of general-backproject (parallel, cone, laminography and the
+tilted / shifted variants), cone-beam weighting and the phantom generator.
+
+Bar: float32 volume within 1e-5 relative L2 of the FP64 oracle and of the
+float32 reference-semantics oracle; integer stores within one gray level.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import phantoms
+import ufomi
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, torch_dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(torch_dev)
+
+
+TORCH_STORE = {"half": "float16", "float": "float32", "double": "float64", "uchar": "uint8",
+               "ushort": "uint16", "uint": "uint32"}
+
+
+def _gbp(ctx, torch_dev, projs, store_type="float", addressing_mode="clamp", overall_angle=2 * math.pi,
+         region=(0, 0, 0), x_region=(0, 0, 0), y_region=(0, 0, 0), gray_map_min=0.0, gray_map_max=0.0,
+         num_projections=None, **geometry):
+    import torch
+    p, h, w = projs.shape
+    n = num_projections if num_projections is not None else p
+    params = ufomi.gbp_params(n, overall_angle=overall_angle, region=region, x_region=x_region,
+                              y_region=y_region, store_type=store_type, addressing_mode=addressing_mode,
+                              gray_map_min=gray_map_min, gray_map_max=gray_map_max)
+    geom = ufomi.gbp_geometry(**geometry)
+    shape = ufomi.gbp_volume_shape(params, w)
+    vol = torch.zeros(shape, dtype=getattr(torch, TORCH_STORE[store_type]), device=torch_dev)
+    ctx.gbp(_dev(projs, torch_dev), w, h, p, geom, params, vol)
+    return vol.cpu().numpy()
+
+
+def _check(ctx, torch_dev, projs, tol=1e-5, **kw):
+    got = _gbp(ctx, torch_dev, projs, **kw)
+    ref64 = oracle.gbp(projs, **kw)
+    ref32 = oracle.gbp(projs, mode="ref32", **kw)
+    assert got.shape == ref64.shape
+    assert rel_l2(got, ref64) < tol
+    assert rel_l2(got, ref32) < tol
+    return got, ref64
+
+
+def _projs(seed, p, h, w):
+    return np.random.default_rng(seed).standard_normal((p, h, w)).astype(np.float32)
+
+
+def test_parallel_tomography(ctx, torch_dev):
+    projs = _projs(1, 24, 8, 32)
+    _check(ctx, torch_dev, projs, center_position_x=[16.0], center_position_z=[4.0], region=(-3.0, 3.0, 1.0))
+
+
+def test_parallel_matches_backproject(ctx, torch_dev):
+    # the untilted parallel general backprojection equals `backproject` (y flipped, half-pixel grid)
+    import torch
+    rng = np.random.default_rng(21)
+    p, w, a = 40, 48, 23.5
+    sino = rng.standard_normal((p, w)).astype(np.float32)
+    vol = _gbp(ctx, torch_dev, sino[:, None, :], overall_angle=math.pi, x_region=(0.5 - a, 0.5 - a + w, 1.0),
+               y_region=(a - 0.5, a - 0.5 - w, -1.0), center_position_x=[a], center_position_z=[0.5])
+    out = torch.empty((1, w, w), device=torch_dev)
+    ctx.bp_parallel(_dev(sino, torch_dev), w, p, out, axis_pos=a)
+    assert rel_l2(vol[0], out.cpu().numpy()[0]) < 1e-5
+
+
+@pytest.mark.parametrize("tilt", [0.0, math.radians(30), 0.9])
+def test_laminography(ctx, torch_dev, tilt):
+    projs = _projs(2, 30, 24, 24)
+    _check(ctx, torch_dev, projs, axis_angle_x=[tilt], center_position_x=[12.0], center_position_z=[12.0],
+           region=(-4.0, 4.0, 1.0))
+
+
+def test_cone_beam(ctx, torch_dev):
+    projs = _projs(3, 36, 20, 24)
+    _check(ctx, torch_dev, projs, source_position_y=[-60.0], detector_position_y=[60.0],
+           center_position_x=[12.0], center_position_z=[10.0], region=(-5.0, 5.0, 1.0))
+
+
+@pytest.mark.parametrize("variant", [
+    dict(detector_angle_x=[0.1], detector_angle_z=[0.05]),
+    dict(detector_position_x=[1.5], detector_position_z=[-0.7]),
+    dict(source_position_x=[2.0], source_position_z=[1.0]),
+    dict(volume_angle_z=[0.3], volume_angle_x=[0.1]),
+    dict(axis_angle_y=[0.05], axis_angle_x=[0.2]),
+])
+def test_cone_beam_variants(ctx, torch_dev, variant):
+    projs = _projs(4, 20, 16, 20)
+    _check(ctx, torch_dev, projs, source_position_y=[-50.0], detector_position_y=[40.0],
+           center_position_x=[10.0], center_position_z=[8.0], region=(-3.0, 3.0, 1.0), **variant)
+
+
+def test_parallel_tilted_detector(ctx, torch_dev):
+    projs = _projs(5, 20, 16, 20)
+    _check(ctx, torch_dev, projs, detector_angle_x=[0.1], detector_angle_y=[0.05], center_position_x=[10.0],
+           center_position_z=[8.0], region=(-3.0, 3.0, 1.0))
+
+
+def test_per_projection_geometry(ctx, torch_dev):
+    p = 16
+    projs = _projs(6, p, 16, 20)
+    rng = np.random.default_rng(7)
+    _check(ctx, torch_dev, projs, source_position_y=[-50.0], detector_position_y=[40.0],
+           center_position_x=10.0 + rng.uniform(-1, 1, p), center_position_z=[8.0],
+           axis_angle_z=np.sort(rng.uniform(0, 2 * math.pi, p)), region=(-3.0, 3.0, 1.0))
+
+
+def test_regions_and_clamp_to_edge(ctx, torch_dev):
+    projs = _projs(8, 12, 16, 16)
+    _check(ctx, torch_dev, projs, addressing_mode="clamp_to_edge", x_region=(-10.0, 10.0, 0.5),
+           y_region=(-6.0, 9.0, 1.5), region=(-2.0, 2.0, 0.5), center_position_x=[8.0], center_position_z=[8.0])
+
+
+@pytest.mark.parametrize("store", ["half", "double", "uchar", "ushort", "uint"])
+def test_store_types(ctx, torch_dev, store):
+    projs = np.abs(_projs(9, 12, 12, 16))
+    kw = dict(center_position_x=[8.0], center_position_z=[6.0], region=(-2.0, 2.0, 1.0))
+    got = _gbp(ctx, torch_dev, projs, store_type=store, gray_map_min=0.0, gray_map_max=30.0, **kw)
+    ref = oracle.gbp(projs, **kw)
+    if store in ("half", "double"):
+        assert rel_l2(got.astype(np.float64), ref) < (1e-3 if store == "half" else 1e-5)
+    else:
+        maxval = {"uchar": 255.0, "ushort": 65535.0, "uint": 4294967295.0}[store]
+        exp = np.clip(maxval / 30.0 * ref, 0, maxval)
+        assert np.abs(got.astype(np.float64) - exp).max() <= 1.0 + 1e-6 * maxval
+
+
+def test_errors(ctx, torch_dev):
+    projs = _projs(10, 4, 8, 8)
+    with pytest.raises(ufomi.UfomiError) as e:
+        _gbp(ctx, torch_dev, projs, num_projections=0)
+    assert e.value.code == -10
+    with pytest.raises(ufomi.UfomiError) as e:
+        _gbp(ctx, torch_dev, projs, num_projections=6)
+    assert e.value.code == -12
+    with pytest.raises(ufomi.UfomiError) as e:
+        _gbp(ctx, torch_dev, projs, store_type="uchar", gray_map_min=1.0, gray_map_max=1.0)
+    assert e.value.code == -8
+    with pytest.raises(ufomi.UfomiError) as e:
+        _gbp(ctx, torch_dev, projs, addressing_mode="repeat")
+    assert e.value.code == -4
+
+
+def test_cone_weight(ctx, torch_dev):
+    import torch
+    p, h, w = 3, 20, 24
+    x = _projs(11, p, h, w)
+    out = torch.empty((p, h, w), device=torch_dev)
+    cx, cz, sod = [12.0, 11.5, 12.5], 10.0, 500.0
+    for tilt, odd in ((0.0, 700.0), (0.3, 500.0)):
+        ctx.cone_weight(_dev(x, torch_dev), out, w, h, p, cx, cz, sod, odd, tilt)
+        got = out.cpu().numpy()
+        for i in range(p):
+            ref = oracle.cone_weight_ref32(x[i], cx[i], cz, sod, odd, tilt)
+            np.testing.assert_allclose(got[i], ref, rtol=2e-6, atol=1e-7)
+
+
+def test_phantom_sinogram_vs_numpy(ctx, torch_dev):
+    import torch
+    W, P, axis = 96, 40, 47.5
+    ell = ufomi.phantom_ellipses(1234, 10, W / 2)
+    out = torch.empty((P, W), device=torch_dev)
+    ctx.sinogram_ellipses(ell, W, P, axis, math.pi / P, out)
+    ref = phantoms.sinogram(ell, W, P, axis, math.pi / P)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-5, atol=1e-4)
+    # ellipsoid plane sections: a slice through the centre of a single ellipsoid
+    e3 = np.array([[3.0, -2.0, 1.0, 10.0, 6.0, 4.0, 0.3, 0.7]])
+    sec = ufomi.slice_ellipsoids(e3, 1.0)
+    np.testing.assert_allclose(sec, [[3.0, -2.0, 10.0, 6.0, 0.3, 0.7]])
+    sec2 = ufomi.slice_ellipsoids(e3, 3.0)  # (z - z0) / c = 0.5 -> scale sqrt(0.75)
+    np.testing.assert_allclose(sec2[0, 2:4], np.array([10.0, 6.0]) * math.sqrt(0.75))
+
+
+def test_noise_is_seeded(ctx, torch_dev):
+    import torch
+    a = torch.ones(100000, device=torch_dev)
+    b = torch.ones(100000, device=torch_dev)
+    m = ctx.add_noise(a, 0.01, 99)
+    ctx.add_noise(b, 0.01, 99)
+    assert m == 1.0
+    assert torch.equal(a, b)
+    d = (a - 1).cpu().numpy()
+    assert abs(d.std() - 0.01) < 5e-4 and abs(d.mean()) < 2e-4

@@ -1,0 +1,325 @@
+"""Pins the CPU oracle (oracle/ufo_oracle.c) before it is trusted as the checker.
+
+Each check uses a source independent of the oracle's own formula: numpy.fft
+golden vectors, closed forms, brute force on tiny inputs, invariants
+(linearity, symmetry), or a second geometric formulation (matrix algebra).
+A planted mistake in the oracle (dropped term, wrong sign, swapped operand)
+fails at least one of them; see test_planted_mistakes_are_caught.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+GOLDEN = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+
+
+# ------------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [4, 8, 16, 64, 128])
+def test_fft_vs_naive_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for d in (-1, 1):
+        assert np.abs(oracle.fft_c2c(x, d) - oracle.dft_naive(x, d)).max() < 1e-12 * n
+
+
+def test_naive_dft_vs_numpy():
+    x = np.random.default_rng(3).standard_normal(48)
+    np.testing.assert_allclose(oracle.dft_naive(x), np.fft.fft(x), atol=1e-11)
+
+
+def test_fft_task_vs_golden():
+    for key in GOLDEN.files:
+        if key.startswith("fft_in_"):
+            p, w, n = map(int, key.split("_")[2:])
+            got = oracle.fft_task(GOLDEN[key], n)
+            ref = GOLDEN["fft_out_%d_%d_%d" % (p, w, n)]
+            assert np.abs(got[:, 0::2] + 1j * got[:, 1::2] - ref).max() < 1e-9
+
+
+def test_fft_round_trip_and_parseval():
+    x = np.random.default_rng(1).standard_normal(256) + 0j
+    X = oracle.fft_c2c(x, -1)
+    np.testing.assert_allclose(oracle.fft_c2c(X, +1) / 256, x, atol=1e-13)
+    assert abs(np.sum(np.abs(X) ** 2) / 256 - np.sum(np.abs(x) ** 2)) < 1e-9
+
+
+def test_ifft_task_crop_and_scale():
+    p, w, n = 5, 30, 64
+    x = np.linspace(-1, 2, num=p * w, dtype=np.float32).reshape(p, w)
+    back = oracle.ifft_task(oracle.fft_task(x, n), w)
+    np.testing.assert_allclose(back, x, atol=1e-12)
+
+
+# --------------------------------------------------------- filter coefficients
+
+def _mirror_ok(c):
+    w = len(c)
+    for k in range(w // 2 + 2, w, 2):
+        assert c[k] == c[w - k] and c[k + 1] == c[w - k + 1]
+
+
+@pytest.mark.parametrize("n", [8, 64, 1024, 4096])
+def test_ramp_fromreal_closed_form(n):
+    c = oracle.filter_coefficients(2 * n, "ramp-fromreal")
+    h = GOLDEN["ramp_fromreal_%d" % n]
+    np.testing.assert_allclose(c[0::2], h, atol=2e-8)
+    np.testing.assert_allclose(c[1::2], h, atol=2e-8)
+    assert np.array_equal(c[0::2], c[1::2])           # real spectrum (re == im of (1+i) H)
+    assert np.array_equal(c[2::2], c[2::2][::-1])     # even: H_k == H_{N-k}
+    assert 0 < c[0] < 0.25 / n * 4                   # small positive DC
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.5])
+def test_ramp_closed_form(scale):
+    n = 64
+    c = oracle.filter_coefficients(2 * n, "ramp", scale=scale)
+    k = np.arange(1, n // 2 + 1)
+    np.testing.assert_allclose(c[2 * k], (k / n * scale).astype(np.float32), rtol=1e-7)
+    assert c[0] == np.float32(0.5 / (2 * n))
+    _mirror_ok(c)
+
+
+def test_window_filters_closed_form():
+    n = 128
+    f = np.arange(n // 2 + 1) / n
+    bw = f / (1 + (f / np.float32(0.5)) ** (2 * 4.0))
+    np.testing.assert_allclose(oracle.filter_coefficients(2 * n, "butterworth")[0:n + 2:2], bw, rtol=1e-6, atol=1e-9)
+    hm = np.where(f < 0.5, f * (0.54 + 0.46 * np.cos(np.pi * f / 0.5)), 0.0)
+    np.testing.assert_allclose(oracle.filter_coefficients(2 * n, "hamming")[0:n + 2:2], hm, rtol=1e-6, atol=1e-9)
+    bh = f * (0.42 + 0.5 * np.cos(f * np.pi) + 0.08 * np.cos(2 * f * np.pi))
+    got = oracle.filter_coefficients(2 * n, "bh3")
+    np.testing.assert_allclose(got[2:n + 2:2], bh[1:], rtol=1e-6, atol=1e-9)
+    assert got[0] == np.float32(0.5 / (2 * n))  # bh3 keeps the DC of :303-304
+    for name in ("butterworth", "hamming", "bh3"):
+        _mirror_ok(oracle.filter_coefficients(2 * n, name))
+
+
+def test_faris_byer_index_pattern():
+    # Value of every index as derived by hand from ufo-filter-task.c:254-276:
+    # odd j <= W/2-3 -> -sin(theta)/2 / ((j+2)^2 pi^2 tau); j = W/2-1 -> 1/(pi^2 tau (W/2-1));
+    # other indices below W/2 -> 0, except index 1 when W/2-1 == 1; then the mirror.
+    w, tau, theta = 32, np.float32(0.1), np.float32(1.0)
+    c = oracle.filter_coefficients(w, "faris-byer", tau=tau, theta=theta)
+    pst = np.pi ** 2 * float(tau)
+    exp = np.zeros(w)
+    for j in range(1, w // 2 - 2, 2):
+        exp[j] = -math.sin(theta) / 2 / ((j + 2) ** 2 * pst)
+    exp[w // 2 - 1] = 1 / (pst * (w // 2 - 1))
+    exp[w // 2 + 1] = 1 / (pst * (w // 2 + 1))  # x = W/2+1 is written by the first loop (x <= W/2 only when odd)... see below
+    # The first loop stops at x <= W/2, so index W/2+1 is never written before the mirror:
+    exp[w // 2 + 1] = 0.0
+    for k in range(w // 2 + 2, w, 2):
+        exp[k] = exp[w - k]
+        exp[k + 1] = exp[w - k + 1]
+    np.testing.assert_allclose(c, exp.astype(np.float32), rtol=1e-6, atol=1e-12)
+
+
+def test_non_power_of_two_rejected():
+    with pytest.raises(ValueError):
+        oracle.filter_coefficients(96)
+
+
+def test_ramp_chain_vs_numpy():
+    p, w, n = 6, 50, 128
+    x = np.random.default_rng(8).standard_normal((p, w)).astype(np.float32)
+    c = oracle.filter_coefficients(2 * n, "hamming")
+    X = np.fft.fft(x.astype(np.float64), n=n, axis=1)
+    Y = X.real * c[0::2] + 1j * X.imag * c[1::2]
+    ref = np.fft.ifft(Y, axis=1).real[:, :w]
+    np.testing.assert_allclose(oracle.ramp_chain(x, n, c, w), ref, atol=1e-12)
+
+
+# ------------------------------------------------------------ backprojection
+
+def test_bp_vs_brute_force_golden():
+    sino = GOLDEN["bp_brute_sino"]
+    axis = float(GOLDEN["bp_brute_axis"])
+    ref = GOLDEN["bp_brute_out"]
+    np.testing.assert_allclose(oracle.bp_texture_f64(sino, axis_pos=axis), ref, atol=1e-12)
+    assert rel_l2(oracle.bp_texture_ref32(sino, axis_pos=axis), ref) < 1e-6
+
+
+def test_bp_ref32_close_to_f64():
+    s = np.random.default_rng(4).standard_normal((90, 64)).astype(np.float32)
+    assert rel_l2(oracle.bp_texture_ref32(s), oracle.bp_texture_f64(s)) < 1e-5
+
+
+def test_bp_constant_sinogram():
+    w = 64
+    img = oracle.bp_texture_f64(np.full((50, w), 3.0))
+    y, x = np.mgrid[0:w, 0:w] + 0.5 - w / 2
+    inside = np.hypot(x, y) < w / 2 - 1.5
+    np.testing.assert_allclose(img[inside], 3.0 * math.pi, rtol=1e-12)
+    assert np.all(img[np.hypot(x, y) > w / 2 * math.sqrt(2)] == 0)
+
+
+def test_bp_linearity_and_pi_flip():
+    rng = np.random.default_rng(6)
+    a, b = rng.standard_normal((2, 20, 32))
+    np.testing.assert_allclose(oracle.bp_texture_f64(2 * a - 3 * b),
+                               2 * oracle.bp_texture_f64(a) - 3 * oracle.bp_texture_f64(b), atol=1e-11)
+    # shifting every angle by pi rotates the image by pi about the axis (axis = W/2)
+    r0 = oracle.bp_texture_f64(a)
+    r1 = oracle.bp_texture_f64(a, angle_offset=math.pi)
+    np.testing.assert_allclose(r1, r0[::-1, ::-1], atol=1e-10)
+
+
+def test_bp_nearest_vs_numpy_float32():
+    rng = np.random.default_rng(12)
+    p, w = 9, 16
+    s = rng.standard_normal((p, w)).astype(np.float32)
+    got = oracle.bp_nearest_ref32(s)
+    sl, cl, _ = oracle.bp_lut(p)
+    a = np.float32(w / 2)
+    f32 = np.float32
+    out = np.zeros((w, w), np.float32)
+    flat = s.ravel()
+    for iy in range(w):
+        for ix in range(w):
+            bx = f32(f32(f32(ix) - a) + f32(0)) + f32(0.5)
+            by = f32(f32(f32(iy) - a) + f32(0)) + f32(0.5)
+            acc = f32(0)
+            for q in range(p):
+                h = f32(f32(a + f32(bx * cl[q])) + f32(by * sl[q]))
+                i = int(f32(f32(q * w) + h))
+                if 0 <= i < flat.size:
+                    acc = f32(acc + flat[i])
+            out[iy, ix] = f32(f32(acc * f32(math.pi)) / f32(p))
+    np.testing.assert_array_equal(got, out)
+
+
+def test_fbp_of_disc_approaches_phantom():
+    # analytic sinogram of a centred disc of radius R: 2 sqrt(R^2 - t^2)
+    w, p, n, r = 128, 256, 256, 40.0
+    t = np.arange(w) + 0.5 - w / 2
+    sino = np.tile(2 * np.sqrt(np.maximum(r * r - t * t, 0)), (p, 1)).astype(np.float32)
+    filt = oracle.ramp_chain(sino, n, oracle.filter_coefficients(2 * n), w)
+    img = oracle.bp_texture_f64(filt)
+    y, x = np.mgrid[0:w, 0:w] + 0.5 - w / 2
+    rr = np.hypot(x, y)
+    assert abs(np.mean(img[rr < r - 4]) - 1.0) < 0.02
+    assert abs(np.mean(img[(rr > r + 4) & (rr < w / 2 - 4)])) < 0.02
+
+
+# --------------------------------------------------------------- cone weight
+
+def test_cone_weight_closed_form():
+    h, w = 5, 7
+    x = np.random.default_rng(2).standard_normal((h, w)).astype(np.float32)
+    cx, cz, sod, odd, tilt = 3.2, 2.1, 500.0, 700.0, 0.3
+    got = oracle.cone_weight_ref32(x, cx, cz, sod, odd, tilt)
+    m = sod / (sod + odd)
+    i = (np.arange(w)[None, :] - cx + 0.5) * m
+    j = (np.arange(h)[:, None] - cz + 0.5) * m
+    L = sod + odd
+    ref = x * sod * math.cos(tilt) / np.sqrt(L * L + i * i + j * j)
+    np.testing.assert_allclose(got, ref, rtol=2e-6)
+    got0 = oracle.cone_weight_ref32(x, cx, cz, sod, odd, 0.0)  # cos > 0.9999999 -> L = SOD
+    ref0 = x * sod / np.sqrt(sod * sod + i * i + j * j)
+    np.testing.assert_allclose(got0, ref0, rtol=2e-6)
+
+
+# -------------------------------------------------------- general backproject
+
+def test_gbp_parallel_equals_backproject():
+    # untilted parallel general-backproject == backproject after the y flip and
+    # half-pixel grid shift: x = bx, y = -by, center-x = axis, one detector row.
+    rng = np.random.default_rng(21)
+    p, w = 24, 32
+    sino = rng.standard_normal((p, w)).astype(np.float32)
+    a = 15.5
+    projs = sino[:, None, :]
+    vol = oracle.gbp(projs, overall_angle=math.pi, x_region=(0.5 - a, 0.5 - a + w, 1.0),
+                     y_region=(a - 0.5, a - 0.5 - w, -1.0), center_position_x=[a], center_position_z=[0.5])
+    ref = oracle.bp_texture_f64(sino, axis_pos=a)
+    np.testing.assert_allclose(vol[0], ref, atol=1e-10)
+    vol32 = oracle.gbp(projs, overall_angle=math.pi, x_region=(0.5 - a, 0.5 - a + w, 1.0),
+                       y_region=(a - 0.5, a - 0.5 - w, -1.0), center_position_x=[a], center_position_z=[0.5],
+                       mode="ref32")
+    assert rel_l2(vol32[0], ref) < 1e-5
+
+
+def _matrix_chain(x, y, z, th, tilt, cx, cz):
+    # Second formulation of parallel laminography: R = Rx(tilt) Rz(theta) as matrices.
+    rz = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    rx = np.array([[1, 0, 0], [0, math.cos(tilt), -math.sin(tilt)], [0, math.sin(tilt), math.cos(tilt)]])
+    v = rx @ rz @ np.array([x, y, z])
+    return v[0] + cx, v[2] + cz
+
+
+def _bilinear0(img, u, v):
+    h, w = img.shape
+    u, v = u - 0.5, v - 0.5
+    i0, j0 = math.floor(u), math.floor(v)
+    a, b = u - i0, v - j0
+    def t(i, j):
+        return float(img[j, i]) if 0 <= i < w and 0 <= j < h else 0.0
+    return ((1 - a) * (1 - b) * t(i0, j0) + a * (1 - b) * t(i0 + 1, j0)
+            + (1 - a) * b * t(i0, j0 + 1) + a * b * t(i0 + 1, j0 + 1))
+
+
+def test_gbp_laminography_vs_matrix_brute_force():
+    rng = np.random.default_rng(31)
+    p, h, w = 6, 9, 10
+    projs = rng.standard_normal((p, h, w)).astype(np.float32)
+    tilt, cx, cz = 0.4, 5.0, 4.5
+    vol = oracle.gbp(projs, axis_angle_x=[tilt], center_position_x=[cx], center_position_z=[cz],
+                     region=(-2.0, 2.0, 1.0))
+    nz, ny, nx = vol.shape
+    ref = np.zeros_like(vol)
+    for iz in range(nz):
+        for iy in range(ny):
+            for ix in range(nx):
+                x, y, z = -nx / 2 + ix, -ny / 2 + iy, -2.0 + iz
+                s = 0.0
+                for i in range(p):
+                    u, v = _matrix_chain(x, y, z, i / p * 2 * math.pi, tilt, cx, cz)
+                    s += _bilinear0(projs[i], u, v)
+                ref[iz, iy, ix] = s * 2 * math.pi / p
+    np.testing.assert_allclose(vol, ref, atol=1e-9)
+
+
+def test_gbp_cone_isocentre_closed_form():
+    # Constant projections: the isocentre voxel sees coeff = (S_y - D_y) / S_y at every angle,
+    # so the volume there is norm * P * coeff^2 = |overall| * coeff^2.
+    p, h, w = 12, 16, 16
+    projs = np.ones((p, h, w), np.float32)
+    sy, dy = -40.0, 40.0
+    vol = oracle.gbp(projs, source_position_y=[sy], detector_position_y=[dy], center_position_x=[8.0],
+                     center_position_z=[8.0], region=(0.0, 1.0, 1.0))
+    coeff = (sy - dy) / sy
+    assert abs(vol[0, 8, 8] - 2 * math.pi * coeff ** 2) < 1e-9
+
+
+def test_gbp_cone_magnification_closed_form():
+    # A voxel (x, 0, 0) at theta = 0 lands on u = x + cx (magnification S/(D-S) then (D-S)/S).
+    p, h, w = 1, 8, 32
+    projs = np.zeros((p, h, w), np.float32)
+    projs[0, 4, 20] = 1.0  # texel centre (20.5, 4.5)
+    sy, dy, cx, cz = -100.0, 100.0, 16.0, 4.5
+    vol = oracle.gbp(projs, source_position_y=[sy], detector_position_y=[dy], center_position_x=[cx],
+                     center_position_z=[cz], x_region=(4.5, 4.5 + 1, 1.0), y_region=(0.0, 1.0, 1.0),
+                     region=(0.0, 1.0, 1.0))
+    # x = 4.5 -> u = 20.5 exactly: the texel's full weight; coeff = (S_y - D_y) / S_y = 2 at y = 0
+    assert abs(vol[0, 0, 0] - 2 * math.pi * 4) < 1e-9
+
+
+def test_planted_mistakes_are_caught():
+    """A wrong sign or a swapped operand in the BP formula is caught by the brute-force golden."""
+    sino = GOLDEN["bp_brute_sino"]
+    axis = float(GOLDEN["bp_brute_axis"])
+    ref = GOLDEN["bp_brute_out"]
+    good = oracle.bp_texture_f64(sino, axis_pos=axis)
+    assert np.abs(good - ref).max() < 1e-12
+    # planted on the CPU: sin and cos swapped (equivalently, transposed image axes),
+    # the +0.5 pixel centre dropped (axis shifted by half a pixel), the angle
+    # order reversed
+    assert np.abs(good.T - ref).max() > 1e-3
+    assert np.abs(oracle.bp_texture_f64(sino, axis_pos=axis + 0.5) - ref).max() > 1e-3
+    assert np.abs(oracle.bp_texture_f64(sino[::-1], axis_pos=axis) - ref).max() > 1e-3

@@ -1,0 +1,398 @@
+// Parallel-beam backprojection (`backproject` task) for gfx950.
+//
+// Texture mode (backproject.cl:50-94) is a gather: voxel (x, y) at angle p
+// reads the filtered sinogram row p at h = by sin_p + bx cos_p + axis with a
+// linear interpolation and a zero border.  Hardware texture filtering keeps
+// 8-bit weights, too coarse for 1e-5 parity, so interpolation is done in
+// float32 in registers from LDS:
+//
+//  * A 256-thread workgroup owns a 32x32 voxel tile of S slices that share the
+//    geometry (S = 1, 2 or 4).  Thread (lx, ly) owns voxels
+//    (lx + 16 kx, ly + 16 ky), kx, ky in {0, 1}; a wave covers 16x4 adjacent
+//    voxels, so its 64 LDS reads fall inside ~18 consecutive entries.
+//  * Angles are processed in chunks of A = 16.  For every angle of a chunk the
+//    detector span the tile can touch (<= 31 sqrt 2 + 3 < 64 entries) is
+//    staged into LDS as pairs (v_j, v_{j+1} - v_j) with the zero border applied,
+//    so one interpolation costs one ds_read_b64 (b128 for two slices) and a
+//    single fma: v + f * d.
+//  * The next chunk is prefetched into registers while the current one is
+//    consumed, so one LDS buffer suffices (2 barriers per chunk).
+//  * Per voxel and angle: u = bx cos + (by sin + a'), i = (int) u,
+//    f = fract(u): 4 VALU shared by the S slices, then 2 VALU per slice.
+//  * Workgroup ids are remapped so each XCD walks a contiguous range of
+//    (slice group, tile) work and keeps its sinograms in its own L2.
+#include "ufomi_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 32;       // voxels per tile side
+constexpr int kA = 16;          // angles per chunk
+constexpr int kStride = 64;     // LDS entries per staged angle
+constexpr float kPiF = 3.14159274101257324219f;  // M_PI_F
+
+template <int S> struct Entry;
+template <> struct Entry<1> { using T = float2; };
+template <> struct Entry<2> { using T = float4; };
+template <> struct Entry<4> { using T = float4; };  // two float4 planes
+
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    const int q = nb >> 3;
+    if (b < (q << 3))
+        return (b & 7) * q + (b >> 3);
+    return b;
+}
+
+// Per-angle staging values for this tile: (sin, cos, a' = axis - 0.5 - lo, lo).
+struct AngleSpan {
+    float s, c, a;
+    int lo;
+};
+
+__device__ __forceinline__ AngleSpan angle_span(const float2 *__restrict__ trig, int p, int P,
+                                                float bx0, float bx1, float by0, float by1, float am)
+{
+    AngleSpan r;
+    if (p < P) {
+        const float2 sc = trig[p];
+        r.s = sc.x;
+        r.c = sc.y;
+        const float umin = am + fminf(by0 * r.s, by1 * r.s) + fminf(bx0 * r.c, bx1 * r.c);
+        r.lo = (int) floorf(umin) - 1;
+        r.a = am - (float) r.lo;
+    } else {
+        r.s = r.c = r.a = 0.0f;
+        r.lo = 0;
+    }
+    return r;
+}
+
+template <int S, typename InT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void bp_texture_kernel(
+    const InT *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
+    float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
+    float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y)
+{
+    using E = typename Entry<S>::T;
+    constexpr int NPLANE = S == 4 ? 2 : 1;
+    constexpr int PASSES = kA * kStride / kThreads;  // staging passes per chunk (4)
+    __shared__ E lds[NPLANE][kA][kStride];
+    __shared__ float4 par[kA];
+
+    const int nb = tiles_x * tiles_y * ((n_slices + S - 1) / S);
+    const int b = xcd_remap(blockIdx.x, nb);
+    const int tiles = tiles_x * tiles_y;
+    const int g = b / tiles;
+    const int ty = (b % tiles) / tiles_x, tx = b % tiles_x;
+    const int x0 = tx * kTile, y0 = ty * kTile;
+    const int slice0 = g * S;
+
+    const int tid = threadIdx.x;
+    const int lx = tid & 15, ly = tid >> 4;
+
+    // Voxel-centre coordinates exactly as backproject.cl:63-64 writes them.
+    float bx[2], by[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        bx[k] = (float) (x0 + lx + 16 * k) - axis + roi_x + 0.5f;
+        by[k] = (float) (y0 + ly + 16 * k) - axis + roi_y + 0.5f;
+    }
+    // Tile corners for the span computation.
+    const float cbx0 = (float) x0 - axis + roi_x + 0.5f, cbx1 = (float) (x0 + kTile - 1) - axis + roi_x + 0.5f;
+    const float cby0 = (float) y0 - axis + roi_y + 0.5f, cby1 = (float) (y0 + kTile - 1) - axis + roi_y + 0.5f;
+    const float am = axis - 0.5f;
+
+    const InT *src[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int sl = slice0 + s < n_slices ? slice0 + s : n_slices - 1;
+        src[s] = sino + (long) sl * sino_pitch;
+    }
+
+    // Staging registers: this thread fills entry e of angles (tid >> 6) + 4 m.
+    const int e = tid & (kStride - 1);
+    const int arow = tid >> 6;
+    float sv[PASSES][S], sd[PASSES][S];
+    AngleSpan sp[PASSES];
+
+    auto fetch = [&](int p0) {
+#pragma unroll
+        for (int m = 0; m < PASSES; m++) {
+            const int pa = arow + 4 * m;
+            const int p = p0 + pa;
+            sp[m] = angle_span(trig, p, P, cbx0, cbx1, cby0, cby1, am);
+            const int j = sp[m].lo + e;
+            const bool in0 = p < P && j >= 0 && j < W;
+            const bool in1 = p < P && j + 1 >= 0 && j + 1 < W;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const InT *row = src[s] + (long) (p < P ? p : 0) * W;
+                const float v0 = in0 ? ufomi_load(row, j) : 0.0f;
+                const float v1 = in1 ? ufomi_load(row, j + 1) : 0.0f;
+                sv[m][s] = v0;
+                sd[m][s] = v1 - v0;
+            }
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int m = 0; m < PASSES; m++) {
+            const int pa = arow + 4 * m;
+            if constexpr (S == 1) {
+                lds[0][pa][e] = make_float2(sv[m][0], sd[m][0]);
+            } else if constexpr (S == 2) {
+                lds[0][pa][e] = make_float4(sv[m][0], sd[m][0], sv[m][1], sd[m][1]);
+            } else {
+                lds[0][pa][e] = make_float4(sv[m][0], sd[m][0], sv[m][1], sd[m][1]);
+                lds[1][pa][e] = make_float4(sv[m][2], sd[m][2], sv[m][3], sd[m][3]);
+            }
+            if (e == 0)
+                par[pa] = make_float4(sp[m].s, sp[m].c, sp[m].a, 0.0f);
+        }
+    };
+
+    float acc[2][2][S];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int s = 0; s < S; s++)
+                acc[i][k][s] = 0.0f;
+
+    const int nchunks = (P + kA - 1) / kA;
+    fetch(0);
+    commit();
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch++) {
+        if (ch + 1 < nchunks)
+            fetch((ch + 1) * kA);
+#pragma unroll 2
+        for (int pa = 0; pa < kA; pa++) {
+            const float4 pr = par[pa];
+            // Addresses and weights of the 4 voxels first, then all LDS reads,
+            // then the accumulation: keeps 4 (8) reads in flight per wave.
+            int iv[4];
+            float fv[4];
+#pragma unroll
+            for (int ky = 0; ky < 2; ky++) {
+                const float t = fmaf(by[ky], pr.x, pr.z);
+#pragma unroll
+                for (int kx = 0; kx < 2; kx++) {
+                    const float u = fmaf(bx[kx], pr.y, t);
+                    iv[2 * ky + kx] = (int) u;
+                    fv[2 * ky + kx] = __builtin_amdgcn_fractf(u);
+                }
+            }
+            E q[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                q[v] = lds[0][pa][iv[v]];
+            if constexpr (S == 1) {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[v >> 1][v & 1][0] += fmaf(fv[v], q[v].y, q[v].x);
+            } else {
+                float4 q2[4];
+                if constexpr (S == 4) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        q2[v] = lds[1][pa][iv[v]];
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    acc[v >> 1][v & 1][0] += fmaf(fv[v], q[v].y, q[v].x);
+                    acc[v >> 1][v & 1][1] += fmaf(fv[v], q[v].w, q[v].z);
+                    if constexpr (S == 4) {
+                        acc[v >> 1][v & 1][2] += fmaf(fv[v], q2[v].y, q2[v].x);
+                        acc[v >> 1][v & 1][3] += fmaf(fv[v], q2[v].w, q2[v].z);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (ch + 1 < nchunks) {
+            commit();
+            __syncthreads();
+        }
+    }
+
+    const float fp = (float) P;
+#pragma unroll
+    for (int ky = 0; ky < 2; ky++) {
+        const int y = y0 + ly + 16 * ky;
+#pragma unroll
+        for (int kx = 0; kx < 2; kx++) {
+            const int x = x0 + lx + 16 * kx;
+            if (x < roi_w && y < roi_h) {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    if (slice0 + s < n_slices) {
+                        const float r = acc[ky][kx][s] * kPiF;
+                        out[(long) (slice0 + s) * out_pitch + (long) y * roi_w + x] = r / fp;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// backproject_nearest (backproject.cl:24-48): flat index (int)(p * roi_w + h)
+// into the sinogram, with the reference's operation order; indices outside the
+// sinogram (undefined in the reference) read 0.
+__global__ __launch_bounds__(kThreads) void bp_nearest_kernel(
+    const float *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
+    float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
+    float *__restrict__ out, long out_pitch)
+{
+    // Every product rounds on its own, as written in the reference kernel: the
+    // truncated index is sensitive to the last bit of h.
+#pragma clang fp contract(off)
+    const int idx = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int idy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int sl = blockIdx.z;
+    if (idx >= roi_w || idy >= roi_h)
+        return;
+    const float *s = sino + (long) sl * sino_pitch;
+    const long total = (long) P * W;
+    const float bx = (float) idx - axis + roi_x + 0.5f;
+    const float by = (float) idy - axis + roi_y + 0.5f;
+    float sum = 0.0f;
+    for (int p = 0; p < P; p++) {
+        const float2 sc = trig[p];
+        float h = axis + bx * sc.y;
+        h = h + by * sc.x;
+        const float fi = (float) (p * roi_w) + h;
+        if (fi >= -2147483648.0f && fi < 2147483648.0f) {
+            const long i = (long) (int) fi;
+            if (i >= 0 && i < total)
+                sum += s[i];
+        }
+    }
+    const float r = sum * kPiF;
+    out[(long) sl * out_pitch + (long) idy * roi_w + idx] = r / (float) P;
+}
+
+// LUT of ufo-backproject-task.c:174-196: FP64 sin/cos of angle_offset + i*step,
+// rounded to float; entries offset..offset+P-1 are uploaded, as (sin, cos).
+int get_trig(ufomi_ctx *ctx, double angle_offset, double step, unsigned offset, int P, float2 **out)
+{
+    const bool cached = ctx->lut && ctx->lut_n == P && ctx->lut_offset == angle_offset &&
+                        ctx->lut_step == step && ctx->lut_first == offset;
+    if (!cached) {
+        std::vector<float2> h(P);
+        for (int i = 0; i < P; i++) {
+            const double a = angle_offset + (double) (offset + i) * step;
+            h[i] = make_float2((float) sin(a), (float) cos(a));
+        }
+        if (ctx->lut) {
+            UFOMI_HIP(hipStreamSynchronize(ctx->stream));
+            UFOMI_HIP(hipFree(ctx->lut));
+            ctx->lut = nullptr;
+        }
+        UFOMI_HIP(hipMalloc(&ctx->lut, (size_t) P * sizeof(float2)));
+        UFOMI_HIP(hipMemcpy(ctx->lut, h.data(), (size_t) P * sizeof(float2), hipMemcpyHostToDevice));
+        ctx->lut_n = P;
+        ctx->lut_offset = angle_offset;
+        ctx->lut_step = step;
+        ctx->lut_first = offset;
+    }
+    *out = ctx->lut;
+    return UFOMI_OK;
+}
+
+}  // namespace
+
+extern "C" void ufomi_bp_params_init(ufomi_bp_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->axis_pos = -1.0;
+    p->angle_step = -1.0;
+    p->angle_offset = 0.0;
+    p->mode = UFOMI_BP_TEXTURE;
+}
+
+extern "C" int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in,
+                                 int n_slices, size_t sino_pitch, const ufomi_bp_params *prm,
+                                 float *d_out, size_t out_pitch)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!d_sino || !d_out || !prm || W <= 0 || P_in <= 0 || n_slices <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid backprojection arguments");
+    if (dtype != UFOMI_F32 && dtype != UFOMI_F16 && dtype != UFOMI_BF16)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown sinogram dtype %d", dtype);
+
+    // ufo_backproject_task_get_requisition (ufo-backproject-task.c:212-267)
+    const unsigned n_proj = prm->num_projections ? prm->num_projections : (unsigned) P_in;
+    if ((unsigned) P_in > n_proj)
+        return ufomi_fail(UFOMI_ERR_PROJECTIONS,
+                          "Total number of projections (%u) must be greater than or equal to sinogram height (%u)",
+                          n_proj, (unsigned) P_in);
+    if ((unsigned long) prm->offset + (unsigned) P_in > n_proj)
+        return ufomi_fail(UFOMI_ERR_ARG, "offset %u + sinogram height %d exceeds the %u LUT entries",
+                          prm->offset, P_in, n_proj);
+    const int roi_w = prm->roi_width ? (int) prm->roi_width : W;
+    const int roi_h = prm->roi_height ? (int) prm->roi_height : W;
+    const double step = prm->angle_step <= 0.0 ? M_PI / (double) n_proj : prm->angle_step;
+    // ufo_backproject_task_process (:125-134): axis <= 0 means W / 2.
+    const float axis = prm->axis_pos <= 0.0 ? (float) W / 2.0f : (float) prm->axis_pos;
+    if (!std::isfinite(axis) || !std::isfinite(step) || !std::isfinite(prm->angle_offset))
+        return ufomi_fail(UFOMI_ERR_ARG, "non-finite geometry");
+    // Coordinates stay below 2^20 so float32 rounding of u stays far inside the
+    // one-entry margin of the staged detector span.
+    const double lim = (double) (1 << 20);
+    if ((double) roi_w + prm->roi_x > lim || (double) roi_h + prm->roi_y > lim || W > (1 << 20) ||
+        fabs((double) axis) > lim)
+        return ufomi_fail(UFOMI_ERR_ARG, "region of interest or axis beyond 2^20 pixels");
+    if (!sino_pitch) sino_pitch = (size_t) P_in * W;
+    if (!out_pitch) out_pitch = (size_t) roi_w * roi_h;
+
+    float2 *trig = nullptr;
+    if ((rc = get_trig(ctx, prm->angle_offset, step, prm->offset, P_in, &trig))) return rc;
+
+    if (prm->mode == UFOMI_BP_NEAREST) {
+        if (dtype != UFOMI_F32)
+            return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "nearest mode needs a float32 sinogram");
+        dim3 grid((roi_w + 15) / 16, (roi_h + 15) / 16, n_slices);
+        hipLaunchKernelGGL(bp_nearest_kernel, grid, dim3(kThreads), 0, ctx->stream,
+                           (const float *) d_sino, (long) sino_pitch, W, P_in, trig, axis,
+                           (float) prm->roi_x, (float) prm->roi_y, roi_w, roi_h, n_slices, d_out, (long) out_pitch);
+        UFOMI_CHECK_LAUNCH();
+        return UFOMI_OK;
+    }
+    if (prm->mode != UFOMI_BP_TEXTURE)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown backprojection mode %d", prm->mode);
+
+    const int S = n_slices >= 4 ? 4 : (n_slices >= 2 ? 2 : 1);
+    const int tiles_x = (roi_w + kTile - 1) / kTile, tiles_y = (roi_h + kTile - 1) / kTile;
+    const long nb = (long) tiles_x * tiles_y * ((n_slices + S - 1) / S);
+    if (nb > 0x7fffffffL)
+        return ufomi_fail(UFOMI_ERR_ARG, "too many tiles");
+    auto go = [&](auto s_const, auto *in) {
+        constexpr int SS = decltype(s_const)::value;
+        using InT = std::remove_const_t<std::remove_pointer_t<decltype(in)>>;
+        hipLaunchKernelGGL((bp_texture_kernel<SS, InT>), dim3((unsigned) nb), dim3(kThreads), 0, ctx->stream,
+                           in, (long) sino_pitch, W, P_in, trig, axis, (float) prm->roi_x, (float) prm->roi_y,
+                           roi_w, roi_h, n_slices, d_out, (long) out_pitch, tiles_x, tiles_y);
+        UFOMI_CHECK_LAUNCH();
+        return (int) UFOMI_OK;
+    };
+    auto by_s = [&](auto *in) {
+        switch (S) {
+        case 4: return go(std::integral_constant<int, 4>(), in);
+        case 2: return go(std::integral_constant<int, 2>(), in);
+        default: return go(std::integral_constant<int, 1>(), in);
+        }
+    };
+    switch (dtype) {
+    case UFOMI_F16: return by_s((const __half *) d_sino);
+    case UFOMI_BF16: return by_s((const __hip_bfloat16 *) d_sino);
+    default: return by_s((const float *) d_sino);
+    }
+}

@@ -1,0 +1,364 @@
+// Ramp filtering: `fft ! filter ! ifft` as LDS-resident batched FFTs.
+//
+// Every kernel runs 256-thread workgroups.  A transform of N points is done by
+// TF = min(256, N/4) threads with a radix-4 Stockham autosort FFT (one radix-2
+// pass first when log2 N is odd) entirely in LDS; G = 256 / TF transforms
+// share a workgroup when N < 1024.  Each thread keeps its B = N / (4 TF)
+// butterflies in registers between the read and write halves of a pass, so a
+// single LDS buffer of N complex values per transform suffices.
+// Twiddles come from the context's FP64-accurate table (read through L2).
+#include "ufomi_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// Multiply by -i (DIR = -1) or +i (DIR = +1).
+template <int DIR>
+__device__ __forceinline__ float2 rot(float2 a)
+{
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// N-point FFT in LDS (x[0..N)), exp(DIR * 2 pi i k n / N), cooperative over the
+// TF threads with local index t.  Caller synchronises before (data ready); the
+// function ends with a barrier (result ready).
+template <int B, int DIR>
+__device__ void lds_fft(float2 *x, int N, int log2N, int t, int TF, const float2 *__restrict__ tw)
+{
+    const int M = UFOMI_FFT_MAX;
+    int Ns = 1;
+    if (log2N & 1) {
+        // radix-2 pass at Ns = 1: no twiddles; y[2j] = a + b, y[2j+1] = a - b.
+        float2 a[2 * B], b[2 * B];
+#pragma unroll
+        for (int q = 0; q < 2 * B; q++) {
+            const int j = t + q * TF;
+            a[q] = x[j];
+            b[q] = x[j + N / 2];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2 * B; q++) {
+            const int j = t + q * TF;
+            x[2 * j] = cadd(a[q], b[q]);
+            x[2 * j + 1] = csub(a[q], b[q]);
+        }
+        __syncthreads();
+        Ns = 2;
+    }
+    for (; Ns < N; Ns *= 4) {
+        float2 v[B][4];
+        const int quarter = N / 4;
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int j = t + q * TF;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                v[q][r] = x[j + r * quarter];
+        }
+        __syncthreads();
+        const int tstep = M / (Ns * 4);  // table stride of one unit of angle 2 pi / (4 Ns)
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int j = t + q * TF;
+            const int k = j & (Ns - 1);
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < 4; r++) {
+                    float2 w = tw[k * r * tstep];
+                    if (DIR > 0)
+                        w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            // 4-point DFT with W4 = exp(DIR * 2 pi i / 4)
+            const float2 s02 = cadd(v[q][0], v[q][2]), d02 = csub(v[q][0], v[q][2]);
+            const float2 s13 = cadd(v[q][1], v[q][3]), d13 = rot<DIR>(csub(v[q][1], v[q][3]));
+            const int idx = (j - k) * 4 + k;
+            x[idx] = cadd(s02, s13);
+            x[idx + Ns] = cadd(d02, d13);
+            x[idx + 2 * Ns] = csub(s02, s13);
+            x[idx + 3 * Ns] = csub(d02, d13);
+        }
+        __syncthreads();
+    }
+}
+
+struct Layout {
+    int TF, G, f, t;  // threads per transform, transforms per WG, this thread's transform and lane
+};
+
+__device__ __forceinline__ Layout layout(int N)
+{
+    Layout L;
+    L.TF = N / 4 < kThreads ? N / 4 : kThreads;
+    L.G = kThreads / L.TF;
+    L.f = threadIdx.x / L.TF;
+    L.t = threadIdx.x % L.TF;
+    return L;
+}
+
+// Fused `fft ! filter ! ifft`.  The coefficient class comes from the device
+// flag written by classify_kernel in the same stream (no host round trip):
+//  * real, even spectrum (c[2k] == c[2k+1], c_k == c_{N-k}; every filter but
+//    faris-byer): rows 2m and 2m+1 ride in the real and imaginary parts of one
+//    complex transform.  A real even spectrum maps real rows to real rows, so
+//    the two results come back separated in Re and Im.  Blocks past the last
+//    pair exit at once (the grid is sized for one row per transform).
+//  * otherwise: one row per transform and Y_k = Re X_k c[2k] + i Im X_k c[2k+1],
+//    element-wise exactly as filter.cl does.
+template <int B, typename OutT>
+__global__ __launch_bounds__(kThreads) void ramp_kernel(
+    const float *__restrict__ in, int W, long rows, int N, int log2N, const float *__restrict__ coef,
+    const int *__restrict__ pair_flag, int crop, OutT *__restrict__ out, float scale, const float2 *__restrict__ tw)
+{
+    extern __shared__ float2 lds[];
+    const Layout L = layout(N);
+    float2 *x = lds + (size_t) L.f * N;
+    const bool pair = *pair_flag != 0;
+    const long unit = (long) blockIdx.x * L.G + L.f;
+    const long r0 = pair ? 2 * unit : unit, r1 = r0 + 1;
+    if (pair && 2 * (long) blockIdx.x * L.G >= rows)
+        return;  // whole block beyond the last pair
+    const bool v0 = r0 < rows, v1 = pair && r1 < rows;
+    const float *a = in + r0 * W, *b = in + r1 * W;
+
+    for (int i = L.t; i < N; i += L.TF) {
+        float re = 0.0f, im = 0.0f;
+        if (i < W) {
+            if (v0) re = a[i];
+            if (v1) im = b[i];
+        }
+        x[i] = make_float2(re, im);
+    }
+    __syncthreads();
+    lds_fft<B, -1>(x, N, log2N, L.t, L.TF, tw);
+    for (int k = L.t; k < N; k += L.TF) {
+        const float2 z = x[k];
+        const float cr = coef[2 * k];
+        const float ci = pair ? cr : coef[2 * k + 1];
+        x[k] = make_float2(z.x * cr, z.y * ci);
+    }
+    __syncthreads();
+    lds_fft<B, +1>(x, N, log2N, L.t, L.TF, tw);
+    for (int i = L.t; i < crop; i += L.TF) {
+        const float2 z = x[i];
+        if (v0) ufomi_store(out, r0 * crop + i, z.x * scale);
+        if (v1) ufomi_store(out, r1 * crop + i, z.y * scale);
+    }
+}
+
+// flag = 1 when c[2k] == c[2k+1] and c[2k] == c[2(N-k)] for all k (one block).
+__global__ __launch_bounds__(kThreads) void classify_kernel(const float *__restrict__ c, int N, int *__restrict__ flag)
+{
+    int ok = 1;
+    for (int k = threadIdx.x; k < N; k += kThreads) {
+        const float re = c[2 * k];
+        if (re != c[2 * k + 1] || (k && re != c[2 * (N - k)]))
+            ok = 0;
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0)
+        *flag = ok;
+}
+
+// `fft`: real rows -> zero-padded, forward-transformed interleaved complex rows.
+template <int B>
+__global__ __launch_bounds__(kThreads) void fft_rows_kernel(
+    const float *__restrict__ in, int W, long rows, int N, int log2N, float2 *__restrict__ out,
+    const float2 *__restrict__ tw)
+{
+    extern __shared__ float2 lds[];
+    const Layout L = layout(N);
+    float2 *x = lds + (size_t) L.f * N;
+    const long r = (long) blockIdx.x * L.G + L.f;
+    const bool valid = r < rows;
+    const float *a = in + r * W;
+    for (int i = L.t; i < N; i += L.TF)
+        x[i] = make_float2((valid && i < W) ? a[i] : 0.0f, 0.0f);
+    __syncthreads();
+    lds_fft<B, -1>(x, N, log2N, L.t, L.TF, tw);
+    if (valid)
+        for (int i = L.t; i < N; i += L.TF)
+            out[r * N + i] = x[i];
+}
+
+// `ifft`: interleaved complex rows -> Re(backward transform)[0:crop] * scale.
+template <int B>
+__global__ __launch_bounds__(kThreads) void ifft_rows_kernel(
+    const float2 *__restrict__ in, long rows, int N, int log2N, int crop, float *__restrict__ out,
+    float scale, const float2 *__restrict__ tw)
+{
+    extern __shared__ float2 lds[];
+    const Layout L = layout(N);
+    float2 *x = lds + (size_t) L.f * N;
+    const long r = (long) blockIdx.x * L.G + L.f;
+    const bool valid = r < rows;
+    for (int i = L.t; i < N; i += L.TF)
+        x[i] = valid ? in[r * N + i] : make_float2(0.0f, 0.0f);
+    __syncthreads();
+    lds_fft<B, +1>(x, N, log2N, L.t, L.TF, tw);
+    if (valid)
+        for (int i = L.t; i < crop; i += L.TF)
+            out[r * crop + i] = x[i].x * scale;
+}
+
+// `filter`: element-wise product with the 2N coefficients, 4 floats per lane.
+__global__ __launch_bounds__(kThreads) void filter_apply_kernel(
+    const float *__restrict__ in, float *__restrict__ out, const float *__restrict__ coef, int width, long total)
+{
+    const long stride = (long) gridDim.x * kThreads;
+    for (long i = (long) blockIdx.x * kThreads + threadIdx.x; i < total; i += stride) {
+        const int c = (int) (i % width);
+        out[i] = in[i] * coef[c];
+    }
+}
+
+int fft_geometry(int N, int *B, int *G, size_t *lds_bytes)
+{
+    const int TF = N / 4 < kThreads ? N / 4 : kThreads;
+    *G = kThreads / TF;
+    *B = N / (4 * TF);
+    *lds_bytes = (size_t) (*G) * N * sizeof(float2);
+    return UFOMI_OK;
+}
+
+int check_n(int N)
+{
+    if (!ufomi_is_pow2(N) || N < 4 || N > UFOMI_FFT_MAX_N)
+        return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "FFT size %d must be a power of two in [4, %d]", N, UFOMI_FFT_MAX_N);
+    return UFOMI_OK;
+}
+
+template <typename K>
+int launch_by_b(int B, K k)
+{
+    switch (B) {
+    case 1: return k(std::integral_constant<int, 1>());
+    case 2: return k(std::integral_constant<int, 2>());
+    case 4: return k(std::integral_constant<int, 4>());
+    case 8: return k(std::integral_constant<int, 8>());
+    case 16: return k(std::integral_constant<int, 16>());
+    default: return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "no FFT kernel for %d butterflies per thread", B);
+    }
+}
+
+}  // namespace
+
+extern "C" int ufomi_fft_rows(ufomi_ctx *ctx, const float *d_in, int W, int P, int N, float *d_out)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_n(N))) return rc;
+    if (!d_in || !d_out || W <= 0 || P <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid fft arguments");
+    if (N < W)
+        return ufomi_fail(UFOMI_ERR_SIZE, "Specified size must be greater than or equal to input size");
+    int B, G;
+    size_t lds;
+    fft_geometry(N, &B, &G, &lds);
+    const unsigned blocks = (unsigned) ((P + G - 1) / G);
+    return launch_by_b(B, [&](auto b) {
+        constexpr int BB = decltype(b)::value;
+        hipLaunchKernelGGL(fft_rows_kernel<BB>, dim3(blocks), dim3(kThreads), lds, ctx->stream,
+                           d_in, W, (long) P, N, ufomi_log2(N), (float2 *) d_out, ctx->twiddles);
+        UFOMI_CHECK_LAUNCH();
+        return (int) UFOMI_OK;
+    });
+}
+
+extern "C" int ufomi_ifft_rows(ufomi_ctx *ctx, const float *d_in, int N, int P, int crop, float *d_out)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_n(N))) return rc;
+    if (!d_in || !d_out || P <= 0 || crop < 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid ifft arguments");
+    if (crop > N)
+        return ufomi_fail(UFOMI_ERR_SIZE, "Cropped size must be less than or equal to input size");
+    if (crop == 0) crop = N;
+    int B, G;
+    size_t lds;
+    fft_geometry(N, &B, &G, &lds);
+    const unsigned blocks = (unsigned) ((P + G - 1) / G);
+    const float scale = 1.0f / (float) N;  // fft_pack scale of ufo-ifft-task.c:296-303
+    return launch_by_b(B, [&](auto b) {
+        constexpr int BB = decltype(b)::value;
+        hipLaunchKernelGGL(ifft_rows_kernel<BB>, dim3(blocks), dim3(kThreads), lds, ctx->stream,
+                           (const float2 *) d_in, (long) P, N, ufomi_log2(N), crop, d_out, scale, ctx->twiddles);
+        UFOMI_CHECK_LAUNCH();
+        return (int) UFOMI_OK;
+    });
+}
+
+extern "C" int ufomi_filter_apply(ufomi_ctx *ctx, const float *d_in, float *d_out, const float *d_coef,
+                                  int width, int P)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!d_in || !d_out || !d_coef || P <= 0 || width <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid filter arguments");
+    if (!ufomi_is_pow2(width))
+        return ufomi_fail(UFOMI_ERR_NOT_POW2, "Invalid filter input width %d, must be a power of two", width);
+    const long total = (long) width * P;
+    long blocks = (total + kThreads - 1) / kThreads;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(filter_apply_kernel, dim3((unsigned) blocks), dim3(kThreads), 0, ctx->stream,
+                       d_in, d_out, d_coef, width, total);
+    UFOMI_CHECK_LAUNCH();
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_ramp_filter(ufomi_ctx *ctx, const float *d_in, int W, int P, int N,
+                                 const float *d_coef, int crop, void *d_out, int out_dtype)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_n(N))) return rc;
+    if (!d_in || !d_out || !d_coef || W <= 0 || P <= 0 || crop < 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid ramp filter arguments");
+    if (N < W)
+        return ufomi_fail(UFOMI_ERR_SIZE, "Specified size must be greater than or equal to input size");
+    if (crop > N)
+        return ufomi_fail(UFOMI_ERR_SIZE, "Cropped size must be less than or equal to input size");
+    if (crop == 0) crop = N;
+    if (out_dtype != UFOMI_F32 && out_dtype != UFOMI_F16 && out_dtype != UFOMI_BF16)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown output dtype %d", out_dtype);
+
+    // Classify the coefficients on the device, in stream order.
+    int *d_flag = ctx->flags;
+    hipLaunchKernelGGL(classify_kernel, dim3(1), dim3(kThreads), 0, ctx->stream, d_coef, N, d_flag);
+    UFOMI_CHECK_LAUNCH();
+
+    int B, G;
+    size_t lds;
+    fft_geometry(N, &B, &G, &lds);
+    const unsigned blocks = (unsigned) (((long) P + G - 1) / G);
+    const float scale = 1.0f / (float) N;
+    const int log2N = ufomi_log2(N);
+    auto go = [&](auto b, auto *out) {
+        constexpr int BB = decltype(b)::value;
+        using OutT = std::remove_pointer_t<decltype(out)>;
+        hipLaunchKernelGGL((ramp_kernel<BB, OutT>), dim3(blocks), dim3(kThreads), lds, ctx->stream,
+                           d_in, W, (long) P, N, log2N, d_coef, d_flag, crop, out, scale, ctx->twiddles);
+        UFOMI_CHECK_LAUNCH();
+        return (int) UFOMI_OK;
+    };
+    return launch_by_b(B, [&](auto b) {
+        switch (out_dtype) {
+        case UFOMI_F16: return go(b, (__half *) d_out);
+        case UFOMI_BF16: return go(b, (__hip_bfloat16 *) d_out);
+        default: return go(b, (float *) d_out);
+        }
+    });
+}
+

@@ -1,0 +1,400 @@
+// General backprojection (`general-backproject` task, parameter z) for gfx950.
+//
+// The reference generates an OpenCL kernel per geometry (make_kernel,
+// ufo-general-backproject-task.c:1069-1180) that walks, per voxel and
+// projection, the transformation chain of make_static_transformations and
+// make_transformations (:807-1048): magnification, volume rotation,
+// tomographic rotation, axis tilt, projection from the source onto the
+// (possibly tilted, shifted) detector, then the detector frame.  Every step is
+// affine in the voxel except the division by the projection depth, so per
+// projection the chain is a projective map
+//     u = U.X / D.X,  v = V.X / D.X,  coeff = 1 / C.X,   X = (x, y, z, 1)
+// with U, V, D, C four affine rows.  The host evaluates the chain in FP64 at
+// four points to obtain them exactly (up to FP64 rounding) and rounds once to
+// float; the kernel then needs 12 (parallel beam) or 16 (cone) FMAs, one
+// reciprocal and one bilinear fetch per voxel and projection, and keeps the
+// whole projection sum of a voxel in a register (the reference re-reads and
+// re-writes the volume every `burst` projections).
+#include "ufomi_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct Geo {
+    double tomo, ax, ay, vx, vy, vz, dax, day, daz, cx, cz, sx, sy, sz, dx, dy, dz;
+};
+
+struct Flags {
+    bool with_axis, with_volume, perpendicular, shifted_detector, shifted_source, parallel;
+};
+
+struct V3 {
+    double x, y, z;
+};
+
+// general_bp_definitions.in:1-3 with trig = (sin, cos)
+V3 rot_x(double s, double c, V3 p) { return {p.x, c * p.y - s * p.z, s * p.y + c * p.z}; }
+V3 rot_y(double s, double c, V3 p) { return {c * p.x + s * p.z, p.y, -s * p.x + c * p.z}; }
+V3 rot_z(double s, double c, V3 p) { return {c * p.x - s * p.y, s * p.x + c * p.y, p.z}; }
+
+// Detector coordinates (u, v), projection depth d (u = U/d etc. with U, V affine)
+// and 1/coeff of one voxel, FP64, following the generated kernel's statement order.
+void chain(const Geo &g, const Flags &f, V3 X, double out[4])
+{
+    V3 v0 = X, n = {0.0, -1.0, 0.0};
+    double det_off = 0.0, project_tmp = 0.0, tmp_tr = 0.0;
+    if (!f.parallel) {
+        const double m = -g.sy / (g.dy - g.sy);
+        v0 = {v0.x * m, v0.y * m, v0.z * m};
+    }
+    if (!f.perpendicular) {
+        n = rot_z(sin(g.daz), cos(g.daz), n);
+        n = rot_y(sin(g.day), cos(g.day), n);
+        n = rot_x(sin(g.dax), cos(g.dax), n);
+        det_off = -(g.dx * n.x + g.dy * n.y + g.dz * n.z);
+    } else if (!f.parallel) {
+        project_tmp = g.dy - g.sy;
+    }
+    if (f.with_volume) {
+        v0 = rot_z(sin(g.vz), cos(g.vz), v0);
+        v0 = rot_y(sin(g.vy), cos(g.vy), v0);
+        v0 = rot_x(sin(g.vx), cos(g.vx), v0);
+    }
+    if (!(f.perpendicular || f.parallel))
+        tmp_tr = -(det_off + (g.sx * n.x + g.sy * n.y + g.sz * n.z));
+
+    V3 v = rot_z(sin(g.tomo), cos(g.tomo), v0);
+    if (f.with_axis) {
+        v = rot_y(sin(g.ay), cos(g.ay), v);
+        v = rot_x(sin(g.ax), cos(g.ax), v);
+    }
+    double inv_coeff = 1.0, depth = 1.0;  // depth = 1 / t of the projection step
+    if (!f.parallel)
+        inv_coeff = (g.sy - v.y) / (g.sy - g.dy);
+    if (f.perpendicular) {
+        if (!f.parallel) {
+            depth = (v.y - g.sy) / project_tmp;
+            const double t = 1.0 / depth;
+            if (f.shifted_source)
+                v = {t * (v.x - g.sx) + g.sx, t * (v.y - g.sy) + g.sy, t * (v.z - g.sz) + g.sz};
+            else
+                v = {t * v.x + g.sx, t * v.y + g.sy, t * v.z + g.sz};
+        }
+    } else {
+        if (f.parallel) {
+            v.y = -(v.z * n.z + (v.x * n.x + det_off)) / n.y;
+        } else {
+            v = {v.x - g.sx, v.y - g.sy, v.z - g.sz};
+            depth = (v.x * n.x + v.y * n.y + v.z * n.z) / tmp_tr;
+            const double t = 1.0 / depth;
+            v = {t * v.x + g.sx, t * v.y + g.sy, t * v.z + g.sz};
+        }
+    }
+    if (!f.perpendicular || f.shifted_detector) {
+        v = {v.x - g.dx, v.y - g.dy, v.z - g.dz};
+        if (!f.perpendicular) {
+            v = rot_x(-sin(g.dax), cos(g.dax), v);
+            v = rot_y(-sin(g.day), cos(g.day), v);
+            v = rot_z(-sin(g.daz), cos(g.daz), v);
+        }
+    }
+    out[0] = (v.x + g.cx) * depth;
+    out[1] = (v.z + g.cz) * depth;
+    out[2] = depth;
+    out[3] = inv_coeff;
+}
+
+// Rows of the affine maps (U, V, D, C) from four evaluations of the chain.
+void compose(const Geo &g, const Flags &f, float m[16])
+{
+    double o[4], ex[4], ey[4], ez[4];
+    chain(g, f, {0, 0, 0}, o);
+    chain(g, f, {1, 0, 0}, ex);
+    chain(g, f, {0, 1, 0}, ey);
+    chain(g, f, {0, 0, 1}, ez);
+    for (int r = 0; r < 4; r++) {
+        m[4 * r + 0] = (float) (ex[r] - o[r]);
+        m[4 * r + 1] = (float) (ey[r] - o[r]);
+        m[4 * r + 2] = (float) (ez[r] - o[r]);
+        m[4 * r + 3] = (float) o[r];
+    }
+}
+
+template <typename T> struct StoreOp;
+
+template <bool CONE, int ADDR, typename ST>
+__global__ __launch_bounds__(256) void gbp_kernel(
+    const float *__restrict__ proj, int W, int H, int P, const float4 *__restrict__ mats, float x0, float xs,
+    float y0, float ys, const float *__restrict__ zs, int nx, int ny, int nz, float norm, float gmin, float gscale,
+    float gmax, ST *__restrict__ vol)
+{
+    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int iz = blockIdx.z;
+    if (ix >= nx || iy >= ny)
+        return;
+    const float x = fmaf((float) ix, xs, x0), y = fmaf((float) iy, ys, y0), z = zs[iz];
+    float result = 0.0f;
+    const long plane = (long) W * H;
+    for (int p = 0; p < P; p++) {
+        const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1];
+        float u = fmaf(mu.x, x, fmaf(mu.y, y, fmaf(mu.z, z, mu.w)));
+        float v = fmaf(mv.x, x, fmaf(mv.y, y, fmaf(mv.z, z, mv.w)));
+        float w = 1.0f;
+        if constexpr (CONE) {
+            const float4 md = mats[4 * p + 2], mc = mats[4 * p + 3];
+            const float d = fmaf(md.x, x, fmaf(md.y, y, fmaf(md.z, z, md.w)));
+            const float c = fmaf(mc.x, x, fmaf(mc.y, y, fmaf(mc.z, z, mc.w)));
+            const float rd = __builtin_amdgcn_rcpf(d);
+            u *= rd;
+            v *= rd;
+            const float coeff = __builtin_amdgcn_rcpf(c);
+            w = coeff * coeff;
+        }
+        // read_imagef, unnormalised coordinates, linear filter
+        const float fu = u - 0.5f, fv = v - 0.5f;
+        const float flu = floorf(fu), flv = floorf(fv);
+        const float a = fu - flu, b = fv - flv;
+        int i0 = (int) flu, j0 = (int) flv;
+        int i1 = i0 + 1, j1 = j0 + 1;
+        float t00, t10, t01, t11;
+        const float *img = proj + (long) p * plane;
+        if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
+            const bool ci0 = i0 >= 0 && i0 < W, ci1 = i1 >= 0 && i1 < W;
+            const bool cj0 = j0 >= 0 && j0 < H, cj1 = j1 >= 0 && j1 < H;
+            t00 = (ci0 && cj0) ? img[(long) j0 * W + i0] : 0.0f;
+            t10 = (ci1 && cj0) ? img[(long) j0 * W + i1] : 0.0f;
+            t01 = (ci0 && cj1) ? img[(long) j1 * W + i0] : 0.0f;
+            t11 = (ci1 && cj1) ? img[(long) j1 * W + i1] : 0.0f;
+        } else {
+            i0 = min(max(i0, 0), W - 1);
+            i1 = min(max(i1, 0), W - 1);
+            j0 = min(max(j0, 0), H - 1);
+            j1 = min(max(j1, 0), H - 1);
+            t00 = img[(long) j0 * W + i0];
+            t10 = img[(long) j0 * W + i1];
+            t01 = img[(long) j1 * W + i0];
+            t11 = img[(long) j1 * W + i1];
+        }
+        const float r0 = fmaf(a, t10 - t00, t00);
+        const float r1 = fmaf(a, t11 - t01, t01);
+        const float val = fmaf(b, r1 - r0, r0);
+        result = fmaf(val, w, result);
+    }
+    const float r = norm * result;
+    const long o = ((long) iz * ny + iy) * nx + ix;
+    if constexpr (std::is_same<ST, float>::value) {
+        vol[o] = r;
+    } else if constexpr (std::is_same<ST, double>::value) {
+        vol[o] = (double) r;
+    } else if constexpr (std::is_same<ST, __half>::value) {
+        vol[o] = __float2half_rn(r);
+    } else {
+        // integer stores: clamp(gray_limit.y * (norm * result - gray_limit.x), 0, max), truncated
+        float g = gscale * (r - gmin);
+        g = fminf(fmaxf(g, 0.0f), gmax);
+        if constexpr (std::is_same<ST, uint32_t>::value)
+            vol[o] = g >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t) g;
+        else
+            vol[o] = (ST) g;
+    }
+}
+
+double at(const double *a, int n, int i, double dflt) { return (!a || n <= 0) ? dflt : (n == 1 ? a[0] : a[i]); }
+
+bool almost_zero(const double *a, int n)
+{
+    if (!a || n <= 0)
+        return true;
+    for (int i = 0; i < n; i++)
+        if (!(fabs(a[i]) < 1e-7))
+            return false;
+    return true;
+}
+
+int check_len(const char *name, int n, unsigned P)
+{
+    if (n > 1 && (unsigned) n < P)
+        return ufomi_fail(UFOMI_ERR_ARG, "%s has %d values, needs 1 or %u", name, n, P);
+    return UFOMI_OK;
+}
+
+}  // namespace
+
+extern "C" void ufomi_gbp_params_init(ufomi_gbp_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->overall_angle = 2.0 * M_PI;
+    p->store_type = UFOMI_ST_FLOAT;
+    p->addressing_mode = UFOMI_ADDR_CLAMP;
+}
+
+extern "C" void ufomi_gbp_geometry_init(ufomi_gbp_geometry *g) { memset(g, 0, sizeof(*g)); }
+
+extern "C" int ufomi_gbp_volume_shape(const ufomi_gbp_params *p, int W, int *nx, int *ny, int *nz)
+{
+    if (!p || W <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid volume shape arguments");
+    // ufo-general-backproject-task.c:1494-1507 (REGION_SIZE) and :1516-1528
+    *nx = p->x_region[2] == 0.0 ? W : (int) ceil((p->x_region[1] - p->x_region[0]) / p->x_region[2]);
+    *ny = p->y_region[2] == 0.0 ? W : (int) ceil((p->y_region[1] - p->y_region[0]) / p->y_region[2]);
+    if (fabs(p->region[2]) < 1e-7)
+        *nz = 1;
+    else
+        *nz = (int) ceil((p->region[1] - p->region[0]) / p->region[2]);
+    if (*nx <= 0 || *ny <= 0 || *nz <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "empty region");
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_gbp(ufomi_ctx *ctx, const float *d_proj, int W, int H, int P, const ufomi_gbp_geometry *g,
+                         const ufomi_gbp_params *prm, void *d_vol)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!d_proj || !d_vol || !g || !prm || W <= 0 || H <= 0 || P <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid general backprojection arguments");
+    const unsigned n = prm->num_projections;
+    if (!n)
+        return ufomi_fail(UFOMI_ERR_NUM_PROJECTIONS, "Number of projections not set");
+    if ((unsigned) P < n)
+        return ufomi_fail(UFOMI_ERR_INCOMPLETE,
+                          "general-backproject received only %d projections out of %u specified", P, n);
+    if ((unsigned) P > n)
+        return ufomi_fail(UFOMI_ERR_ARG, "%d projections given but num-projections is %u", P, n);
+    const int st = prm->store_type;
+    if (st < UFOMI_ST_HALF || st > UFOMI_ST_UINT)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown store type %d", st);
+    const bool integer_store = st == UFOMI_ST_UCHAR || st == UFOMI_ST_USHORT || st == UFOMI_ST_UINT;
+    if (integer_store && prm->gray_map_min >= prm->gray_map_max)
+        return ufomi_fail(UFOMI_ERR_GRAY_MAP, "Gray mapping minimum must be less then the maximum");
+    int addr = prm->addressing_mode;
+    if (addr == UFOMI_ADDR_REPEAT || addr == UFOMI_ADDR_MIRRORED_REPEAT)
+        return ufomi_fail(UFOMI_ERR_UNSUPPORTED,
+                          "repeat addressing needs normalized coordinates; unnormalized sampling is undefined");
+    if (addr != UFOMI_ADDR_CLAMP && addr != UFOMI_ADDR_CLAMP_TO_EDGE && addr != UFOMI_ADDR_NONE)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown addressing mode %d", addr);
+    // "none" leaves out-of-range reads undefined in OpenCL; clamp to the edge here.
+    const int kaddr = addr == UFOMI_ADDR_CLAMP ? UFOMI_ADDR_CLAMP : UFOMI_ADDR_CLAMP_TO_EDGE;
+
+    struct { const char *name; int n; } lens[] = {
+        {"axis-angle-x", g->n_axis_angle_x}, {"axis-angle-y", g->n_axis_angle_y},
+        {"volume-angle-x", g->n_volume_angle_x}, {"volume-angle-y", g->n_volume_angle_y},
+        {"volume-angle-z", g->n_volume_angle_z}, {"detector-angle-x", g->n_detector_angle_x},
+        {"detector-angle-y", g->n_detector_angle_y}, {"detector-angle-z", g->n_detector_angle_z},
+        {"center-position-x", g->n_center_position_x}, {"center-position-z", g->n_center_position_z},
+        {"source-position-x", g->n_source_position_x}, {"source-position-y", g->n_source_position_y},
+        {"source-position-z", g->n_source_position_z}, {"detector-position-x", g->n_detector_position_x},
+        {"detector-position-y", g->n_detector_position_y}, {"detector-position-z", g->n_detector_position_z}};
+    for (auto &l : lens)
+        if ((rc = check_len(l.name, l.n, n))) return rc;
+
+    // Flags of node_setup (ufo-general-backproject-task.c:1305-1336).
+    Flags f;
+    f.with_axis = !(almost_zero(g->axis_angle_x, g->n_axis_angle_x) && almost_zero(g->axis_angle_y, g->n_axis_angle_y));
+    f.with_volume = !(almost_zero(g->volume_angle_x, g->n_volume_angle_x) &&
+                      almost_zero(g->volume_angle_y, g->n_volume_angle_y) &&
+                      almost_zero(g->volume_angle_z, g->n_volume_angle_z));
+    f.shifted_detector = !(almost_zero(g->detector_position_x, g->n_detector_position_x) &&
+                           almost_zero(g->detector_position_z, g->n_detector_position_z));
+    f.shifted_source = !(almost_zero(g->source_position_x, g->n_source_position_x) &&
+                         almost_zero(g->source_position_z, g->n_source_position_z));
+    f.perpendicular = almost_zero(g->detector_angle_x, g->n_detector_angle_x) &&
+                      almost_zero(g->detector_angle_y, g->n_detector_angle_y) &&
+                      almost_zero(g->detector_angle_z, g->n_detector_angle_z);
+    f.parallel = true;
+    for (unsigned i = 0; i < n; i++)
+        f.parallel = f.parallel && std::isinf(at(g->source_position_y, g->n_source_position_y, i, -INFINITY));
+
+    // Tomographic angles: axis-angle-z if it has n values, else i / n * overall (:1435-1446).
+    const bool own_tomo = g->axis_angle_z && g->n_axis_angle_z == (int) n;
+    std::vector<float> mats(16 * (size_t) n);
+    for (unsigned i = 0; i < n; i++) {
+        Geo q;
+        q.tomo = own_tomo ? g->axis_angle_z[i] : ((double) i) / n * prm->overall_angle;
+        q.ax = at(g->axis_angle_x, g->n_axis_angle_x, i, 0.0);
+        q.ay = at(g->axis_angle_y, g->n_axis_angle_y, i, 0.0);
+        q.vx = at(g->volume_angle_x, g->n_volume_angle_x, i, 0.0);
+        q.vy = at(g->volume_angle_y, g->n_volume_angle_y, i, 0.0);
+        q.vz = at(g->volume_angle_z, g->n_volume_angle_z, i, 0.0);
+        q.dax = at(g->detector_angle_x, g->n_detector_angle_x, i, 0.0);
+        q.day = at(g->detector_angle_y, g->n_detector_angle_y, i, 0.0);
+        q.daz = at(g->detector_angle_z, g->n_detector_angle_z, i, 0.0);
+        q.cx = at(g->center_position_x, g->n_center_position_x, i, 0.0);
+        q.cz = at(g->center_position_z, g->n_center_position_z, i, 0.0);
+        q.sx = at(g->source_position_x, g->n_source_position_x, i, 0.0);
+        q.sy = at(g->source_position_y, g->n_source_position_y, i, -INFINITY);
+        q.sz = at(g->source_position_z, g->n_source_position_z, i, 0.0);
+        q.dx = at(g->detector_position_x, g->n_detector_position_x, i, 0.0);
+        q.dy = at(g->detector_position_y, g->n_detector_position_y, i, 0.0);
+        q.dz = at(g->detector_position_z, g->n_detector_position_z, i, 0.0);
+        if (!f.parallel && !(std::isfinite(q.sy) && std::isfinite(q.dy) && q.dy != q.sy))
+            return ufomi_fail(UFOMI_ERR_ARG, "cone beam needs finite source and detector y positions");
+        compose(q, f, &mats[16 * (size_t) i]);
+    }
+
+    int nx, ny, nz;
+    if ((rc = ufomi_gbp_volume_shape(prm, W, &nx, &ny, &nz))) return rc;
+    if (nz > 65535)
+        return ufomi_fail(UFOMI_ERR_ARG, "at most 65535 slices per call");
+    // compute_slice_region (:246-259): step 0 -> (-(float) length / 2, 1)
+    const float x0 = prm->x_region[2] ? (float) prm->x_region[0] : -((float) nx / 2);
+    const float xs = prm->x_region[2] ? (float) prm->x_region[2] : 1.0f;
+    const float y0 = prm->y_region[2] ? (float) prm->y_region[0] : -((float) ny / 2);
+    const float ys = prm->y_region[2] ? (float) prm->y_region[2] : 1.0f;
+    // create_regions (:38-76): z_k = start + k * step in FP64, rounded.
+    double zstart = 0.0, zstep = 1.0;
+    if (!(fabs(prm->region[2]) < 1e-7)) {
+        zstart = prm->region[0];
+        zstep = prm->region[2];
+    }
+    std::vector<float> zs(nz);
+    for (int k = 0; k < nz; k++)
+        zs[k] = (float) (zstart + k * zstep);
+
+    const float norm = (float) (fabs(prm->overall_angle) / n);
+    double maxval = st == UFOMI_ST_UCHAR ? 255.0 : st == UFOMI_ST_USHORT ? 65535.0 : 4294967295.0;
+    const float gmin = (float) prm->gray_map_min;
+    const float gscale = integer_store ? (float) (maxval / (prm->gray_map_max - prm->gray_map_min)) : 0.0f;
+    const float gmax = (float) maxval;
+
+    const size_t mbytes = mats.size() * sizeof(float), zbytes = zs.size() * sizeof(float);
+    char *scratch = nullptr;
+    if ((rc = ufomi_scratch(ctx, mbytes + zbytes + 256, (void **) &scratch))) return rc;
+    std::vector<char> host(mbytes + zbytes + 256);
+    memcpy(host.data(), mats.data(), mbytes);
+    const size_t zoff = (mbytes + 255) & ~(size_t) 255;
+    memcpy(host.data() + zoff, zs.data(), zbytes);
+    if ((rc = ufomi_upload(ctx, scratch, host.data(), zoff + zbytes))) return rc;
+
+    dim3 grid((nx + 15) / 16, (ny + 15) / 16, nz);
+    auto go = [&](auto cone, auto addr_c, auto *vol) {
+        constexpr bool C = decltype(cone)::value;
+        constexpr int A = decltype(addr_c)::value;
+        using ST = std::remove_pointer_t<decltype(vol)>;
+        hipLaunchKernelGGL((gbp_kernel<C, A, ST>), grid, dim3(256), 0, ctx->stream, d_proj, W, H, (int) n,
+                           (const float4 *) scratch, x0, xs, y0, ys, (const float *) (scratch + zoff), nx, ny, nz,
+                           norm, gmin, gscale, gmax, vol);
+        UFOMI_CHECK_LAUNCH();
+        return (int) UFOMI_OK;
+    };
+    auto by_store = [&](auto cone, auto addr_c) {
+        switch (st) {
+        case UFOMI_ST_HALF: return go(cone, addr_c, (__half *) d_vol);
+        case UFOMI_ST_DOUBLE: return go(cone, addr_c, (double *) d_vol);
+        case UFOMI_ST_UCHAR: return go(cone, addr_c, (uint8_t *) d_vol);
+        case UFOMI_ST_USHORT: return go(cone, addr_c, (uint16_t *) d_vol);
+        case UFOMI_ST_UINT: return go(cone, addr_c, (uint32_t *) d_vol);
+        default: return go(cone, addr_c, (float *) d_vol);
+        }
+    };
+    auto by_addr = [&](auto cone) {
+        if (kaddr == UFOMI_ADDR_CLAMP)
+            return by_store(cone, std::integral_constant<int, UFOMI_ADDR_CLAMP>());
+        return by_store(cone, std::integral_constant<int, UFOMI_ADDR_CLAMP_TO_EDGE>());
+    };
+    return f.parallel ? by_addr(std::false_type()) : by_addr(std::true_type());
+}

@@ -1,0 +1,247 @@
+// Seeded ellipse / ellipsoid phantoms and their analytic parallel-beam
+// sinograms, generated on the GPU in FP64 for benchmark-sized inputs.  Not
+// part of the reference; it produces the synthetic data of the benchmark
+// configurations.
+//
+// Geometry matches the backprojector: a detector pixel j samples the line
+// x cos(theta) + y sin(theta) = t_j with t_j = j + 0.5 - axis, theta_p =
+// p * angle_step, pixel units relative to the rotation axis.
+#include "ufomi_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// splitmix64 -> xoshiro256** (Blackman & Vigna), uniform doubles in [0, 1).
+struct Rng {
+    uint64_t s[4];
+    static uint64_t splitmix(uint64_t &x)
+    {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    explicit Rng(uint64_t seed)
+    {
+        for (int i = 0; i < 4; i++)
+            s[i] = splitmix(seed);
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next()
+    {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double) (next() >> 11) * 0x1.0p-53; }
+    double uniform(double a, double b) { return a + (b - a) * uniform(); }
+};
+
+// Line integral of one ellipse (x0, y0, a, b, cos phi, sin phi, density) along
+// x cos(theta) + y sin(theta) = t: 2 rho a b sqrt(a'^2 - t'^2) / a'^2 with
+// a'^2 = a^2 cos^2(theta - phi) + b^2 sin^2(theta - phi) and t' the offset of
+// the line from the centre.
+__device__ double ellipse_line_integral(const double *e, double c, double s, double t)
+{
+    const double tp = t - (e[0] * c + e[1] * s);
+    const double ct = c * e[4] + s * e[5], st = s * e[4] - c * e[5];  // angle theta - phi
+    const double a2 = e[2] * e[2] * ct * ct + e[3] * e[3] * st * st;
+    const double d = a2 - tp * tp;
+    return d > 0.0 ? 2.0 * e[6] * e[2] * e[3] * sqrt(d) / a2 : 0.0;
+}
+
+constexpr int kMaxEll = 64;
+
+__global__ __launch_bounds__(256) void sinogram_kernel(const double *__restrict__ ell, int n_ell, int W, int P,
+                                                       double axis, double step, float *__restrict__ out)
+{
+    __shared__ double e[kMaxEll * 7];
+    const long sl = blockIdx.z;
+    for (int k = threadIdx.x; k < n_ell; k += 256) {
+        const double *src = ell + (sl * n_ell + k) * 6;
+        e[7 * k + 0] = src[0];
+        e[7 * k + 1] = src[1];
+        e[7 * k + 2] = src[2];
+        e[7 * k + 3] = src[3];
+        e[7 * k + 4] = cos(src[4]);
+        e[7 * k + 5] = sin(src[4]);
+        e[7 * k + 6] = src[5];
+    }
+    __syncthreads();
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    if (j >= W)
+        return;
+    const double th = p * step;
+    const double c = cos(th), s = sin(th);
+    const double t = j + 0.5 - axis;
+    double acc = 0.0;
+    for (int k = 0; k < n_ell; k++)
+        acc += ellipse_line_integral(e + 7 * k, c, s, t);
+    out[(sl * P + p) * W + j] = (float) acc;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Counter-based Gaussian noise: element i draws from hash(seed, i) (Box-Muller),
+// independent of the launch shape.
+__global__ __launch_bounds__(256) void noise_kernel(float *__restrict__ d, long n, float sigma, uint64_t seed)
+{
+    const long stride = (long) gridDim.x * 256;
+    for (long i = (long) blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const uint64_t h1 = mix64(seed ^ mix64(2 * (uint64_t) i + 1));
+        const uint64_t h2 = mix64(seed ^ mix64(2 * (uint64_t) i + 2));
+        const double u1 = ((double) (h1 >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+        const double u2 = (double) (h2 >> 11) * 0x1.0p-53;
+        const double g = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+        d[i] += sigma * (float) g;
+    }
+}
+
+__global__ __launch_bounds__(256) void absmax_kernel(const float *__restrict__ d, long n, unsigned *__restrict__ out)
+{
+    __shared__ float red[256];
+    float m = 0.0f;
+    const long stride = (long) gridDim.x * 256;
+    for (long i = (long) blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+        m = fmaxf(m, fabsf(d[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMax(out, __float_as_uint(red[0]));  // non-negative floats order like their bits
+}
+
+}  // namespace
+
+extern "C" int ufomi_phantom_ellipses(uint64_t seed, int n, double radius_px, double *e)
+{
+    if (!e || n < 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid phantom arguments");
+    Rng rng(seed);
+    for (int k = 0; k < n; k++) {
+        // centre uniform in the unit disk scaled by 0.7, semi-axes U[0.05, 0.4],
+        // rotation U[0, pi), density U[-0.5, 1]
+        const double r = sqrt(rng.uniform()) * 0.7, psi = 2.0 * M_PI * rng.uniform();
+        e[6 * k + 0] = r * cos(psi) * radius_px;
+        e[6 * k + 1] = r * sin(psi) * radius_px;
+        e[6 * k + 2] = rng.uniform(0.05, 0.4) * radius_px;
+        e[6 * k + 3] = rng.uniform(0.05, 0.4) * radius_px;
+        e[6 * k + 4] = rng.uniform(0.0, M_PI);
+        e[6 * k + 5] = rng.uniform(-0.5, 1.0);
+    }
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_phantom_ellipsoids(uint64_t seed, int n, double radius_px, double *e)
+{
+    if (!e || n < 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid phantom arguments");
+    Rng rng(seed);
+    for (int k = 0; k < n; k++) {
+        // centre uniform in the unit ball scaled by 0.7, semi-axes U[0.05, 0.4],
+        // rotation about z U[0, pi), density U[-0.5, 1]
+        double x, y, z;
+        do {
+            x = rng.uniform(-1, 1);
+            y = rng.uniform(-1, 1);
+            z = rng.uniform(-1, 1);
+        } while (x * x + y * y + z * z > 1.0);
+        double *o = e + 8 * k;
+        o[0] = 0.7 * x * radius_px;
+        o[1] = 0.7 * y * radius_px;
+        o[2] = 0.7 * z * radius_px;
+        o[3] = rng.uniform(0.05, 0.4) * radius_px;
+        o[4] = rng.uniform(0.05, 0.4) * radius_px;
+        o[5] = rng.uniform(0.05, 0.4) * radius_px;
+        o[6] = rng.uniform(0.0, M_PI);
+        o[7] = rng.uniform(-0.5, 1.0);
+    }
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_slice_ellipsoids(const double *ell, int n, double z, double *out)
+{
+    // Plane section z of ellipsoids rotated about z: an ellipse with the same
+    // centre and rotation, semi-axes scaled by sqrt(1 - ((z - z0) / c)^2);
+    // ellipsoids the plane misses get density 0.
+    if (!ell || !out || n < 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid slice arguments");
+    for (int k = 0; k < n; k++) {
+        const double *e = ell + 8 * k;
+        double *o = out + 6 * k;
+        const double q = 1.0 - ((z - e[2]) / e[5]) * ((z - e[2]) / e[5]);
+        const double f = q > 0.0 ? sqrt(q) : 0.0;
+        o[0] = e[0];
+        o[1] = e[1];
+        o[2] = q > 0.0 ? e[3] * f : 1.0;
+        o[3] = q > 0.0 ? e[4] * f : 1.0;
+        o[4] = e[6];
+        o[5] = q > 0.0 ? e[7] : 0.0;
+    }
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_sinogram_ellipses(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, int n_slices,
+                                       int W, int P, double axis_pos, double angle_step, float *d_sino)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!ellipses || !d_sino || n_ellipses <= 0 || n_ellipses > kMaxEll || n_slices <= 0 || n_slices > 65535 ||
+        W <= 0 || P <= 0 || P > 65535)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid sinogram arguments");
+    const size_t bytes = (size_t) n_slices * n_ellipses * 6 * sizeof(double);
+    double *d_e = nullptr;
+    if ((rc = ufomi_scratch(ctx, bytes, (void **) &d_e))) return rc;
+    if ((rc = ufomi_upload(ctx, d_e, ellipses, bytes))) return rc;
+    dim3 grid((W + 255) / 256, P, n_slices);
+    hipLaunchKernelGGL(sinogram_kernel, grid, dim3(256), 0, ctx->stream, d_e, n_ellipses, W, P, axis_pos,
+                       angle_step, d_sino);
+    UFOMI_CHECK_LAUNCH();
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_add_noise(ufomi_ctx *ctx, float *d, size_t n, double rel_sigma, uint64_t seed, float *max_out)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!d || !n)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid noise arguments");
+    unsigned *d_max = nullptr;
+    if ((rc = ufomi_scratch(ctx, sizeof(unsigned), (void **) &d_max))) return rc;
+    UFOMI_HIP(hipMemsetAsync(d_max, 0, sizeof(unsigned), ctx->stream));
+    long blocks = ((long) n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned) blocks), dim3(256), 0, ctx->stream, d, (long) n, d_max);
+    UFOMI_CHECK_LAUNCH();
+    unsigned bits = 0;
+    UFOMI_HIP(hipMemcpyAsync(&bits, d_max, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    UFOMI_HIP(hipStreamSynchronize(ctx->stream));
+    float m;
+    memcpy(&m, &bits, sizeof(m));
+    if (max_out)
+        *max_out = m;
+    const float sigma = (float) (rel_sigma * m);
+    if (sigma > 0.0f) {
+        hipLaunchKernelGGL(noise_kernel, dim3((unsigned) blocks), dim3(256), 0, ctx->stream, d, (long) n, sigma, seed);
+        UFOMI_CHECK_LAUNCH();
+    }
+    return UFOMI_OK;
+}

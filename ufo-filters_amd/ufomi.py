@@ -1,0 +1,304 @@
+"""Python binding of libufomi.so (include/ufomi.h) over ctypes.
+
+The library is the product: every call below goes to a HIP kernel for gfx950
+through the C ABI.  There is no CPU fallback; a missing or unloadable library
+raises.  Device buffers are torch tensors on a ROCm device (torch is used for
+device memory and streams only), passed by data_ptr().
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libufomi.so")
+
+OK = 0
+STATUS = {
+    -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOT_POW2", -4: "ERR_UNSUPPORTED", -5: "ERR_PROJECTIONS",
+    -6: "ERR_SIZE", -7: "ERR_NO_DEVICE", -8: "ERR_GRAY_MAP", -9: "ERR_MEMORY", -10: "ERR_NUM_PROJECTIONS",
+    -11: "ERR_LAYOUT", -12: "ERR_INCOMPLETE",
+}
+F32, F16, BF16 = 0, 1, 2
+FILTERS = {"ramp": 0, "ramp-fromreal": 1, "butterworth": 2, "faris-byer": 3, "hamming": 4, "bh3": 5,
+           "shepp-logan": 6}
+BP_NEAREST, BP_TEXTURE = 0, 1
+STORE_TYPES = {"half": 0, "float": 1, "double": 2, "uchar": 3, "ushort": 4, "uint": 5}
+ADDRESSING = {"none": 0, "clamp_to_edge": 1, "clamp": 2, "repeat": 3, "mirrored_repeat": 4}
+
+# Every symbol include/ufomi.h declares.
+EXPORTS = [
+    "ufomi_create", "ufomi_destroy", "ufomi_set_stream", "ufomi_synchronize", "ufomi_last_error",
+    "ufomi_version", "ufomi_filter_coefficients", "ufomi_fft_rows", "ufomi_filter_apply", "ufomi_ifft_rows",
+    "ufomi_ramp_filter", "ufomi_bp_params_init", "ufomi_bp_parallel",
+    "ufomi_cone_weight", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
+    "ufomi_gbp", "ufomi_phantom_ellipses", "ufomi_phantom_ellipsoids", "ufomi_slice_ellipsoids",
+    "ufomi_sinogram_ellipses", "ufomi_add_noise", "ufomi_bench_lds_read", "ufomi_bench_copy",
+]
+
+
+class UfomiError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "ERR"), code, message))
+        self.code = code
+
+
+class BpParams(ctypes.Structure):
+    _fields_ = [("num_projections", ctypes.c_uint), ("offset", ctypes.c_uint), ("axis_pos", ctypes.c_double),
+                ("angle_step", ctypes.c_double), ("angle_offset", ctypes.c_double), ("mode", ctypes.c_int),
+                ("roi_x", ctypes.c_uint), ("roi_y", ctypes.c_uint), ("roi_width", ctypes.c_uint),
+                ("roi_height", ctypes.c_uint)]
+
+
+_GEOM_NAMES = ["axis_angle_x", "axis_angle_y", "axis_angle_z", "volume_angle_x", "volume_angle_y",
+               "volume_angle_z", "detector_angle_x", "detector_angle_y", "detector_angle_z",
+               "center_position_x", "center_position_z", "source_position_x", "source_position_y",
+               "source_position_z", "detector_position_x", "detector_position_y", "detector_position_z"]
+
+
+class GbpGeometry(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in _GEOM_NAMES] + \
+               [("n_" + n, ctypes.c_int) for n in _GEOM_NAMES]
+
+
+class GbpParams(ctypes.Structure):
+    _fields_ = [("num_projections", ctypes.c_uint), ("overall_angle", ctypes.c_double), ("z", ctypes.c_double),
+                ("region", ctypes.c_double * 3), ("x_region", ctypes.c_double * 3),
+                ("y_region", ctypes.c_double * 3), ("store_type", ctypes.c_int), ("addressing_mode", ctypes.c_int),
+                ("gray_map_min", ctypes.c_double), ("gray_map_max", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libufomi.so; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libufomi.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ci, cu, cf, cd, csz = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "ufomi_create": [ci, ctypes.POINTER(vp)], "ufomi_destroy": [vp], "ufomi_set_stream": [vp, vp],
+        "ufomi_synchronize": [vp], "ufomi_last_error": [], "ufomi_version": [],
+        "ufomi_filter_coefficients": [ci, cu, cf, cf, cf, cf, cf, vp],
+        "ufomi_fft_rows": [vp, vp, ci, ci, ci, vp], "ufomi_filter_apply": [vp, vp, vp, vp, ci, ci],
+        "ufomi_ifft_rows": [vp, vp, ci, ci, ci, vp], "ufomi_ramp_filter": [vp, vp, ci, ci, ci, vp, ci, vp, ci],
+        "ufomi_bp_params_init": [ctypes.POINTER(BpParams)],
+        "ufomi_bp_parallel": [vp, vp, ci, ci, ci, ci, csz, ctypes.POINTER(BpParams), vp, csz],
+        "ufomi_cone_weight": [vp, vp, vp, ci, ci, ci, dp, dp, dp, dp, dp, ci],
+        "ufomi_gbp_params_init": [ctypes.POINTER(GbpParams)], "ufomi_gbp_geometry_init": [ctypes.POINTER(GbpGeometry)],
+        "ufomi_gbp_volume_shape": [ctypes.POINTER(GbpParams), ci, ctypes.POINTER(ci), ctypes.POINTER(ci),
+                                   ctypes.POINTER(ci)],
+        "ufomi_gbp": [vp, vp, ci, ci, ci, ctypes.POINTER(GbpGeometry), ctypes.POINTER(GbpParams), vp],
+        "ufomi_phantom_ellipses": [ctypes.c_uint64, ci, cd, dp],
+        "ufomi_phantom_ellipsoids": [ctypes.c_uint64, ci, cd, dp],
+        "ufomi_slice_ellipsoids": [dp, ci, cd, dp],
+        "ufomi_sinogram_ellipses": [vp, dp, ci, ci, ci, ci, cd, cd, vp],
+        "ufomi_add_noise": [vp, vp, csz, cd, ctypes.c_uint64, ctypes.POINTER(cf)],
+        "ufomi_bench_lds_read": [vp, ci, ci, vp, ctypes.POINTER(cd)],
+        "ufomi_bench_copy": [vp, vp, vp, csz],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ci
+    L.ufomi_last_error.restype = ctypes.c_char_p
+    L.ufomi_version.restype = ctypes.c_char_p
+    L.ufomi_bp_params_init.restype = None
+    L.ufomi_gbp_params_init.restype = None
+    L.ufomi_gbp_geometry_init.restype = None
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise UfomiError(rc, load().ufomi_last_error().decode())
+    return rc
+
+
+def _dptr(a):
+    return ctypes.cast(np.ascontiguousarray(a, dtype=np.float64).ctypes.data, ctypes.POINTER(ctypes.c_double))
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+# ------------------------------------------------------------------ host-only
+
+def filter_coefficients(width, filter="ramp-fromreal", scale=1.0, cutoff=0.5, order=4.0, tau=0.1, theta=1.0):
+    """float32[width] coefficients of the filter task (width = 2N)."""
+    out = np.empty(width, dtype=np.float32)
+    check(load().ufomi_filter_coefficients(FILTERS[filter], width, scale, cutoff, order, tau, theta,
+                                           out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def phantom_ellipses(seed, n, radius_px):
+    out = np.empty((n, 6), dtype=np.float64)
+    check(load().ufomi_phantom_ellipses(seed, n, radius_px, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def phantom_ellipsoids(seed, n, radius_px):
+    out = np.empty((n, 8), dtype=np.float64)
+    check(load().ufomi_phantom_ellipsoids(seed, n, radius_px, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def slice_ellipsoids(ellipsoids, z):
+    e = np.ascontiguousarray(ellipsoids, dtype=np.float64)
+    out = np.empty((len(e), 6), dtype=np.float64)
+    check(load().ufomi_slice_ellipsoids(_dptr(e), len(e), z, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def gbp_volume_shape(params, width):
+    nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(load().ufomi_gbp_volume_shape(ctypes.byref(params), width, ctypes.byref(nx), ctypes.byref(ny),
+                                        ctypes.byref(nz)))
+    return nz.value, ny.value, nx.value
+
+
+def gbp_params(num_projections, overall_angle=2 * math.pi, region=(0, 0, 0), x_region=(0, 0, 0),
+               y_region=(0, 0, 0), store_type="float", addressing_mode="clamp", gray_map_min=0.0,
+               gray_map_max=0.0):
+    p = GbpParams()
+    load().ufomi_gbp_params_init(ctypes.byref(p))
+    p.num_projections = num_projections
+    p.overall_angle = overall_angle
+    p.region[:] = [float(v) for v in region]
+    p.x_region[:] = [float(v) for v in x_region]
+    p.y_region[:] = [float(v) for v in y_region]
+    p.store_type = STORE_TYPES[store_type]
+    p.addressing_mode = ADDRESSING[addressing_mode]
+    p.gray_map_min = gray_map_min
+    p.gray_map_max = gray_map_max
+    return p
+
+
+def gbp_geometry(**arrays):
+    """Geometry struct from keyword arrays (names with '_'); keeps the numpy arrays alive."""
+    g = GbpGeometry()
+    load().ufomi_gbp_geometry_init(ctypes.byref(g))
+    keep = []
+    for name, val in arrays.items():
+        if name not in _GEOM_NAMES:
+            raise KeyError(name)
+        a = np.ascontiguousarray(np.atleast_1d(val), dtype=np.float64)
+        keep.append(a)
+        setattr(g, name, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        setattr(g, "n_" + name, len(a))
+    g._keep = keep
+    return g
+
+
+# ---------------------------------------------------------------- device API
+
+class Context:
+    """A ufomi context on one device, launching on torch's current stream."""
+
+    def __init__(self, device=0, stream=None):
+        self.L = load()
+        self.h = ctypes.c_void_p()
+        check(self.L.ufomi_create(device, ctypes.byref(self.h)))
+        self.device = device
+        if stream is not None:
+            self.set_stream(stream)
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream (or raw hipStream_t int)."""
+        raw = getattr(stream, "cuda_stream", stream)
+        check(self.L.ufomi_set_stream(self.h, ctypes.c_void_p(raw)))
+
+    def use_current_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream(self.device))
+
+    def synchronize(self):
+        check(self.L.ufomi_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            self.L.ufomi_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- ramp filter chain
+    def fft_rows(self, d_in, W, P, N, d_out):
+        check(self.L.ufomi_fft_rows(self.h, _ptr(d_in), W, P, N, _ptr(d_out)))
+
+    def filter_apply(self, d_in, d_out, d_coef, width, P):
+        check(self.L.ufomi_filter_apply(self.h, _ptr(d_in), _ptr(d_out), _ptr(d_coef), width, P))
+
+    def ifft_rows(self, d_in, N, P, crop, d_out):
+        check(self.L.ufomi_ifft_rows(self.h, _ptr(d_in), N, P, crop, _ptr(d_out)))
+
+    def ramp_filter(self, d_in, W, P, N, d_coef, crop, d_out, dtype=F32):
+        check(self.L.ufomi_ramp_filter(self.h, _ptr(d_in), W, P, N, _ptr(d_coef), crop, _ptr(d_out), dtype))
+
+    # -- backprojection
+    def bp_parallel(self, d_sino, W, P_in, d_out, n_slices=1, dtype=F32, sino_pitch=0, out_pitch=0, **props):
+        p = bp_params(**props)
+        check(self.L.ufomi_bp_parallel(self.h, _ptr(d_sino), dtype, W, P_in, n_slices, sino_pitch,
+                                       ctypes.byref(p), _ptr(d_out), out_pitch))
+
+    def cone_weight(self, d_in, d_out, W, H, n_proj, center_x, center_z, source_distance, detector_distance,
+                    axis_angle_x=0.0):
+        arrs = [np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)
+                for a in (center_x, center_z, source_distance, detector_distance, axis_angle_x)]
+        n = max(len(a) for a in arrs)
+        arrs = [np.full(n, a[0]) if len(a) == 1 else a for a in arrs]
+        check(self.L.ufomi_cone_weight(self.h, _ptr(d_in), _ptr(d_out), W, H, n_proj,
+                                       *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrs], n))
+
+    def gbp(self, d_proj, W, H, P, geometry, params, d_vol):
+        check(self.L.ufomi_gbp(self.h, _ptr(d_proj), W, H, P, ctypes.byref(geometry), ctypes.byref(params),
+                               _ptr(d_vol)))
+
+    # -- synthetic data
+    def sinogram_ellipses(self, ellipses, W, P, axis_pos, angle_step, d_out):
+        e = np.ascontiguousarray(ellipses, dtype=np.float64)
+        if e.ndim == 2:
+            e = e[None]
+        check(self.L.ufomi_sinogram_ellipses(self.h, _dptr(e), e.shape[1], e.shape[0], W, P, axis_pos,
+                                             angle_step, _ptr(d_out)))
+
+    def add_noise(self, d, rel_sigma, seed):
+        m = ctypes.c_float()
+        check(self.L.ufomi_add_noise(self.h, _ptr(d), d.numel(), rel_sigma, seed, ctypes.byref(m)))
+        return m.value
+
+    # -- roofline microbenchmarks
+    def bench_lds_read(self, blocks, iters, d_sink):
+        b = ctypes.c_double()
+        check(self.L.ufomi_bench_lds_read(self.h, blocks, iters, _ptr(d_sink), ctypes.byref(b)))
+        return b.value
+
+    def bench_copy(self, d_in, d_out):
+        check(self.L.ufomi_bench_copy(self.h, _ptr(d_in), _ptr(d_out), d_in.numel()))
+
+
+def bp_params(num_projections=0, offset=0, axis_pos=-1.0, angle_step=-1.0, angle_offset=0.0, mode="texture",
+              roi_x=0, roi_y=0, roi_width=0, roi_height=0):
+    p = BpParams()
+    load().ufomi_bp_params_init(ctypes.byref(p))
+    p.num_projections = num_projections
+    p.offset = offset
+    p.axis_pos = axis_pos
+    p.angle_step = angle_step
+    p.angle_offset = angle_offset
+    p.mode = BP_TEXTURE if mode == "texture" else BP_NEAREST
+    p.roi_x, p.roi_y, p.roi_width, p.roi_height = roi_x, roi_y, roi_width, roi_height
+    return p
