@@ -66,6 +66,15 @@ int ufomi_synchronize(ufomi_ctx *ctx);
 const char *ufomi_last_error(void);
 const char *ufomi_version(void);
 
+/* Device memory of the context's device, for hosts that do not link HIP
+ * (the task layer of include/ufo_tasks.h uses these).  ufomi_memcpy kinds:
+ * 0 host->device, 1 device->host, 2 device->device, ordered on the context's
+ * stream; copies touching host memory return after completion. */
+int ufomi_device_alloc(ufomi_ctx *ctx, size_t bytes, void **out);
+int ufomi_device_free(ufomi_ctx *ctx, void *ptr);
+int ufomi_memcpy(ufomi_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+int ufomi_device_memory(ufomi_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+
 /* --------------------------------------------------------- ramp filtering */
 
 /*

@@ -17,6 +17,16 @@ int ufomi_fail(int code, const char *fmt, ...)
     return code;
 }
 
+// C entry for the host task layer (host/*.c): records a status and message.
+extern "C" int ufomi_status_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
 extern "C" const char *ufomi_last_error(void) { return g_error; }
 extern "C" const char *ufomi_version(void) { return "ufomi 0.1 (gfx950)"; }
 
@@ -128,6 +138,59 @@ extern "C" int ufomi_set_stream(ufomi_ctx *ctx, void *stream)
     if (!ctx)
         return ufomi_fail(UFOMI_ERR_ARG, "null context");
     ctx->stream = (hipStream_t) stream;
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_device_alloc(ufomi_ctx *ctx, size_t bytes, void **out)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!out)
+        return ufomi_fail(UFOMI_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+        (void) hipGetLastError();
+        *out = nullptr;
+        return ufomi_fail(UFOMI_ERR_MEMORY, "cannot allocate %zu bytes of device memory", bytes);
+    }
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_device_free(ufomi_ctx *ctx, void *ptr)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (ptr) {
+        UFOMI_HIP(hipStreamSynchronize(ctx->stream));
+        UFOMI_HIP(hipFree(ptr));
+    }
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_memcpy(ufomi_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if ((!dst || !src) && bytes)
+        return ufomi_fail(UFOMI_ERR_ARG, "null copy pointer");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                           : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown copy kind %d", kind);
+    UFOMI_HIP(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+    if (kind != 2)
+        UFOMI_HIP(hipStreamSynchronize(ctx->stream));
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_device_memory(ufomi_ctx *ctx, size_t *free_bytes, size_t *total_bytes)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    size_t f = 0, t = 0;
+    UFOMI_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
     return UFOMI_OK;
 }
 

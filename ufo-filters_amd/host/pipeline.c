@@ -1,0 +1,304 @@
+/*
+ * A linear pipeline in ufo-launch syntax over the C task layer.
+ *
+ * ufo-core's scheduler pushes one item at a time through every node
+ * (docs/examples.rst:90-105).  This runner keeps the per-task semantics
+ * (get_requisition before process, reductor generate after the last item) but
+ * moves whole batches between stages, which is what a GPU wants:
+ *   - `fft ! filter ! ifft` (1-D) becomes one launch of the fused LDS
+ *     ramp-filter kernel over every row of every item;
+ *   - `backproject` runs all items as one multi-slice launch;
+ *   - other processors run item by item;
+ *   - `general-backproject` takes every item as a projection and emits its
+ *     slices through generate.
+ */
+#include "tasks_internal.h"
+
+#include <ctype.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAX_STAGES 16
+
+struct UfomiPipeline {
+    UfomiTask *tasks[MAX_STAGES];
+    int n;
+    int fused;
+};
+
+static char *trim(char *s)
+{
+    while (isspace((unsigned char) *s))
+        s++;
+    char *e = s + strlen(s);
+    while (e > s && isspace((unsigned char) e[-1]))
+        *--e = '\0';
+    return s;
+}
+
+UfomiPipeline *ufomi_pipeline_new(const char *description, UfomiError *e)
+{
+    if (!description) {
+        task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "null pipeline");
+        return NULL;
+    }
+    UfomiPipeline *pl = calloc(1, sizeof(UfomiPipeline));
+    char *copy = strdup(description), *save = NULL;
+    for (char *stage = strtok_r(copy, "!", &save); stage; stage = strtok_r(NULL, "!", &save)) {
+        char *s = trim(stage), *save2 = NULL;
+        char *name = strtok_r(s, " \t", &save2);
+        if (!name || pl->n == MAX_STAGES) {
+            task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "invalid pipeline stage `%s'", s);
+            goto fail;
+        }
+        UfomiTask *t = ufomi_task_new(name);
+        if (!t) {
+            task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "no task named `%s'", name);
+            goto fail;
+        }
+        pl->tasks[pl->n++] = t;
+        for (char *kv = strtok_r(NULL, " \t", &save2); kv; kv = strtok_r(NULL, " \t", &save2)) {
+            char *eq = strchr(kv, '=');
+            if (!eq) {
+                task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "expected key=value, got `%s'", kv);
+                goto fail;
+            }
+            *eq = '\0';
+            if (ufomi_task_set_property(t, kv, eq + 1)) {
+                task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "%s", ufomi_last_error());
+                goto fail;
+            }
+        }
+    }
+    free(copy);
+    if (!pl->n) {
+        task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "empty pipeline");
+        ufomi_pipeline_free(pl);
+        return NULL;
+    }
+    return pl;
+fail:
+    free(copy);
+    ufomi_pipeline_free(pl);
+    return NULL;
+}
+
+void ufomi_pipeline_free(UfomiPipeline *pl)
+{
+    if (!pl)
+        return;
+    for (int i = 0; i < pl->n; i++)
+        ufomi_task_free(pl->tasks[i]);
+    free(pl);
+}
+
+int ufomi_pipeline_num_tasks(const UfomiPipeline *pl) { return pl ? pl->n : 0; }
+UfomiTask *ufomi_pipeline_task(UfomiPipeline *pl, int i) { return pl && i >= 0 && i < pl->n ? pl->tasks[i] : NULL; }
+int ufomi_pipeline_used_fused_filter(const UfomiPipeline *pl) { return pl ? pl->fused : 0; }
+
+static int is(const UfomiTask *t, const char *name) { return !strcmp(ufomi_task_name(t), name); }
+
+/* A batch of items between stages: n items of `req`, contiguous. */
+typedef struct {
+    UfomiBuffer buf;   /* req = one item; data holds n items */
+    int n;
+    size_t item_bytes;
+} Batch;
+
+static size_t elem_size(int dtype) { return dtype == UFOMI_F32 ? 4 : 2; }
+
+static int batch_alloc(ufomi_ctx *ctx, Batch *b, const UfomiRequisition *req, int n, size_t esize, int dtype,
+                       UfomiBufferLayout layout)
+{
+    memset(b, 0, sizeof(*b));
+    b->n = n;
+    b->item_bytes = ufomi_requisition_elements(req) * esize;
+    b->buf.req = *req;
+    b->buf.dtype = dtype;
+    b->buf.layout = layout;
+    int rc = ufomi_device_alloc(ctx, b->item_bytes * (size_t) (n ? n : 1), &b->buf.data);
+    b->buf.owned = rc == 0;
+    return rc;
+}
+
+static UfomiBuffer item_view(const Batch *b, int i)
+{
+    UfomiBuffer v = b->buf;
+    v.data = (char *) b->buf.data + (size_t) i * b->item_bytes;
+    v.owned = 0;
+    return v;
+}
+
+static size_t rows_of(const UfomiRequisition *r)
+{
+    return (r->n_dims >= 2 ? r->dims[1] : 1) * (r->n_dims == 3 ? r->dims[2] : 1);
+}
+
+int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const UfomiRequisition *in_req,
+                       int n_items, void *d_out, size_t out_capacity, int *n_out, UfomiRequisition *out_req,
+                       UfomiError *e)
+{
+    if (!pl || !ctx || !d_in || !in_req || n_items <= 0 || !d_out || !n_out || !out_req)
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "invalid pipeline arguments");
+    int rc;
+    for (int i = 0; i < pl->n; i++)
+        if ((rc = ufomi_task_setup(pl->tasks[i], ctx, e)))
+            return rc;
+    pl->fused = 0;
+
+    Batch cur;
+    memset(&cur, 0, sizeof(cur));
+    cur.buf.data = (void *) d_in;
+    cur.buf.req = *in_req;
+    cur.buf.layout = UFOMI_LAYOUT_REAL;
+    cur.buf.dtype = UFOMI_F32;
+    cur.n = n_items;
+    cur.item_bytes = ufomi_requisition_elements(in_req) * sizeof(float);
+    int cur_owned = 0;
+
+    for (int s = 0; s < pl->n; s++) {
+        UfomiTask *t = pl->tasks[s];
+        Batch next;
+        UfomiBuffer in0 = item_view(&cur, 0), *ins[1] = {&in0};
+        UfomiRequisition req;
+
+        if (is(t, "fft") && s + 2 < pl->n && is(pl->tasks[s + 1], "filter") && is(pl->tasks[s + 2], "ifft")) {
+            /* fused fft ! filter ! ifft: requisitions of the three tasks, one launch */
+            UfomiRequisition r_fft, r_filt, r_ifft;
+            UfomiBuffer spec = in0, filt;
+            if ((rc = ufomi_task_get_requisition(t, ins, &r_fft, e)))
+                goto out;
+            spec.req = r_fft;
+            spec.layout = UFOMI_LAYOUT_COMPLEX_INTERLEAVED;
+            UfomiBuffer *sp[1] = {&spec};
+            if ((rc = ufomi_task_get_requisition(pl->tasks[s + 1], sp, &r_filt, e)))
+                goto out;
+            filt = spec;
+            filt.req = r_filt;
+            UfomiBuffer *fp[1] = {&filt};
+            if ((rc = ufomi_task_get_requisition(pl->tasks[s + 2], fp, &r_ifft, e)))
+                goto out;
+            if ((rc = batch_alloc(ctx, &next, &r_ifft, cur.n, sizeof(float), UFOMI_F32, UFOMI_LAYOUT_REAL)))
+                goto fail_alloc;
+            const int W = (int) cur.buf.req.dims[0];
+            const int N = (int) (r_fft.dims[0] / 2);
+            const long rows = (long) rows_of(&cur.buf.req) * cur.n;
+            rc = ufomi_ramp_filter(ctx, cur.buf.data, W, (int) rows, N, filter_task_device_coefficients(pl->tasks[s + 1]),
+                                   (int) r_ifft.dims[0], next.buf.data, UFOMI_F32);
+            if (rc) {
+                ufomi_buffer_release(ctx, &next.buf);
+                task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "fused ramp filter: %s", ufomi_last_error());
+                goto out;
+            }
+            for (int k = 0; k < 3; k++)
+                pl->tasks[s + k]->num_processed += (unsigned) cur.n;
+            pl->fused = 1;
+            s += 2;
+        } else if (is(t, "backproject")) {
+            if ((rc = ufomi_task_get_requisition(t, ins, &req, e)))
+                goto out;
+            if ((rc = batch_alloc(ctx, &next, &req, cur.n, sizeof(float), UFOMI_F32, UFOMI_LAYOUT_REAL)))
+                goto fail_alloc;
+            ufomi_bp_params bp;
+            backproject_task_params(t, &bp);
+            rc = ufomi_bp_parallel(ctx, cur.buf.data, cur.buf.dtype, (int) cur.buf.req.dims[0],
+                                   (int) cur.buf.req.dims[1], cur.n, cur.item_bytes / elem_size(cur.buf.dtype), &bp,
+                                   next.buf.data, 0);
+            if (rc) {
+                ufomi_buffer_release(ctx, &next.buf);
+                task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "backproject: %s", ufomi_last_error());
+                goto out;
+            }
+            t->num_processed += (unsigned) cur.n;
+        } else if (ufomi_task_get_mode(t) & UFOMI_TASK_MODE_REDUCTOR) {
+            if ((rc = ufomi_task_get_requisition(t, ins, &req, e)))
+                goto out;
+            for (int i = 0; i < cur.n; i++) {
+                UfomiBuffer v = item_view(&cur, i), *vi[1] = {&v};
+                UfomiBuffer dummy = v;
+                if (!ufomi_task_process(t, vi, &dummy, &req)) {
+                    rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, t->err_status ? t->err_status : UFOMI_ERR_ARG, "%s",
+                                    ufomi_task_last_error(t));
+                    goto out;
+                }
+            }
+            /* general-backproject stores its volume in the store type */
+            const int st = gbp_task_store_type(t);
+            static const size_t st_size[] = {2, 4, 8, 1, 2, 4};
+            int nz = 0;
+            {
+                ufomi_gbp_params gp;
+                ufomi_gbp_params_init(&gp);
+                const PropValue *reg = &t->v[3];
+                for (int k = 0; k < 3; k++)
+                    gp.region[k] = reg->n > k ? reg->arr[k] : 0.0;
+                int nx, ny;
+                ufomi_gbp_volume_shape(&gp, (int) cur.buf.req.dims[0], &nx, &ny, &nz);
+            }
+            if ((rc = batch_alloc(ctx, &next, &req, nz, st_size[st], UFOMI_F32, UFOMI_LAYOUT_REAL)))
+                goto fail_alloc;
+            int produced = 0;
+            for (;;) {
+                UfomiBuffer v = item_view(&next, produced);
+                if (produced >= nz || !ufomi_task_generate(t, &v, &req))
+                    break;
+                produced++;
+            }
+            if (t->err_status) {
+                ufomi_buffer_release(ctx, &next.buf);
+                rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, t->err_status, "%s", ufomi_task_last_error(t));
+                goto out;
+            }
+            next.n = produced;
+            next.buf.dtype = st == UFOMI_ST_FLOAT ? UFOMI_F32 : (st == UFOMI_ST_HALF ? UFOMI_F16 : -1 - st);
+        } else {
+            if ((rc = ufomi_task_get_requisition(t, ins, &req, e)))
+                goto out;
+            if ((rc = batch_alloc(ctx, &next, &req, cur.n, sizeof(float), UFOMI_F32, ufomi_task_output_layout(t))))
+                goto fail_alloc;
+            for (int i = 0; i < cur.n; i++) {
+                UfomiBuffer v = item_view(&cur, i), o = item_view(&next, i), *vi[1] = {&v};
+                if (!ufomi_task_process(t, vi, &o, &req)) {
+                    ufomi_buffer_release(ctx, &next.buf);
+                    rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, t->err_status ? t->err_status : UFOMI_ERR_ARG, "%s",
+                                    ufomi_task_last_error(t));
+                    goto out;
+                }
+            }
+            next.buf.layout = ufomi_task_output_layout(t);
+        }
+        if (cur_owned)
+            ufomi_buffer_release(ctx, &cur.buf);
+        cur = next;
+        cur_owned = 1;
+        continue;
+    fail_alloc:
+        task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_MEMORY, "out of device memory: %s", ufomi_last_error());
+        goto out;
+    }
+
+    {
+        const size_t total = cur.item_bytes * (size_t) cur.n;
+        if (total > out_capacity) {
+            rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_SIZE, "output needs %zu bytes, capacity %zu", total,
+                            out_capacity);
+            goto out;
+        }
+        if ((rc = ufomi_memcpy(ctx, d_out, cur.buf.data, total, 2))) {
+            task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "%s", ufomi_last_error());
+            goto out;
+        }
+        if ((rc = ufomi_synchronize(ctx))) {
+            task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "%s", ufomi_last_error());
+            goto out;
+        }
+        *n_out = cur.n;
+        *out_req = cur.buf.req;
+        rc = UFOMI_OK;
+    }
+out:
+    if (cur_owned)
+        ufomi_buffer_release(ctx, &cur.buf);
+    return rc;
+}

@@ -1,0 +1,914 @@
+/*
+ * The reference's hot-path tasks in plain C over the ufomi C ABI
+ * (include/ufo_tasks.h): property tables with the GObject names, ranges and
+ * effective defaults, and the setup / get_requisition / process / generate
+ * rules of
+ *   fft                          src/ufo-fft-task.c
+ *   filter                       src/ufo-filter-task.c
+ *   ifft                         src/ufo-ifft-task.c
+ *   backproject                  src/ufo-backproject-task.c
+ *   cone-beam-projection-weight  src/ufo-cone-beam-projection-weight-task.c
+ *   general-backproject          src/ufo-general-backproject-task.c
+ */
+#include "tasks_internal.h"
+
+#include <ctype.h>
+#include <errno.h>
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ errors */
+
+int task_error(UfomiError *e, int code, int status, const char *fmt, ...)
+{
+    if (e) {
+        va_list ap;
+        e->code = code;
+        e->status = status;
+        va_start(ap, fmt);
+        vsnprintf(e->message, sizeof(e->message), fmt, ap);
+        va_end(ap);
+    }
+    return status;
+}
+
+static int process_fail(UfomiTask *t, int status, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t->err, sizeof(t->err), fmt, ap);
+    va_end(ap);
+    t->err_status = status;
+    return 0;
+}
+
+const char *ufomi_task_last_error(const UfomiTask *t) { return t ? t->err : ""; }
+
+/* --------------------------------------------------------------- properties */
+
+static const char *const filter_nicks[] = {"ramp", "ramp-fromreal", "butterworth", "faris-byer", "hamming", "bh3",
+                                           "shepp-logan", NULL};
+static const char *const bp_mode_nicks[] = {"nearest", "texture", NULL};
+static const char *const gbp_parameter_nicks[] = {
+    "axis-angle-x", "axis-angle-y", "axis-angle-z", "volume-angle-x", "volume-angle-y", "volume-angle-z",
+    "detector-angle-x", "detector-angle-y", "detector-angle-z", "detector-position-x", "detector-position-y",
+    "detector-position-z", "source-position-x", "source-position-y", "source-position-z", "center-position-x",
+    "center-position-z", "z", NULL};
+static const char *const compute_nicks[] = {"float", "double", NULL};
+static const char *const ft_nicks[] = {"half", "float", "double", NULL};
+static const char *const st_nicks[] = {"half", "float", "double", "uchar", "ushort", "uint", NULL};
+static const char *const addr_nicks[] = {"none", "clamp_to_edge", "clamp", "repeat", "mirrored_repeat", NULL};
+
+#define U32MAX 4294967295.0
+
+/* fft (ufo-fft-task.c:498-531, init :541-568: auto-zeropadding is FALSE after init) */
+static const PropSpec fft_props[] = {
+    {"auto-zeropadding", PT_BOOL, 0, 1, 0, NULL, 0},
+    {"dimensions", PT_UINT, 1, 3, 1, NULL, 0},
+    {"size-x", PT_UINT, 0, 32768, 0, NULL, 0},
+    {"size-y", PT_UINT, 0, 32768, 0, NULL, 0},
+    {"size-z", PT_UINT, 0, 32768, 0, NULL, 0},
+};
+enum { FFT_ZEROPAD, FFT_DIMS, FFT_SX, FFT_SY, FFT_SZ };
+
+/* filter (ufo-filter-task.c:481-521, init :531-545) */
+static const PropSpec filter_props[] = {
+    {"filter", PT_ENUM, 0, 6, 1, filter_nicks, 0},
+    {"cutoff", PT_FLOAT, 0.0, 1.0, 0.5, NULL, 0},
+    {"order", PT_FLOAT, 2.0, 32.0, 4.0, NULL, 0},
+    {"tau", PT_FLOAT, -FLT_MAX, FLT_MAX, 0.1f, NULL, 0},
+    {"theta", PT_FLOAT, -FLT_MAX, FLT_MAX, 1.0, NULL, 0},
+    {"scale", PT_FLOAT, -FLT_MAX, FLT_MAX, 1.0, NULL, 0},
+};
+enum { FLT_TYPE, FLT_CUTOFF, FLT_ORDER, FLT_TAU, FLT_THETA, FLT_SCALE };
+
+/* ifft (ufo-ifft-task.c:461-480) */
+static const PropSpec ifft_props[] = {
+    {"dimensions", PT_UINT, 1, 3, 1, NULL, 0},
+    {"crop-width", PT_UINT, 0, 32768, 0, NULL, 0},
+    {"crop-height", PT_UINT, 0, 32768, 0, NULL, 0},
+};
+enum { IFFT_DIMS, IFFT_CW, IFFT_CH };
+
+/* backproject (ufo-backproject-task.c:442-510, init :520-541) */
+static const PropSpec bp_props[] = {
+    {"num-projections", PT_UINT, 0, 32768, 0, NULL, 0},
+    {"offset", PT_UINT, 0, 32768, 0, NULL, 0},
+    {"axis-pos", PT_DOUBLE, -1.0, 32768.0, -1.0, NULL, 0},
+    {"angle-step", PT_DOUBLE, -DBL_MAX, DBL_MAX, -1.0, NULL, 0},
+    {"angle-offset", PT_DOUBLE, 0.0, DBL_MAX, 0.0, NULL, 0},
+    {"mode", PT_ENUM, 0, 1, 1, bp_mode_nicks, 0},
+    {"roi-x", PT_UINT, 0, U32MAX, 0, NULL, 0},
+    {"roi-y", PT_UINT, 0, U32MAX, 0, NULL, 0},
+    {"roi-width", PT_UINT, 0, U32MAX, 0, NULL, 0},
+    {"roi-height", PT_UINT, 0, U32MAX, 0, NULL, 0},
+};
+enum { BP_NPROJ, BP_OFFSET, BP_AXIS, BP_STEP, BP_AOFF, BP_MODE, BP_RX, BP_RY, BP_RW, BP_RH };
+
+/* cone-beam-projection-weight (ufo-cone-beam-projection-weight-task.c:268-301, init :309-320) */
+static const PropSpec cw_props[] = {
+    {"center-position-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 0},
+    {"center-position-z", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 0},
+    {"source-distance", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 0},
+    {"detector-distance", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 0},
+    {"axis-angle-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+};
+enum { CW_CX, CW_CZ, CW_SD, CW_DD, CW_AX };
+
+/* general-backproject (ufo-general-backproject-task.c:2087-2311, init :2319-2359) */
+static const PropSpec gbp_props[] = {
+    {"burst", PT_UINT, 0, 128, 0, NULL, 0},
+    {"parameter", PT_ENUM, 0, 17, 17, gbp_parameter_nicks, 0},
+    {"z", PT_DOUBLE, -DBL_MAX, DBL_MAX, 0.0, NULL, 0},
+    {"region", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 3},
+    {"x-region", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 3},
+    {"y-region", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 3},
+    {"center-position-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"center-position-z", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"source-position-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"source-position-y", PT_DARRAY, -INFINITY, INFINITY, -INFINITY, NULL, 1},
+    {"source-position-z", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"detector-position-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"detector-position-y", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"detector-position-z", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"detector-angle-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"detector-angle-y", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"detector-angle-z", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"axis-angle-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"axis-angle-y", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"axis-angle-z", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"volume-angle-x", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"volume-angle-y", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"volume-angle-z", PT_DARRAY, -INFINITY, INFINITY, 0, NULL, 1},
+    {"num-projections", PT_UINT, 0, 32768, 0, NULL, 0},
+    {"compute-type", PT_ENUM, 0, 1, 0, compute_nicks, 0},
+    {"result-type", PT_ENUM, 0, 2, 1, ft_nicks, 0},
+    {"store-type", PT_ENUM, 0, 5, 1, st_nicks, 0},
+    {"overall-angle", PT_DOUBLE, -DBL_MAX, DBL_MAX, 2 * M_PI, NULL, 0},
+    {"addressing-mode", PT_ENUM, 0, 4, 2, addr_nicks, 0},
+    {"gray-map-min", PT_DOUBLE, -DBL_MAX, DBL_MAX, 0.0, NULL, 0},
+    {"gray-map-max", PT_DOUBLE, -DBL_MAX, DBL_MAX, 0.0, NULL, 0},
+};
+enum { G_BURST, G_PARAM, G_Z, G_REGION, G_XREGION, G_YREGION, G_CX, G_CZ, G_SX, G_SY, G_SZ, G_DX, G_DY, G_DZ,
+       G_DAX, G_DAY, G_DAZ, G_AX, G_AY, G_AZ, G_VX, G_VY, G_VZ, G_NPROJ, G_COMPUTE, G_RESULT, G_STORE, G_OVERALL,
+       G_ADDR, G_GMIN, G_GMAX };
+
+static void prop_reset(UfomiTask *t)
+{
+    for (int i = 0; i < t->cls->n_props; i++) {
+        const PropSpec *s = &t->cls->props[i];
+        PropValue *v = &t->v[i];
+        free(v->arr);
+        v->arr = NULL;
+        v->n = 0;
+        v->num = s->dflt;
+        if (s->type == PT_DARRAY && s->n_default) {
+            v->n = s->n_default;
+            v->arr = calloc((size_t) v->n, sizeof(double));
+            for (int k = 0; k < v->n; k++)
+                v->arr[k] = s->dflt;
+        }
+    }
+}
+
+static int prop_index(const UfomiTask *t, const char *name)
+{
+    for (int i = 0; i < t->cls->n_props; i++) {
+        if (!strcmp(t->cls->props[i].name, name))
+            return i;
+        /* accept '_' for '-' like GObject's canonical names */
+        const char *a = t->cls->props[i].name, *b = name;
+        while (*a && *b && (*a == *b || (*a == '-' && *b == '_'))) {
+            a++;
+            b++;
+        }
+        if (!*a && !*b)
+            return i;
+    }
+    return -1;
+}
+
+static int parse_double(const char *s, double *out)
+{
+    char *end;
+    while (isspace((unsigned char) *s))
+        s++;
+    if (!strncasecmp(s, "-inf", 4)) {
+        *out = -INFINITY;
+        return 1;
+    }
+    if (!strncasecmp(s, "inf", 3) || !strncasecmp(s, "+inf", 4)) {
+        *out = INFINITY;
+        return 1;
+    }
+    errno = 0;
+    *out = strtod(s, &end);
+    if (end == s || errno == ERANGE)
+        return 0;
+    while (isspace((unsigned char) *end))
+        end++;
+    return *end == '\0';
+}
+
+int ufomi_task_set_property(UfomiTask *t, const char *name, const char *value)
+{
+    if (!t || !name || !value)
+        return ufomi_status_fail(UFOMI_ERR_ARG, "null argument");
+    const int i = prop_index(t, name);
+    if (i < 0)
+        return ufomi_status_fail(UFOMI_ERR_ARG, "%s has no property named `%s'", t->cls->name, name);
+    const PropSpec *s = &t->cls->props[i];
+    PropValue *v = &t->v[i];
+    double d;
+    switch (s->type) {
+    case PT_BOOL:
+        if (!strcasecmp(value, "true") || !strcmp(value, "1"))
+            v->num = 1;
+        else if (!strcasecmp(value, "false") || !strcmp(value, "0"))
+            v->num = 0;
+        else
+            return ufomi_status_fail(UFOMI_ERR_ARG, "invalid boolean `%s' for %s", value, name);
+        return UFOMI_OK;
+    case PT_ENUM:
+        for (int k = 0; s->nicks[k]; k++) {
+            if (!strcmp(s->nicks[k], value)) {
+                v->num = k;
+                return UFOMI_OK;
+            }
+        }
+        return ufomi_status_fail(UFOMI_ERR_ARG, "`%s' is not a valid value of %s", value, name);
+    case PT_UINT:
+    case PT_DOUBLE:
+    case PT_FLOAT:
+        if (!parse_double(value, &d))
+            return ufomi_status_fail(UFOMI_ERR_ARG, "cannot parse `%s' for %s", value, name);
+        if (s->type == PT_UINT && (d != floor(d)))
+            return ufomi_status_fail(UFOMI_ERR_ARG, "%s needs an unsigned integer, got `%s'", name, value);
+        if (s->type == PT_FLOAT)
+            d = (float) d;
+        if (!(d >= s->min && d <= s->max))
+            return ufomi_status_fail(UFOMI_ERR_ARG, "value %s of %s is outside [%g, %g]", value, name, s->min, s->max);
+        v->num = d;
+        return UFOMI_OK;
+    case PT_DARRAY: {
+        double vals[65536];
+        int n = 0;
+        const char *p = value;
+        while (*p) {
+            char tok[128];
+            size_t len = strcspn(p, ",");
+            if (len >= sizeof(tok))
+                return ufomi_status_fail(UFOMI_ERR_ARG, "array element too long in %s", name);
+            memcpy(tok, p, len);
+            tok[len] = '\0';
+            if (n >= (int) (sizeof(vals) / sizeof(vals[0])) || !parse_double(tok, &vals[n]))
+                return ufomi_status_fail(UFOMI_ERR_ARG, "cannot parse array `%s' for %s", value, name);
+            n++;
+            p += len;
+            if (*p == ',')
+                p++;
+        }
+        free(v->arr);
+        v->arr = n ? malloc((size_t) n * sizeof(double)) : NULL;
+        if (n)
+            memcpy(v->arr, vals, (size_t) n * sizeof(double));
+        v->n = n;
+        return UFOMI_OK;
+    }
+    }
+    return ufomi_status_fail(UFOMI_ERR_ARG, "unknown property type");
+}
+
+int ufomi_task_get_property(const UfomiTask *t, const char *name, char *buf, size_t len)
+{
+    if (!t || !name || !buf || !len)
+        return ufomi_status_fail(UFOMI_ERR_ARG, "null argument");
+    const int i = prop_index(t, name);
+    if (i < 0)
+        return ufomi_status_fail(UFOMI_ERR_ARG, "%s has no property named `%s'", t->cls->name, name);
+    const PropSpec *s = &t->cls->props[i];
+    const PropValue *v = &t->v[i];
+    switch (s->type) {
+    case PT_BOOL:
+        snprintf(buf, len, "%s", v->num ? "true" : "false");
+        break;
+    case PT_ENUM:
+        snprintf(buf, len, "%s", s->nicks[(int) v->num]);
+        break;
+    case PT_UINT:
+        snprintf(buf, len, "%.0f", v->num);
+        break;
+    case PT_DOUBLE:
+    case PT_FLOAT:
+        snprintf(buf, len, "%.17g", v->num);
+        break;
+    case PT_DARRAY: {
+        size_t off = 0;
+        buf[0] = '\0';
+        for (int k = 0; k < v->n && off < len; k++)
+            off += (size_t) snprintf(buf + off, len - off, k ? ",%.17g" : "%.17g", v->arr[k]);
+        break;
+    }
+    }
+    return UFOMI_OK;
+}
+
+const char *const *ufomi_task_list_properties(const UfomiTask *t)
+{
+    return t ? t->names : NULL;
+}
+
+static double arr_at(const PropValue *v, unsigned i)
+{
+    /* ufo_scarray_get_double: one element broadcasts (ufo-scarray.c:118-127) */
+    if (v->n == 1)
+        return v->arr[0];
+    return (int) i < v->n ? v->arr[i] : NAN;
+}
+
+/* ------------------------------------------------------------ fft / ifft */
+
+/* ufo_math_compute_closest_smaller_power_of_2 (ufo-math.c:145-151) */
+static size_t closest_smaller_pow2(size_t x)
+{
+    double ip;
+    modf(log2((double) x), &ip);
+    return (size_t) pow(2, ip);
+}
+
+static int is_pow2(size_t x) { return x && !(x & (x - 1)); }
+
+typedef struct {
+    size_t n;     /* transform size */
+    size_t rows;  /* batch */
+} FftPriv;
+
+static int fft_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    FftPriv *p = t->priv;
+    const UfomiRequisition *ir = &in[0]->req;
+    if ((int) t->v[FFT_DIMS].num != 1)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
+                          "fft: only dimensions=1 is built (2-D and 3-D transforms are not)");
+    if (in[0]->layout == UFOMI_LAYOUT_COMPLEX_INTERLEAVED)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
+                          "fft: complex input is not built");
+    const size_t user = (size_t) t->v[FFT_SX].num;
+    const size_t w = ir->dims[0];
+    /* ufo-fft-task.c:139-156 */
+    if (user != 0 && user < w)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
+                          "Specified size must be greater than or equal to input size");
+    size_t n = user ? user : w;
+    if (!user && t->v[FFT_ZEROPAD].num != 0)
+        n = 2 * closest_smaller_pow2(n - 1);
+    if (n != 2 * closest_smaller_pow2(n - 1) || !is_pow2(n))
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
+                          "fft: size %zu is not a power of two; the chirp-z path is not built", n);
+    p->n = n;
+    p->rows = (ir->n_dims >= 2 ? ir->dims[1] : 1) * (ir->n_dims == 3 ? ir->dims[2] : 1);
+    req->n_dims = ir->n_dims;
+    req->dims[0] = 2 * n;
+    req->dims[1] = ir->n_dims >= 2 ? ir->dims[1] : 1;
+    req->dims[2] = ir->n_dims == 3 ? ir->dims[2] : 1;
+    return UFOMI_OK;
+}
+
+static int fft_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    FftPriv *p = t->priv;
+    int rc = ufomi_fft_rows(t->ctx, in[0]->data, (int) in[0]->req.dims[0], (int) p->rows, (int) p->n, out->data);
+    if (rc)
+        return process_fail(t, rc, "fft: %s", ufomi_last_error());
+    out->layout = UFOMI_LAYOUT_COMPLEX_INTERLEAVED;
+    return 1;
+}
+
+typedef struct {
+    size_t n, rows, crop;
+} IfftPriv;
+
+static int ifft_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    IfftPriv *p = t->priv;
+    const UfomiRequisition *ir = &in[0]->req;
+    /* ufo-ifft-task.c:121-125 */
+    if (in[0]->layout != UFOMI_LAYOUT_COMPLEX_INTERLEAVED)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_LAYOUT, "ifft input must be complex");
+    if ((int) t->v[IFFT_DIMS].num != 1)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
+                          "ifft: only dimensions=1 is built");
+    const size_t cw = (size_t) t->v[IFFT_CW].num, ch = (size_t) t->v[IFFT_CH].num;
+    /* :130-134 compares the user size with the input requisition (complex width) */
+    if ((cw && cw > ir->dims[0]) || (ch && ir->n_dims >= 2 && ch > ir->dims[1]))
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
+                          "Cropped size must be less than or equal to input size");
+    const size_t n = ir->dims[0] / 2;
+    if (!is_pow2(n))
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
+                          "ifft: size %zu is not a power of two; the chirp-z path is not built", n);
+    if (ch && ch != (ir->n_dims >= 2 ? ir->dims[1] : 1))
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
+                          "ifft: crop-height needs dimensions=2, which is not built");
+    if (cw > n)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
+                          "Cropped size must be less than or equal to input size");
+    p->n = n;
+    p->rows = (ir->n_dims >= 2 ? ir->dims[1] : 1) * (ir->n_dims == 3 ? ir->dims[2] : 1);
+    p->crop = cw ? cw : n;
+    req->n_dims = ir->n_dims;
+    req->dims[0] = p->crop;
+    req->dims[1] = ir->n_dims >= 2 ? ir->dims[1] : 1;
+    req->dims[2] = ir->n_dims == 3 ? ir->dims[2] : 1;
+    return UFOMI_OK;
+}
+
+static int ifft_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    IfftPriv *p = t->priv;
+    int rc = ufomi_ifft_rows(t->ctx, in[0]->data, (int) p->n, (int) p->rows, (int) p->crop, out->data);
+    if (rc)
+        return process_fail(t, rc, "ifft: %s", ufomi_last_error());
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+/* ---------------------------------------------------------------- filter */
+
+typedef struct {
+    float *d_coef;
+    size_t width;
+} FilterPriv;
+
+int filter_task_coefficients(UfomiTask *t, size_t width, UfomiError *e)
+{
+    FilterPriv *p = t->priv;
+    if (p->d_coef && p->width == width)
+        return UFOMI_OK;
+    /* ufo-filter-task.c:295-300 */
+    if (!is_pow2(width))
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_NOT_POW2,
+                          "Invalid filter input width %zu, must be a power of two", width);
+    float *h = malloc(width * sizeof(float));
+    int rc = ufomi_filter_coefficients((int) t->v[FLT_TYPE].num, (unsigned) width, (float) t->v[FLT_SCALE].num,
+                                       (float) t->v[FLT_CUTOFF].num, (float) t->v[FLT_ORDER].num,
+                                       (float) t->v[FLT_TAU].num, (float) t->v[FLT_THETA].num, h);
+    if (!rc && p->d_coef) {
+        ufomi_device_free(t->ctx, p->d_coef);
+        p->d_coef = NULL;
+    }
+    if (!rc)
+        rc = ufomi_device_alloc(t->ctx, width * sizeof(float), (void **) &p->d_coef);
+    if (!rc)
+        rc = ufomi_memcpy(t->ctx, p->d_coef, h, width * sizeof(float), 0);
+    free(h);
+    if (rc)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, rc, "filter: %s", ufomi_last_error());
+    p->width = width;
+    return UFOMI_OK;
+}
+
+const float *filter_task_device_coefficients(const UfomiTask *t) { return ((FilterPriv *) t->priv)->d_coef; }
+
+static int filter_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    *req = in[0]->req;
+    return filter_task_coefficients(t, in[0]->req.dims[0], e);
+}
+
+static int filter_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    FilterPriv *p = t->priv;
+    const size_t rows = (req->n_dims >= 2 ? req->dims[1] : 1) * (req->n_dims == 3 ? req->dims[2] : 1);
+    int rc = ufomi_filter_apply(t->ctx, in[0]->data, out->data, p->d_coef, (int) p->width, (int) rows);
+    if (rc)
+        return process_fail(t, rc, "filter: %s", ufomi_last_error());
+    out->layout = in[0]->layout;
+    return 1;
+}
+
+static void filter_finalize(UfomiTask *t)
+{
+    FilterPriv *p = t->priv;
+    if (p && p->d_coef && t->ctx)
+        ufomi_device_free(t->ctx, p->d_coef);
+}
+
+/* ----------------------------------------------------------- backproject */
+
+typedef struct {
+    unsigned n_projections, burst;
+} BpPriv;
+
+void backproject_task_params(const UfomiTask *t, ufomi_bp_params *bp)
+{
+    const BpPriv *p = t->priv;
+    ufomi_bp_params_init(bp);
+    bp->num_projections = p->n_projections;
+    bp->offset = (unsigned) t->v[BP_OFFSET].num;
+    bp->axis_pos = t->v[BP_AXIS].num;
+    bp->angle_step = t->v[BP_STEP].num;
+    bp->angle_offset = t->v[BP_AOFF].num;
+    bp->mode = (int) t->v[BP_MODE].num == 0 ? UFOMI_BP_NEAREST : UFOMI_BP_TEXTURE;
+    bp->roi_x = (unsigned) t->v[BP_RX].num;
+    bp->roi_y = (unsigned) t->v[BP_RY].num;
+    bp->roi_width = (unsigned) t->v[BP_RW].num;
+    bp->roi_height = (unsigned) t->v[BP_RH].num;
+}
+
+static int bp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    BpPriv *p = t->priv;
+    const UfomiRequisition *ir = &in[0]->req;
+    /* ufo-backproject-task.c:221-244: num-projections defaults to the first
+     * sinogram's height and stays; the burst is every sinogram's height. */
+    if (!p->n_projections)
+        p->n_projections = (unsigned) t->v[BP_NPROJ].num ? (unsigned) t->v[BP_NPROJ].num : (unsigned) ir->dims[1];
+    p->burst = (unsigned) ir->dims[1];
+    if (p->burst > p->n_projections)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_PROJECTIONS,
+                          "Total number of projections (%u) must be greater than or equal to sinogram height (%u)",
+                          p->n_projections, p->burst);
+    req->n_dims = 2;
+    req->dims[0] = t->v[BP_RW].num ? (size_t) t->v[BP_RW].num : ir->dims[0];
+    req->dims[1] = t->v[BP_RH].num ? (size_t) t->v[BP_RH].num : ir->dims[0];
+    req->dims[2] = 1;
+    return UFOMI_OK;
+}
+
+static int bp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    ufomi_bp_params bp;
+    backproject_task_params(t, &bp);
+    int rc = ufomi_bp_parallel(t->ctx, in[0]->data, in[0]->dtype, (int) in[0]->req.dims[0], (int) in[0]->req.dims[1],
+                               1, 0, &bp, out->data, 0);
+    if (rc)
+        return process_fail(t, rc, "backproject: %s", ufomi_last_error());
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+/* -------------------------------------------------------- cone weighting */
+
+typedef struct {
+    unsigned count;
+} CwPriv;
+
+static int cw_setup(UfomiTask *t, UfomiError *e)
+{
+    static const int need[] = {CW_CX, CW_CZ, CW_SD, CW_DD};
+    for (int k = 0; k < 4; k++)
+        if (t->v[need[k]].n == 0)
+            return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "cone-beam-projection-weight: %s must be set",
+                              t->cls->props[need[k]].name);
+    return UFOMI_OK;
+}
+
+static int cw_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    (void) t;
+    (void) e;
+    *req = in[0]->req;
+    return UFOMI_OK;
+}
+
+static int cw_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    CwPriv *p = t->priv;
+    const unsigned i = p->count;
+    double vals[5];
+    for (int k = 0; k < 5; k++) {
+        const PropValue *v = &t->v[k];
+        if (v->n != 1 && (int) i >= v->n)
+            return process_fail(t, UFOMI_ERR_ARG, "cone-beam-projection-weight: %s has no value for projection %u",
+                                t->cls->props[k].name, i);
+        vals[k] = arr_at(v, i);
+    }
+    int rc = ufomi_cone_weight(t->ctx, in[0]->data, out->data, (int) req->dims[0], (int) req->dims[1], 1, &vals[0],
+                               &vals[1], &vals[2], &vals[3], &vals[4], 1);
+    if (rc)
+        return process_fail(t, rc, "cone-beam-projection-weight: %s", ufomi_last_error());
+    p->count++;
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+/* ---------------------------------------------------- general-backproject */
+
+typedef struct {
+    int width, height;
+    unsigned count, generated;
+    float *d_stack;       /* num_projections projections [H][W] */
+    void *d_volume;       /* [nz][ny][nx] in the store type */
+    int nx, ny, nz, reconstructed;
+    ufomi_gbp_params params;
+} GbpPriv;
+
+static size_t store_size(int st)
+{
+    static const size_t sz[] = {2, 4, 8, 1, 2, 4};
+    return sz[st];
+}
+
+int gbp_task_store_type(const UfomiTask *t) { return (int) t->v[G_STORE].num; }
+
+static void gbp_fill_params(const UfomiTask *t, ufomi_gbp_params *p)
+{
+    ufomi_gbp_params_init(p);
+    p->num_projections = (unsigned) t->v[G_NPROJ].num;
+    p->overall_angle = t->v[G_OVERALL].num;
+    p->z = t->v[G_Z].num;
+    for (int k = 0; k < 3; k++) {
+        p->region[k] = t->v[G_REGION].n > k ? t->v[G_REGION].arr[k] : 0.0;
+        p->x_region[k] = t->v[G_XREGION].n > k ? t->v[G_XREGION].arr[k] : 0.0;
+        p->y_region[k] = t->v[G_YREGION].n > k ? t->v[G_YREGION].arr[k] : 0.0;
+    }
+    p->store_type = (int) t->v[G_STORE].num;
+    static const int addr_map[] = {UFOMI_ADDR_NONE, UFOMI_ADDR_CLAMP_TO_EDGE, UFOMI_ADDR_CLAMP, UFOMI_ADDR_REPEAT,
+                                   UFOMI_ADDR_MIRRORED_REPEAT};
+    p->addressing_mode = addr_map[(int) t->v[G_ADDR].num];
+    p->gray_map_min = t->v[G_GMIN].num;
+    p->gray_map_max = t->v[G_GMAX].num;
+}
+
+static void gbp_fill_geometry(const UfomiTask *t, ufomi_gbp_geometry *g)
+{
+    ufomi_gbp_geometry_init(g);
+#define GEO(field, idx)                        \
+    g->field = t->v[idx].arr;                  \
+    g->n_##field = t->v[idx].n;
+    GEO(axis_angle_x, G_AX) GEO(axis_angle_y, G_AY) GEO(axis_angle_z, G_AZ)
+    GEO(volume_angle_x, G_VX) GEO(volume_angle_y, G_VY) GEO(volume_angle_z, G_VZ)
+    GEO(detector_angle_x, G_DAX) GEO(detector_angle_y, G_DAY) GEO(detector_angle_z, G_DAZ)
+    GEO(center_position_x, G_CX) GEO(center_position_z, G_CZ)
+    GEO(source_position_x, G_SX) GEO(source_position_y, G_SY) GEO(source_position_z, G_SZ)
+    GEO(detector_position_x, G_DX) GEO(detector_position_y, G_DY) GEO(detector_position_z, G_DZ)
+#undef GEO
+}
+
+static int gbp_setup(UfomiTask *t, UfomiError *e)
+{
+    /* ufo-general-backproject-task.c:1429-1453 */
+    if (!(unsigned) t->v[G_NPROJ].num)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_NUM_PROJECTIONS, "Number of projections not set");
+    const int st = (int) t->v[G_STORE].num;
+    if (t->v[G_GMIN].num >= t->v[G_GMAX].num && (st == UFOMI_ST_UCHAR || st == UFOMI_ST_USHORT || st == UFOMI_ST_UINT))
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_GRAY_MAP,
+                          "Gray mapping minimum must be less then the maximum");
+    if ((int) t->v[G_PARAM].num != 17)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
+                          "general-backproject: only parameter=z is built");
+    if ((int) t->v[G_COMPUTE].num != 0 || (int) t->v[G_RESULT].num != 1)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
+                          "general-backproject: compute-type and result-type float only");
+    const int addr = (int) t->v[G_ADDR].num;
+    if (addr == 3 || addr == 4)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
+                          "general-backproject: repeat addressing needs normalized coordinates");
+    return UFOMI_OK;
+}
+
+static int gbp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    GbpPriv *p = t->priv;
+    const UfomiRequisition *ir = &in[0]->req;
+    if (!p->d_stack) {
+        gbp_fill_params(t, &p->params);
+        int rc = ufomi_gbp_volume_shape(&p->params, (int) ir->dims[0], &p->nx, &p->ny, &p->nz);
+        if (rc)
+            return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, rc, "%s", ufomi_last_error());
+        p->width = (int) ir->dims[0];
+        p->height = ir->n_dims >= 2 ? (int) ir->dims[1] : 1;
+        const size_t proj_bytes = (size_t) p->params.num_projections * p->width * p->height * sizeof(float);
+        const size_t vol_bytes = (size_t) p->nx * p->ny * p->nz * store_size(p->params.store_type);
+        size_t free_b = 0, total_b = 0;
+        ufomi_device_memory(t->ctx, &free_b, &total_b);
+        /* :1542-1546 */
+        if (proj_bytes + vol_bytes > total_b)
+            return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_MEMORY, "Volume size doesn't fit to memory");
+        if ((rc = ufomi_device_alloc(t->ctx, proj_bytes, (void **) &p->d_stack)) ||
+            (rc = ufomi_device_alloc(t->ctx, vol_bytes, &p->d_volume)))
+            return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_MEMORY, "Volume size doesn't fit to memory");
+    } else if ((int) ir->dims[0] != p->width || (ir->n_dims >= 2 ? (int) ir->dims[1] : 1) != p->height) {
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG,
+                          "general-backproject: projection size changed");
+    }
+    req->n_dims = 2;
+    req->dims[0] = (size_t) p->nx;
+    req->dims[1] = (size_t) p->ny;
+    req->dims[2] = 1;
+    return UFOMI_OK;
+}
+
+static int gbp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) out;
+    (void) req;
+    GbpPriv *p = t->priv;
+    if (p->count >= p->params.num_projections)
+        return process_fail(t, UFOMI_ERR_ARG, "general-backproject: more projections than num-projections");
+    const size_t plane = (size_t) p->width * p->height;
+    int rc = ufomi_memcpy(t->ctx, p->d_stack + p->count * plane, in[0]->data, plane * sizeof(float), 2);
+    if (rc)
+        return process_fail(t, rc, "general-backproject: %s", ufomi_last_error());
+    p->count++;
+    return 1;
+}
+
+static int gbp_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    GbpPriv *p = t->priv;
+    /* :1733-1742 */
+    if (p->count != p->params.num_projections) {
+        process_fail(t, UFOMI_ERR_INCOMPLETE,
+                     "general-backproject received only %u projections out of %u specified, no outuput will be generated",
+                     p->count, p->params.num_projections);
+        fprintf(stderr, "warning: %s\n", t->err);
+        return 0;
+    }
+    if (p->generated >= (unsigned) p->nz)
+        return 0;
+    if (!p->reconstructed) {
+        ufomi_gbp_geometry g;
+        gbp_fill_geometry(t, &g);
+        int rc = ufomi_gbp(t->ctx, p->d_stack, p->width, p->height, (int) p->count, &g, &p->params, p->d_volume);
+        if (rc)
+            return process_fail(t, rc, "general-backproject: %s", ufomi_last_error());
+        p->reconstructed = 1;
+    }
+    const size_t slice = (size_t) p->nx * p->ny * store_size(p->params.store_type);
+    int rc = ufomi_memcpy(t->ctx, out->data, (char *) p->d_volume + p->generated * slice, slice, 2);
+    if (rc)
+        return process_fail(t, rc, "general-backproject: %s", ufomi_last_error());
+    p->generated++;
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+static void gbp_finalize(UfomiTask *t)
+{
+    GbpPriv *p = t->priv;
+    if (p && t->ctx) {
+        if (p->d_stack)
+            ufomi_device_free(t->ctx, p->d_stack);
+        if (p->d_volume)
+            ufomi_device_free(t->ctx, p->d_volume);
+    }
+}
+
+/* ---------------------------------------------------------------- classes */
+
+#define NPROPS(a) ((int) (sizeof(a) / sizeof((a)[0])))
+
+static const TaskClass classes[] = {
+    {"fft", fft_props, NPROPS(fft_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU, sizeof(FftPriv), NULL,
+     fft_requisition, fft_process, NULL, NULL},
+    {"filter", filter_props, NPROPS(filter_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU,
+     sizeof(FilterPriv), NULL, filter_requisition, filter_process, NULL, filter_finalize},
+    {"ifft", ifft_props, NPROPS(ifft_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU, sizeof(IfftPriv), NULL,
+     ifft_requisition, ifft_process, NULL, NULL},
+    {"backproject", bp_props, NPROPS(bp_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU, sizeof(BpPriv), NULL,
+     bp_requisition, bp_process, NULL, NULL},
+    {"cone-beam-projection-weight", cw_props, NPROPS(cw_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU,
+     sizeof(CwPriv), cw_setup, cw_requisition, cw_process, NULL, NULL},
+    {"general-backproject", gbp_props, NPROPS(gbp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
+     sizeof(GbpPriv), gbp_setup, gbp_requisition, gbp_process, gbp_generate, gbp_finalize},
+};
+
+UfomiTask *ufomi_task_new(const char *name)
+{
+    if (!name)
+        return NULL;
+    for (size_t c = 0; c < sizeof(classes) / sizeof(classes[0]); c++) {
+        if (strcmp(classes[c].name, name))
+            continue;
+        UfomiTask *t = calloc(1, sizeof(UfomiTask));
+        t->cls = &classes[c];
+        t->priv = calloc(1, classes[c].priv_size);
+        for (int i = 0; i < t->cls->n_props; i++)
+            t->names[i] = t->cls->props[i].name;
+        t->names[t->cls->n_props] = NULL;
+        prop_reset(t);
+        return t;
+    }
+    ufomi_status_fail(UFOMI_ERR_ARG, "no task named `%s'", name);
+    return NULL;
+}
+
+void ufomi_task_free(UfomiTask *t)
+{
+    if (!t)
+        return;
+    if (t->cls->finalize)
+        t->cls->finalize(t);
+    for (int i = 0; i < t->cls->n_props; i++)
+        free(t->v[i].arr);
+    free(t->priv);
+    free(t);
+}
+
+const char *ufomi_task_name(const UfomiTask *t) { return t ? t->cls->name : NULL; }
+
+int ufomi_task_setup(UfomiTask *t, ufomi_ctx *ctx, UfomiError *e)
+{
+    if (!t || !ctx)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "null task or context");
+    t->ctx = ctx;
+    return t->cls->setup ? t->cls->setup(t, e) : UFOMI_OK;
+}
+
+int ufomi_task_get_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    if (!t || !in || !in[0] || !req)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG, "null argument");
+    if (!t->ctx)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG, "task not set up");
+    return t->cls->requisition(t, in, req, e);
+}
+
+unsigned ufomi_task_get_num_inputs(const UfomiTask *t) { return t ? 1 : 0; }
+
+unsigned ufomi_task_get_num_dimensions(const UfomiTask *t, unsigned input)
+{
+    (void) input;
+    if (!t)
+        return 0;
+    if (!strcmp(t->cls->name, "fft"))
+        return (int) t->v[FFT_DIMS].num > 2 ? 3 : 2;
+    if (!strcmp(t->cls->name, "ifft"))
+        return (int) t->v[IFFT_DIMS].num > 2 ? 3 : 2;
+    return 2;
+}
+
+unsigned ufomi_task_get_mode(const UfomiTask *t) { return t ? t->cls->mode : 0; }
+
+int ufomi_task_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    if (!t || !in || !in[0] || !out || !req || !t->ctx)
+        return 0;
+    t->err[0] = '\0';
+    t->err_status = 0;
+    const int ok = t->cls->process(t, in, out, req);
+    if (ok)
+        t->num_processed++;
+    return ok;
+}
+
+int ufomi_task_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
+{
+    if (!t || !out || !req || !t->cls->generate)
+        return 0;
+    return t->cls->generate(t, out, req);
+}
+
+UfomiBufferLayout ufomi_task_output_layout(const UfomiTask *t)
+{
+    return t && !strcmp(t->cls->name, "fft") ? UFOMI_LAYOUT_COMPLEX_INTERLEAVED : UFOMI_LAYOUT_REAL;
+}
+
+int ufomi_task_output_dtype(const UfomiTask *t)
+{
+    (void) t;
+    return UFOMI_F32;
+}
+
+/* ---------------------------------------------------------------- buffers */
+
+size_t ufomi_requisition_elements(const UfomiRequisition *r)
+{
+    size_t n = 1;
+    for (unsigned i = 0; i < r->n_dims && i < 3; i++)
+        n *= r->dims[i];
+    return n;
+}
+
+int ufomi_buffer_new(ufomi_ctx *ctx, const UfomiRequisition *req, int dtype, UfomiBuffer *out)
+{
+    memset(out, 0, sizeof(*out));
+    out->req = *req;
+    out->dtype = dtype;
+    const size_t bytes = ufomi_requisition_elements(req) * (dtype == UFOMI_F32 ? 4 : 2);
+    int rc = ufomi_device_alloc(ctx, bytes, &out->data);
+    if (!rc)
+        out->owned = 1;
+    return rc;
+}
+
+void ufomi_buffer_release(ufomi_ctx *ctx, UfomiBuffer *buf)
+{
+    if (buf && buf->owned && buf->data)
+        ufomi_device_free(ctx, buf->data);
+    if (buf) {
+        buf->data = NULL;
+        buf->owned = 0;
+    }
+}

@@ -1,0 +1,352 @@
+"""Python face of the C task layer (include/ufo_tasks.h), shaped like the
+gi.repository.Ufo API the reference's tests drive (tests/test_fft.py:57-112):
+
+    pm = PluginManager()
+    fft = pm.get_task("fft"); fft.props.size_x = 1024
+    graph = TaskGraph(); graph.connect_nodes(mem_in, fft); ...
+    Scheduler(ctx).run(graph)
+
+Scheduler runs a linear graph item by item through ufomi_task_get_requisition /
+process / generate, as ufo-core does; Pipeline runs the C batch runner
+(ufomi_pipeline_*) that fuses fft ! filter ! ifft.  Both go through libufomi.so
+on the GPU; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+import ufomi
+
+LAYOUT_REAL, LAYOUT_COMPLEX = 0, 1
+MODE_PROCESSOR, MODE_REDUCTOR, MODE_GENERATOR, MODE_GPU = 1, 2, 4, 16
+
+
+class Requisition(ctypes.Structure):
+    _fields_ = [("n_dims", ctypes.c_uint), ("dims", ctypes.c_size_t * 3)]
+
+
+class Buffer(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("req", Requisition), ("layout", ctypes.c_int), ("dtype", ctypes.c_int),
+                ("owned", ctypes.c_int)]
+
+
+class Error(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int), ("status", ctypes.c_int), ("message", ctypes.c_char * 512)]
+
+
+class TaskError(RuntimeError):
+    """GError of domain UFO_TASK_ERROR: .code (SETUP=0, GET_REQUISITION=1, PROCESS=2) and .status."""
+
+    def __init__(self, err):
+        super().__init__(err.message.decode())
+        self.code = err.code
+        self.status = err.status
+
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = ufomi.load()
+    if not _bound:
+        vp, ci, cu, cs = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_char_p
+        B, R, E = ctypes.POINTER(Buffer), ctypes.POINTER(Requisition), ctypes.POINTER(Error)
+        sig = {
+            "ufomi_task_new": ([cs], vp), "ufomi_task_free": ([vp], None), "ufomi_task_name": ([vp], cs),
+            "ufomi_task_set_property": ([vp, cs, cs], ci),
+            "ufomi_task_get_property": ([vp, cs, ctypes.c_char_p, ctypes.c_size_t], ci),
+            "ufomi_task_list_properties": ([vp], ctypes.POINTER(cs)),
+            "ufomi_task_setup": ([vp, vp, E], ci),
+            "ufomi_task_get_requisition": ([vp, ctypes.POINTER(B), R, E], ci),
+            "ufomi_task_get_num_inputs": ([vp], cu), "ufomi_task_get_num_dimensions": ([vp, cu], cu),
+            "ufomi_task_get_mode": ([vp], cu),
+            "ufomi_task_process": ([vp, ctypes.POINTER(B), B, R], ci),
+            "ufomi_task_generate": ([vp, B, R], ci),
+            "ufomi_task_output_layout": ([vp], ci), "ufomi_task_last_error": ([vp], cs),
+            "ufomi_pipeline_new": ([cs, E], vp), "ufomi_pipeline_free": ([vp], None),
+            "ufomi_pipeline_num_tasks": ([vp], ci), "ufomi_pipeline_task": ([vp, ci], vp),
+            "ufomi_pipeline_run": ([vp, vp, vp, R, ci, vp, ctypes.c_size_t, ctypes.POINTER(ci), R, E], ci),
+            "ufomi_pipeline_used_fused_filter": ([vp], ci),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _bound = True
+    return L
+
+
+TASK_EXPORTS = [
+    "ufomi_task_new", "ufomi_task_free", "ufomi_task_name", "ufomi_task_set_property", "ufomi_task_get_property",
+    "ufomi_task_list_properties", "ufomi_task_setup", "ufomi_task_get_requisition", "ufomi_task_get_num_inputs",
+    "ufomi_task_get_num_dimensions", "ufomi_task_get_mode", "ufomi_task_process", "ufomi_task_generate",
+    "ufomi_task_output_layout", "ufomi_task_output_dtype", "ufomi_task_last_error", "ufomi_buffer_new",
+    "ufomi_buffer_release", "ufomi_requisition_elements", "ufomi_pipeline_new", "ufomi_pipeline_free",
+    "ufomi_pipeline_num_tasks", "ufomi_pipeline_task", "ufomi_pipeline_run", "ufomi_pipeline_used_fused_filter",
+    "ufomi_device_alloc", "ufomi_device_free", "ufomi_memcpy", "ufomi_device_memory",
+]
+
+
+class _Props:
+    """Attribute access to task properties; names use '_' for '-' like PyGObject."""
+
+    def __init__(self, task):
+        object.__setattr__(self, "_task", task)
+
+    def __setattr__(self, name, value):
+        self._task.set_property(name.replace("_", "-"), value)
+
+    def __getattr__(self, name):
+        return self._task.get_property(name.replace("_", "-"))
+
+
+def _text(value):
+    if isinstance(value, bool):
+        return "true" if value else "false"
+    if isinstance(value, (list, tuple, np.ndarray)):
+        return ",".join(repr(float(v)) for v in value)
+    return str(value)
+
+
+class Task:
+    def __init__(self, name):
+        self.L = _lib()
+        self.h = self.L.ufomi_task_new(name.encode())
+        if not self.h:
+            raise ValueError("no task named %r" % name)
+        self.name = name
+        self.owned = True
+        self.props = _Props(self)
+
+    def __del__(self):
+        try:
+            if self.h and getattr(self, "owned", False):
+                self.L.ufomi_task_free(self.h)
+        except Exception:
+            pass
+
+    def set_property(self, name, value):
+        rc = self.L.ufomi_task_set_property(self.h, name.encode(), _text(value).encode())
+        if rc:
+            raise ufomi.UfomiError(rc, self.L.ufomi_last_error().decode())
+
+    def get_property(self, name):
+        buf = ctypes.create_string_buffer(1 << 16)
+        rc = self.L.ufomi_task_get_property(self.h, name.encode(), buf, len(buf))
+        if rc:
+            raise ufomi.UfomiError(rc, self.L.ufomi_last_error().decode())
+        s = buf.value.decode()
+        if s in ("true", "false"):
+            return s == "true"
+        try:
+            return int(s)
+        except ValueError:
+            pass
+        try:
+            return float(s)
+        except ValueError:
+            pass
+        if "," in s or s == "":
+            return [float(v) for v in s.split(",")] if s else []
+        return s
+
+    def list_properties(self):
+        arr = self.L.ufomi_task_list_properties(self.h)
+        out, i = [], 0
+        while arr[i]:
+            out.append(arr[i].decode())
+            i += 1
+        return out
+
+    @property
+    def mode(self):
+        return self.L.ufomi_task_get_mode(self.h)
+
+    def setup(self, ctx):
+        err = Error()
+        if self.L.ufomi_task_setup(self.h, ctx.h, ctypes.byref(err)):
+            raise TaskError(err)
+
+    def get_requisition(self, buf):
+        err, req = Error(), Requisition()
+        arr = (ctypes.POINTER(Buffer) * 1)(ctypes.pointer(buf))
+        if self.L.ufomi_task_get_requisition(self.h, arr, ctypes.byref(req), ctypes.byref(err)):
+            raise TaskError(err)
+        return req
+
+    def process(self, buf, out, req):
+        arr = (ctypes.POINTER(Buffer) * 1)(ctypes.pointer(buf))
+        ok = self.L.ufomi_task_process(self.h, arr, ctypes.byref(out), ctypes.byref(req))
+        if not ok:
+            raise RuntimeError("%s.process failed: %s" % (self.name, self.L.ufomi_task_last_error(self.h).decode()))
+
+    def generate(self, out, req):
+        return bool(self.L.ufomi_task_generate(self.h, ctypes.byref(out), ctypes.byref(req)))
+
+    def last_error(self):
+        return self.L.ufomi_task_last_error(self.h).decode()
+
+
+class MemoryIn:
+    """memory-in: numpy items [n][h][w] float32 (src/ufo-memory-in-task.c)."""
+
+    def __init__(self, array):
+        self.array = np.ascontiguousarray(array, dtype=np.float32)
+        if self.array.ndim == 2:
+            self.array = self.array[None]
+
+
+class MemoryOut:
+    """memory-out: collects every output item as a numpy array."""
+
+    def __init__(self):
+        self.items = []
+
+    @property
+    def array(self):
+        return np.stack(self.items) if self.items else np.zeros((0,))
+
+
+class PluginManager:
+    def get_task(self, name):
+        if name == "memory-out":
+            return MemoryOut()
+        return Task(name)
+
+
+class TaskGraph:
+    def __init__(self):
+        self.edges = []
+
+    def connect_nodes(self, a, b):
+        self.edges.append((a, b))
+
+    def chain(self):
+        src = [a for a, _ in self.edges if all(b is not a for _, b in self.edges)]
+        if len(src) != 1:
+            raise ValueError("only linear graphs are supported")
+        order, node = [src[0]], src[0]
+        nxt = {id(a): b for a, b in self.edges}
+        while id(node) in nxt:
+            node = nxt[id(node)]
+            order.append(node)
+        return order
+
+
+_TORCH_DT = {0: "float32", 1: "float16", 2: "bfloat16"}
+
+
+def _req_shape(req):
+    return tuple(int(req.dims[i]) for i in range(req.n_dims))[::-1]
+
+
+class Scheduler:
+    """Runs a linear graph item by item on one context (ufo-core's model)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def run(self, graph):
+        import torch
+        nodes = graph.chain()
+        src, sink, tasks = nodes[0], nodes[-1], nodes[1:-1]
+        for t in tasks:
+            t.setup(self.ctx)
+        keep = []
+
+        def make_buf(tensor, layout):
+            b = Buffer()
+            b.data = tensor.data_ptr()
+            b.req.n_dims = tensor.dim()
+            for i, s in enumerate(reversed(tensor.shape)):
+                b.req.dims[i] = s
+            b.layout = layout
+            b.dtype = 0
+            return b
+
+        def push(stage, tensor, layout):
+            """Send one item into tasks[stage:], returns nothing; outputs go to the sink."""
+            if stage == len(tasks):
+                torch.cuda.synchronize()
+                sink.items.append(tensor.cpu().numpy())
+                return
+            t = tasks[stage]
+            buf = make_buf(tensor, layout)
+            req = t.get_requisition(buf)
+            out = torch.empty(_req_shape(req), dtype=torch.float32, device=tensor.device)
+            obuf = make_buf(out, LAYOUT_REAL)
+            keep.append((tensor, out))
+            t.process(buf, obuf, req)
+            if not (t.mode & MODE_REDUCTOR):
+                push(stage + 1, out, obuf.layout)
+
+        def flush(stage):
+            for k in range(stage, len(tasks)):
+                t = tasks[k]
+                if t.mode & MODE_REDUCTOR:
+                    last = reducer_in[k]
+                    req = t.get_requisition(last)
+                    while True:
+                        out = torch.empty(_req_shape(req), dtype=torch.float32, device="cuda")
+                        obuf = make_buf(out, LAYOUT_REAL)
+                        if not t.generate(obuf, req):
+                            break
+                        push(k + 1, out, LAYOUT_REAL)
+
+        reducer_in = {}
+        dev = torch.device("cuda:%d" % self.ctx.device)
+        for item in src.array:
+            x = torch.from_numpy(item).to(dev)
+            buf = make_buf(x, LAYOUT_REAL)
+            for k, t in enumerate(tasks):
+                if t.mode & MODE_REDUCTOR:
+                    reducer_in[k] = buf
+            push(0, x, LAYOUT_REAL)
+        flush(0)
+        torch.cuda.synchronize()
+
+
+class Pipeline:
+    """The C batch runner: `ufo-launch`-style description over device items."""
+
+    def __init__(self, description):
+        self.L = _lib()
+        err = Error()
+        self.h = self.L.ufomi_pipeline_new(description.encode(), ctypes.byref(err))
+        if not self.h:
+            raise TaskError(err)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.ufomi_pipeline_free(self.h)
+        except Exception:
+            pass
+
+    def task(self, i):
+        t = Task.__new__(Task)
+        t.L, t.h, t.name, t.owned = self.L, self.L.ufomi_pipeline_task(self.h, i), None, False
+        t.props = _Props(t)
+        t.pipeline = self  # the pipeline owns the task; keep it alive
+        return t
+
+    @property
+    def used_fused_filter(self):
+        return bool(self.L.ufomi_pipeline_used_fused_filter(self.h))
+
+    def run(self, ctx, items, out_capacity_items=None, out_bytes=None):
+        """items: torch tensor [n][h][w] float32 on the device; returns (tensor of bytes, n_out, shape)."""
+        import torch
+        n, h, w = items.shape
+        req = Requisition()
+        req.n_dims = 2
+        req.dims[0], req.dims[1], req.dims[2] = w, h, 1
+        cap = out_bytes or (items.numel() * 4 * 8)
+        out = torch.empty(cap, dtype=torch.uint8, device=items.device)
+        n_out, oreq, err = ctypes.c_int(), Requisition(), Error()
+        rc = self.L.ufomi_pipeline_run(self.h, ctx.h, items.data_ptr(), ctypes.byref(req), n, out.data_ptr(), cap,
+                                       ctypes.byref(n_out), ctypes.byref(oreq), ctypes.byref(err))
+        if rc:
+            raise TaskError(err)
+        return out, n_out.value, _req_shape(oreq)
