@@ -8,6 +8,7 @@
 // single LDS buffer of N complex values per transform suffices.
 // Twiddles come from the context's FP64-accurate table (read through L2).
 #include "ufomi_internal.h"
+#include "fft16.h"
 
 namespace {
 
@@ -252,6 +253,136 @@ int launch_by_b(int B, K k)
     }
 }
 
+// ------------------------------------------------------------------------
+// Register radix-16 kernels (fft16.h) for 16 <= N <= 8192.  The kernels above
+// remain for N = 4 and 8.
+// ------------------------------------------------------------------------
+
+template <int N, typename OutT>
+__global__ __launch_bounds__(kThreads) void ramp16_kernel(
+    const float *__restrict__ in, int W, long rows, const float *__restrict__ coef, const int *__restrict__ pair_flag,
+    int crop, OutT *__restrict__ out, float scale, const float2 *__restrict__ tw)
+{
+    using P = fft16::Plan<N>;
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NPASS - 1);
+    extern __shared__ float2 lds[];
+    const int f = threadIdx.x / P::TF, t = threadIdx.x % P::TF;
+    float2 *x = lds + (size_t) f * N;
+    const bool pair = *pair_flag != 0;
+    if (pair && 2 * (long) blockIdx.x * P::G >= rows)
+        return;  // whole block beyond the last pair
+    const long unit = (long) blockIdx.x * P::G + f;
+    const long r0 = pair ? 2 * unit : unit, r1 = r0 + 1;
+    const bool v0 = r0 < rows, v1 = pair && r1 < rows;
+    const float *a = in + r0 * W, *b = in + r1 * W;
+
+    float2 v[P::V];
+#pragma unroll
+    for (int s = 0; s < P::V; s++) {
+        const int e = fft16::slot_index<N, R0>(t, s);
+        float re = 0.0f, im = 0.0f;
+        if (e < W) {
+            if (v0) re = a[e];
+            if (v1) im = b[e];
+        }
+        v[s] = make_float2(re, im);
+    }
+    fft16::transform_forward_order<N, -1>(v, x, t, tw);
+#pragma unroll
+    for (int s = 0; s < P::V; s++) {
+        const int e = fft16::slot_index<N, RL>(t, s);
+        const float cr = coef[2 * e];
+        const float ci = pair ? cr : coef[2 * e + 1];
+        v[s] = make_float2(v[s].x * cr, v[s].y * ci);
+    }
+    fft16::transform_reverse_order<N, +1>(v, x, t, tw);
+#pragma unroll
+    for (int s = 0; s < P::V; s++) {
+        const int e = fft16::slot_index<N, R0>(t, s);
+        if (e < crop) {
+            if (v0) ufomi_store(out, r0 * crop + e, v[s].x * scale);
+            if (v1) ufomi_store(out, r1 * crop + e, v[s].y * scale);
+        }
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void fft16_rows_kernel(const float *__restrict__ in, int W, long rows,
+                                                              float2 *__restrict__ out, const float2 *__restrict__ tw)
+{
+    using P = fft16::Plan<N>;
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NPASS - 1);
+    extern __shared__ float2 lds[];
+    const int f = threadIdx.x / P::TF, t = threadIdx.x % P::TF;
+    float2 *x = lds + (size_t) f * N;
+    const long r = (long) blockIdx.x * P::G + f;
+    const bool valid = r < rows;
+    const float *a = in + r * W;
+    float2 v[P::V];
+#pragma unroll
+    for (int s = 0; s < P::V; s++) {
+        const int e = fft16::slot_index<N, R0>(t, s);
+        v[s] = make_float2((valid && e < W) ? a[e] : 0.0f, 0.0f);
+    }
+    fft16::transform_forward_order<N, -1>(v, x, t, tw);
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < P::V; s++)
+            out[r * N + fft16::slot_index<N, RL>(t, s)] = v[s];
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void ifft16_rows_kernel(const float2 *__restrict__ in, long rows, int crop,
+                                                               float *__restrict__ out, float scale,
+                                                               const float2 *__restrict__ tw)
+{
+    using P = fft16::Plan<N>;
+    constexpr int R0 = P::radix(0), RL = P::radix(P::NPASS - 1);
+    extern __shared__ float2 lds[];
+    const int f = threadIdx.x / P::TF, t = threadIdx.x % P::TF;
+    float2 *x = lds + (size_t) f * N;
+    const long r = (long) blockIdx.x * P::G + f;
+    const bool valid = r < rows;
+    float2 v[P::V];
+#pragma unroll
+    for (int s = 0; s < P::V; s++)
+        v[s] = valid ? in[r * N + fft16::slot_index<N, RL>(t, s)] : make_float2(0.0f, 0.0f);
+    fft16::transform_reverse_order<N, +1>(v, x, t, tw);
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < P::V; s++) {
+            const int e = fft16::slot_index<N, R0>(t, s);
+            if (e < crop)
+                out[r * crop + e] = v[s].x * scale;
+        }
+    }
+}
+
+template <typename K>
+int launch_by_n(int N, K k)
+{
+    switch (N) {
+    case 16: return k(std::integral_constant<int, 16>());
+    case 32: return k(std::integral_constant<int, 32>());
+    case 64: return k(std::integral_constant<int, 64>());
+    case 128: return k(std::integral_constant<int, 128>());
+    case 256: return k(std::integral_constant<int, 256>());
+    case 512: return k(std::integral_constant<int, 512>());
+    case 1024: return k(std::integral_constant<int, 1024>());
+    case 2048: return k(std::integral_constant<int, 2048>());
+    case 4096: return k(std::integral_constant<int, 4096>());
+    case 8192: return k(std::integral_constant<int, 8192>());
+    default: return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "no radix-16 FFT kernel for N = %d", N);
+    }
+}
+
+template <int N> size_t lds16_bytes()
+{
+    using P = fft16::Plan<N>;
+    return P::NPASS > 1 ? (size_t) P::G * N * sizeof(float2) : 0;
+}
+
 }  // namespace
 
 extern "C" int ufomi_fft_rows(ufomi_ctx *ctx, const float *d_in, int W, int P, int N, float *d_out)
@@ -263,6 +394,16 @@ extern "C" int ufomi_fft_rows(ufomi_ctx *ctx, const float *d_in, int W, int P, i
         return ufomi_fail(UFOMI_ERR_ARG, "invalid fft arguments");
     if (N < W)
         return ufomi_fail(UFOMI_ERR_SIZE, "Specified size must be greater than or equal to input size");
+    if (N >= 16)
+        return launch_by_n(N, [&](auto n) {
+            constexpr int NN = decltype(n)::value;
+            using PL = fft16::Plan<NN>;
+            const unsigned blocks = (unsigned) (((long) P + PL::G - 1) / PL::G);
+            hipLaunchKernelGGL(fft16_rows_kernel<NN>, dim3(blocks), dim3(kThreads), lds16_bytes<NN>(), ctx->stream,
+                               d_in, W, (long) P, (float2 *) d_out, ctx->twiddles);
+            UFOMI_CHECK_LAUNCH();
+            return (int) UFOMI_OK;
+        });
     int B, G;
     size_t lds;
     fft_geometry(N, &B, &G, &lds);
@@ -286,11 +427,21 @@ extern "C" int ufomi_ifft_rows(ufomi_ctx *ctx, const float *d_in, int N, int P, 
     if (crop > N)
         return ufomi_fail(UFOMI_ERR_SIZE, "Cropped size must be less than or equal to input size");
     if (crop == 0) crop = N;
+    const float scale = 1.0f / (float) N;  // fft_pack scale of ufo-ifft-task.c:296-303
+    if (N >= 16)
+        return launch_by_n(N, [&](auto n) {
+            constexpr int NN = decltype(n)::value;
+            using PL = fft16::Plan<NN>;
+            const unsigned blocks = (unsigned) (((long) P + PL::G - 1) / PL::G);
+            hipLaunchKernelGGL(ifft16_rows_kernel<NN>, dim3(blocks), dim3(kThreads), lds16_bytes<NN>(), ctx->stream,
+                               (const float2 *) d_in, (long) P, crop, d_out, scale, ctx->twiddles);
+            UFOMI_CHECK_LAUNCH();
+            return (int) UFOMI_OK;
+        });
     int B, G;
     size_t lds;
     fft_geometry(N, &B, &G, &lds);
     const unsigned blocks = (unsigned) ((P + G - 1) / G);
-    const float scale = 1.0f / (float) N;  // fft_pack scale of ufo-ifft-task.c:296-303
     return launch_by_b(B, [&](auto b) {
         constexpr int BB = decltype(b)::value;
         hipLaunchKernelGGL(ifft_rows_kernel<BB>, dim3(blocks), dim3(kThreads), lds, ctx->stream,
@@ -339,11 +490,30 @@ extern "C" int ufomi_ramp_filter(ufomi_ctx *ctx, const float *d_in, int W, int P
     hipLaunchKernelGGL(classify_kernel, dim3(1), dim3(kThreads), 0, ctx->stream, d_coef, N, d_flag);
     UFOMI_CHECK_LAUNCH();
 
+    const float scale = 1.0f / (float) N;
+    if (N >= 16) {
+        auto go16 = [&](auto n, auto *out) {
+            constexpr int NN = decltype(n)::value;
+            using OutT = std::remove_pointer_t<decltype(out)>;
+            using PL = fft16::Plan<NN>;
+            const unsigned blocks = (unsigned) (((long) P + PL::G - 1) / PL::G);
+            hipLaunchKernelGGL((ramp16_kernel<NN, OutT>), dim3(blocks), dim3(kThreads), lds16_bytes<NN>(),
+                               ctx->stream, d_in, W, (long) P, d_coef, d_flag, crop, out, scale, ctx->twiddles);
+            UFOMI_CHECK_LAUNCH();
+            return (int) UFOMI_OK;
+        };
+        return launch_by_n(N, [&](auto n) {
+            switch (out_dtype) {
+            case UFOMI_F16: return go16(n, (__half *) d_out);
+            case UFOMI_BF16: return go16(n, (__hip_bfloat16 *) d_out);
+            default: return go16(n, (float *) d_out);
+            }
+        });
+    }
     int B, G;
     size_t lds;
     fft_geometry(N, &B, &G, &lds);
     const unsigned blocks = (unsigned) (((long) P + G - 1) / G);
-    const float scale = 1.0f / (float) N;
     const int log2N = ufomi_log2(N);
     auto go = [&](auto b, auto *out) {
         constexpr int BB = decltype(b)::value;
