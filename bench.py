@@ -1,20 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark of the FBP hot path on one MI355X: batched parallel-beam filtered
-backprojection at scale.
+"""Benchmark of the FBP hot path on one MI355X.
 
-Workload (default): 2048 sinograms of 1800 angles x 2048 detector pixels from a
-seeded 3-D phantom of 20 ellipsoids (analytic plane-section line integrals),
-filtered with the Shepp-Logan-windowed ramp zero-padded to 4096 (fused
-fft -> filter -> ifft kernel) and backprojected into a 2048^3 float32 volume.
-One step = ramp filter of all 3,686,400 rows + backprojection of all 2048
-slices; inputs are resident in HBM before the timed region.  value = voxel
-updates per second (GUPS) over the whole step.
+Default workload (the headline metric): batched parallel-beam FBP at scale.
+2048 sinograms of 1800 angles x 2048 detector pixels from a seeded 3-D phantom
+of 20 ellipsoids (analytic plane-section line integrals) are filtered with the
+Shepp-Logan-windowed ramp zero-padded to 4096 (fused fft -> filter -> ifft
+kernel) and backprojected into a 2048^3 float32 volume.  One step = ramp
+filter of all 3,686,400 rows + backprojection of all 2048 slices; inputs are
+resident in HBM before the timed region.  value = voxel updates per second
+(GUPS) over the whole step.
 
-Prints one JSON line (rank 0).  Also measures, inside the timed region, the
-average duration of the dominant kernel (the backprojector) with HIP events
-on the stream it runs on, and expresses it against the LDS read bandwidth the
-device sustains (measured here by a microbenchmark); the CPU baseline is the
-oracle (C, OpenMP) on one of the 2048 slices.
+Other workloads (--workload):
+  fbp512  one 512^2 slice, 512 angles, 1% noise, Ram-Lak zero-padded to 1024
+  cone    FDK: cosine weight + ramp (N 2048) + cone-beam backprojection of 720
+          projections of 1024^2 into 1024^3 (SOD 2500 px, SDD 5000 px)
+  lamino  ramp (N 2048) + laminographic backprojection, tilt 30 deg, 1500
+          projections of 1024^2 into 1024^2 x 256
+
+Prints one JSON line.  The dominant kernel's average duration is measured
+inside the timed region with HIP events on the stream it runs on and set
+against the LDS read bandwidth the device sustains (microbenchmark); the HBM
+traffic comes from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+the same workload (profiles/traffic.json) when there is one.  The CPU baseline
+is the oracle (C, OpenMP) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -27,8 +35,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ufo-filters_amd"))
 
 MI355X_HBM_TBPS = 8.0
-MI355X_CUS = 256
-MI355X_CLOCK_GHZ = 2.4
 
 
 def log(msg):
@@ -62,20 +68,198 @@ def measure_hbm_copy(ctx, torch):
     return bw
 
 
-def cpu_baseline(sinos, n_fft, axis, threads):
-    """Oracle (C, float32 reference semantics, OpenMP) on a few slices: ramp chain + BP."""
-    import numpy as np
+def traffic_for(key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    entry = table.get(key)
+    return None if entry is None else entry["fetch_bytes"] + entry["write_bytes"]
+
+
+def oracle_module(threads):
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    n, p, w = sinos.shape
-    coef = oracle.filter_coefficients(2 * n_fft, "ramp-fromreal")
+    return oracle
+
+
+def run_timed(torch, steps, warmup, step_fn, n_events):
+    """warmup untimed steps, then `steps` timed steps; returns wall seconds and per-step event lists."""
+    for i in range(warmup):
+        step_fn(None)
+        torch.cuda.synchronize()
+        log("warmup step %d done" % i)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_events)] for _ in range(steps)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in sinos:
-        filt = oracle.ramp_chain(s, n_fft, coef, w).astype(np.float32)
-        oracle.bp_texture_ref32(filt, axis_pos=axis)
-    dt = time.perf_counter() - t0
-    return (n * w * w * p) / dt / 1e9, dt
+    for i in range(steps):
+        step_fn(ev[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ev
+
+
+def mean_ms(ev, a, b):
+    return sum(e[a].elapsed_time(e[b]) for e in ev) / len(ev)
+
+
+# ----------------------------------------------------------------- workloads
+
+def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage):
+    """Batched parallel-beam FBP (the default workload at S=2048, W=2048, P=1800)."""
+    dev = torch.device("cuda:0")
+    N = 1
+    while N < 2 * W:
+        N *= 2
+    axis = W / 2.0
+    step_angle = math.pi / P
+    dtype = {"f32": ufomi.F32, "f16": ufomi.F16, "bf16": ufomi.BF16}[storage]
+    tdtype = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[storage]
+
+    log("generating %d sinograms %dx%d" % (S, P, W))
+    sino = torch.empty((S, P, W), device=dev)
+    if S == 1:
+        ell = ufomi.phantom_ellipses(1000, 10, W / 2.0)
+        ctx.sinogram_ellipses(ell, W, P, axis, step_angle, sino)
+    else:
+        ellipsoids = ufomi.phantom_ellipsoids(1001, 20, W / 2.0)
+        chunk = 256
+        for s0 in range(0, S, chunk):
+            n = min(chunk, S - s0)
+            ells = np.stack([ufomi.slice_ellipsoids(ellipsoids, (s0 + k) + 0.5 - S / 2.0) for k in range(n)])
+            ctx.sinogram_ellipses(ells, W, P, axis, step_angle, sino[s0:s0 + n])
+    if noise:
+        ctx.add_noise(sino, noise, 1000)
+    coef = torch.from_numpy(ufomi.filter_coefficients(2 * N, filt_name)).to(dev)
+    filt = torch.empty((S, P, W), dtype=tdtype, device=dev)
+    vol = torch.empty((S, W, W), device=dev)
+    torch.cuda.synchronize()
+    log("inputs ready (%.2f GB sinograms, %.2f GB volume)" % (sino.numel() * 4 / 1e9, vol.numel() * 4 / 1e9))
+
+    def step(e):
+        if e: e[0].record()
+        ctx.ramp_filter(sino, W, S * P, N, coef, W, filt, dtype)
+        if e: e[1].record()
+        ctx.bp_parallel(filt, W, P, vol, n_slices=S, dtype=dtype, axis_pos=axis)
+        if e: e[2].record()
+
+    wall, ev = run_timed(torch, args.steps, args.warmup, step, 3)
+    filt_ms, bp_ms = mean_ms(ev, 0, 1), mean_ms(ev, 1, 2)
+    log("timed %d steps: %.3f s wall; filter %.2f ms, backprojection %.2f ms per step"
+        % (args.steps, wall, filt_ms, bp_ms))
+    updates = float(S) * W * W * P
+    sample_bytes = 4 if storage == "f32" else 2
+    kernel = "bp_texture_kernel<%d,%s>" % (4 if S >= 4 else (2 if S >= 2 else 1),
+                                           {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[storage])
+    extra = {
+        "filter_kernel_ms": round(filt_ms, 3),
+        "filter_hbm_GBps": round(S * P * W * (4 + sample_bytes) / (filt_ms * 1e-3) / 1e9, 1),
+        "filter_hbm_frac_of_8TBps": round(S * P * W * (4 + sample_bytes) / (filt_ms * 1e-3) / 8e12, 4),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        n_cpu = min(8 if S > 1 else 1, S)
+        some = sino[S // 2:S // 2 + n_cpu].cpu().numpy()
+        del sino, filt, vol
+        oracle = oracle_module(threads)
+        log("cpu baseline: oracle on %d slices (%d threads)" % (n_cpu, threads))
+        coef_o = oracle.filter_coefficients(2 * N, "ramp-fromreal")
+        reps = 1 if S > 1 else 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for s in some:
+                f32 = oracle.ramp_chain(s, N, coef_o, W).astype(np.float32)
+                oracle.bp_texture_ref32(f32, axis_pos=axis)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(reps * n_cpu * W * W * P / dt / 1e9, 3), "unit": "GUPS", "cores": threads,
+               "kind": "port",
+               "sample": "%d of %d slices (%dx%d px, %d angles each)%s: FP64 fft-filter-ifft + float32-semantics "
+                         "backprojection of the C oracle, %.1f s" % (n_cpu, S, W, W, P,
+                                                                     " x %d repeats" % reps if reps > 1 else "", dt)}
+    workload = ("parallel-beam FBP: %d sinogram%s x %d angles x %d px -> %dx%dx%d volume, %s ramp zero-padded "
+                "to %d, %s filtered-sinogram storage" % (S, "s" if S > 1 else "", P, W, S, W, W, filt_name, N, storage))
+    return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=2 * 4, extra=extra,
+                cpu=cpu, workload=workload, dtype="f32" if storage == "f32" else "f32 accumulate, %s storage" % storage,
+                config={"slices": S, "angles": P, "detector": W, "fft_size": N, "storage": storage, "filter": filt_name},
+                traffic_key="%s %dx%dx%d %s" % (kernel, S, P, W, storage),
+                metric="GUPS (voxel*proj/s) for FBP")
+
+
+def general(args, ctx, torch, np, ufomi, kind):
+    """Cone-beam FDK (kind='cone') or laminography (kind='lamino') through general-backproject."""
+    dev = torch.device("cuda:0")
+    W = H = 1024
+    N = 2048
+    if kind == "cone":
+        P, nz = 720, 1024
+        sod = 2500.0
+        geom = dict(source_position_y=[-sod], detector_position_y=[sod], center_position_x=[W / 2.0],
+                    center_position_z=[H / 2.0])
+    else:
+        P, nz = 1500, 256
+        geom = dict(axis_angle_x=[math.radians(30.0)], center_position_x=[W / 2.0], center_position_z=[H / 2.0])
+    params = ufomi.gbp_params(P, region=(-nz / 2.0, nz / 2.0, 1.0))
+    g = ufomi.gbp_geometry(**geom)
+    rows, _ = ufomi.gbp_projection_rows(g, params)
+    ell = ufomi.phantom_ellipsoids(1003, 20, W / 2.0 * (0.9 if kind == "cone" else 0.5))
+    if kind == "lamino":
+        ell[:, 2] *= 0.25  # a flat sample, as laminography expects
+        ell[:, 5] *= 0.25
+    projs = torch.empty((P, H, W), device=dev)
+    ctx.project_ellipsoids(ell, W, H, rows, projs)
+    weighted = torch.empty_like(projs)
+    filt = torch.empty_like(projs)
+    coef = torch.from_numpy(ufomi.filter_coefficients(2 * N, "ramp-fromreal")).to(dev)
+    vol = torch.empty((nz, W, W), device=dev)
+    torch.cuda.synchronize()
+    log("%s: %d projections %dx%d -> volume %dx%dx%d" % (kind, P, H, W, nz, W, W))
+
+    def step(e):
+        if e: e[0].record()
+        src = projs
+        if kind == "cone":
+            ctx.cone_weight(projs, weighted, W, H, P, W / 2.0, H / 2.0, sod, sod, 0.0)
+            src = weighted
+        ctx.ramp_filter(src, W, P * H, N, coef, W, filt)
+        if e: e[1].record()
+        ctx.gbp(filt, W, H, P, g, params, vol)
+        if e: e[2].record()
+
+    wall, ev = run_timed(torch, args.steps, args.warmup, step, 3)
+    pre_ms, bp_ms = mean_ms(ev, 0, 1), mean_ms(ev, 1, 2)
+    log("timed %d steps: %.3f s wall; weight+filter %.2f ms, backprojection %.2f ms per step"
+        % (args.steps, wall, pre_ms, bp_ms))
+    updates = float(nz) * W * W * P
+    mode = 1 if kind == "cone" else 0
+    kernel = "gbp_kernel<%d,2,float>" % mode
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        oracle = oracle_module(threads)
+        sub = 8 if kind == "cone" else 4
+        f_np = filt.cpu().numpy()
+        del projs, weighted, filt, vol
+        x_reg = (-W / 2.0, -W / 2.0 + 64, 1.0)
+        t0 = time.perf_counter()
+        oracle.gbp(f_np, overall_angle=2 * math.pi, region=(0.0, float(sub), 1.0), x_region=x_reg,
+                   y_region=x_reg, mode="ref32", **geom)
+        dt = time.perf_counter() - t0
+        n_up = sub * 64 * 64 * P
+        cpu = {"value": round(n_up / dt / 1e9, 4), "unit": "GUPS", "cores": threads, "kind": "port",
+               "sample": "64x64x%d voxels, all %d projections, float32-semantics general backprojection of the "
+                         "C oracle, %.1f s" % (sub, P, dt)}
+    desc = ("cone-beam FDK: %d projections of %dx%d, SOD %.0f px, SDD %.0f px, cosine weight + Ram-Lak (N %d) -> "
+            "%dx%dx%d volume" % (P, H, W, sod, 2 * sod, N, nz, W, W) if kind == "cone" else
+            "laminography: %d projections of %dx%d, tilt 30 deg, Ram-Lak (N %d) -> %dx%dx%d volume"
+            % (P, H, W, N, nz, W, W))
+    return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=4 * 4,
+                extra={"weight_filter_ms": round(pre_ms, 3)}, cpu=cpu, workload=desc, dtype="f32",
+                config={"projections": P, "detector": [H, W], "volume": [nz, W, W], "fft_size": N},
+                traffic_key="%s %s" % (kernel, kind),
+                metric="GUPS (voxel*proj/s) for %s" % ("FDK cone-beam" if kind == "cone" else "laminography"))
 
 
 def main():
@@ -83,9 +267,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["fbp2048", "fbp512", "cone", "lamino"], default="fbp2048")
     ap.add_argument("--slices", type=int, default=2048)
-    ap.add_argument("--width", type=int, default=2048)
-    ap.add_argument("--angles", type=int, default=1800)
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -100,129 +283,57 @@ def main():
     import torch
     import ufomi
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
+    torch.cuda.set_device(0)
     ctx = ufomi.Context(0)
     ctx.set_stream(torch.cuda.current_stream(0))
 
-    S, W, P = args.slices, args.width, args.angles
-    N = 1
-    while N < 2 * W:
-        N *= 2
-    axis = W / 2.0
-    step_angle = math.pi / P
-    dtype = {"f32": ufomi.F32, "f16": ufomi.F16, "bf16": ufomi.BF16}[args.storage]
-    tdtype = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[args.storage]
+    if args.workload == "fbp2048":
+        r = fbp_parallel(args, ctx, torch, np, ufomi, args.slices, 2048, 1800, "shepp-logan", 0.0, args.storage)
+    elif args.workload == "fbp512":
+        r = fbp_parallel(args, ctx, torch, np, ufomi, 1, 512, 512, "ramp-fromreal", 0.01, args.storage)
+    else:
+        r = general(args, ctx, torch, np, ufomi, args.workload)
 
-    # ---- synthetic input, resident in HBM
-    log("generating %d sinograms %dx%d from 20 seeded ellipsoids" % (S, P, W))
-    ellipsoids = ufomi.phantom_ellipsoids(1001, 20, W / 2.0)
-    sino = torch.empty((S, P, W), device=dev)
-    chunk = 256
-    for s0 in range(0, S, chunk):
-        n = min(chunk, S - s0)
-        ells = np.stack([ufomi.slice_ellipsoids(ellipsoids, (s0 + k) + 0.5 - S / 2.0) for k in range(n)])
-        ctx.sinogram_ellipses(ells, W, P, axis, step_angle, sino[s0:s0 + n])
-    coef = torch.from_numpy(ufomi.filter_coefficients(2 * N, "shepp-logan")).to(dev)
-    filt = torch.empty((S, P, W), dtype=tdtype, device=dev)
-    vol = torch.empty((S, W, W), device=dev)
-    torch.cuda.synchronize()
-    log("inputs ready (%.1f GB sinograms, %.1f GB volume)" % (sino.numel() * 4 / 1e9, vol.numel() * 4 / 1e9))
-
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-
-    def step(events=None):
-        if events:
-            events[0].record()
-        ctx.ramp_filter(sino, W, S * P, N, coef, W, filt, dtype)
-        if events:
-            events[1].record()
-        ctx.bp_parallel(filt, W, P, vol, n_slices=S, dtype=dtype, axis_pos=axis)
-        if events:
-            events[2].record()
-
-    for i in range(args.warmup):
-        step()
-        torch.cuda.synchronize()
-        log("warmup step %d done" % i)
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(ev[i])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    filt_ms = [ev[i][0].elapsed_time(ev[i][1]) for i in range(args.steps)]
-    bp_ms = [ev[i][1].elapsed_time(ev[i][2]) for i in range(args.steps)]
-    log("timed %d steps: %.3f s wall; filter %.2f ms, backprojection %.1f ms per step"
-        % (args.steps, wall, float(np.mean(filt_ms)), float(np.mean(bp_ms))))
-
-    updates = float(S) * W * W * P
-    ms_per_step = wall / args.steps * 1e3
-    gups = updates / (wall / args.steps) / 1e9
-
-    # ---- roofline of the dominant kernel (backprojector): 2 taps x sample bytes per update
     lds_peak = measure_lds_peak(ctx, torch)
     hbm_copy = measure_hbm_copy(ctx, torch)
-    sample_bytes = 4 if args.storage == "f32" else 2
-    bp_s = float(np.mean(bp_ms)) * 1e-3
-    achieved = updates * 2 * 4 / bp_s  # LDS bytes: the interpolation reads (v, dv) float32 pairs
-    filt_s = float(np.mean(filt_ms)) * 1e-3
-    filt_bytes = float(S) * P * W * (4 + sample_bytes)
-    log("backprojector %.1f GUPS, LDS gather %.1f TB/s of %.1f TB/s measured peak; "
-        "filter %.2f TB/s HBM (algorithmic) of %.2f TB/s copy"
-        % (updates / bp_s / 1e9, achieved / 1e12, lds_peak / 1e12, filt_bytes / filt_s / 1e12, hbm_copy / 1e12))
-
+    bp_s = r["bp_ms"] * 1e-3
+    achieved = r["updates"] * r["bytes_per_update"] / bp_s
+    traffic = traffic_for(r["traffic_key"])
+    log("%s: %.1f GUPS kernel, gather %.1f TB/s vs %.1f TB/s LDS peak, HBM copy %.2f TB/s"
+        % (r["kernel"], r["updates"] / bp_s / 1e9, achieved / 1e12, lds_peak / 1e12, hbm_copy / 1e12))
+    roof = {
+        "kernel": r["kernel"],
+        "bound": "lds",
+        "achieved": round(achieved / 1e9, 1),
+        "peak": round(lds_peak / 1e9, 1),
+        "unit": "GB/s",
+        "frac": round(achieved / lds_peak, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_update": r["bytes_per_update"],
+        "kernel_ms": round(r["bp_ms"], 3),
+        "kernel_gups": round(r["updates"] / bp_s / 1e9, 1),
+        "peak_source": "measured: ufomi_bench_lds_read (ds_read_b128 on every CU)",
+        "hbm_copy_GBps": round(hbm_copy / 1e9, 1),
+    }
+    roof.update(r["extra"])
     result = {
-        "metric": "GUPS (voxel*proj/s) for FBP",
-        "value": round(gups, 2),
+        "metric": r["metric"],
+        "value": round(r["updates"] / (r["wall"] / args.steps) / 1e9, 2),
         "unit": "GUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.storage == "f32" else "f32 accumulate, %s storage" % args.storage,
-        "data": "synthetic: seeded 20-ellipsoid phantom, analytic parallel-beam sinograms",
-        "config": {
-            "workload": "batched parallel-beam FBP: %d sinograms x %d angles x %d px -> %dx%dx%d volume, "
-                        "Shepp-Logan-windowed ramp zero-padded to %d, %s filtered-sinogram storage"
-                        % (S, P, W, S, W, W, N, args.storage),
-            "slices": S, "angles": P, "detector": W, "fft_size": N, "storage": args.storage,
-        },
-        "roofline": {
-            "kernel": "bp_texture_kernel (4 slices per workgroup)",
-            "bound": "lds",
-            "achieved": round(achieved / 1e9, 1),
-            "peak": round(lds_peak / 1e9, 1),
-            "unit": "GB/s",
-            "frac": round(achieved / lds_peak, 4),
-            "traffic": None,
-            "algorithmic_bytes_per_update": 8,
-            "kernel_ms": round(bp_s * 1e3, 2),
-            "kernel_gups": round(updates / bp_s / 1e9, 1),
-            "peak_source": "measured: ufomi_bench_lds_read (ds_read_b128, all CUs)",
-            "hbm_copy_GBps": round(hbm_copy / 1e9, 1),
-            "filter_kernel_ms": round(filt_s * 1e3, 2),
-            "filter_hbm_GBps": round(filt_bytes / filt_s / 1e9, 1),
-            "filter_hbm_frac_of_8TBps": round(filt_bytes / filt_s / (MI355X_HBM_TBPS * 1e12), 4),
-        },
+        "dtype": r["dtype"],
+        "data": "synthetic: seeded ellipse/ellipsoid phantoms, analytic projections",
+        "config": dict({"workload": r["workload"]}, **r["config"]),
+        "roofline": roof,
     }
-
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        n_cpu = min(8, S)
-        some = sino[S // 2:S // 2 + n_cpu].cpu().numpy()
-        del sino, filt, vol
-        log("cpu baseline: oracle on %d slices (%d threads)" % (n_cpu, threads))
-        cg, dt = cpu_baseline(some, N, axis, threads)
-        result["cpu_baseline"] = {
-            "value": round(cg, 3), "unit": "GUPS", "cores": threads, "kind": "port",
-            "sample": "%d of %d slices (%dx%d px, %d angles each): FP64 fft-filter-ifft + float32-semantics "
-                      "backprojection of the C oracle, %.1f s" % (n_cpu, S, W, W, P, dt),
-        }
+    if r["cpu"] is not None:
+        result["cpu_baseline"] = r["cpu"]
     print(json.dumps(result), flush=True)
     ctx.close()
 

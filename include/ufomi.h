@@ -202,6 +202,16 @@ typedef struct {
 void ufomi_gbp_params_init(ufomi_gbp_params *p);
 void ufomi_gbp_geometry_init(ufomi_gbp_geometry *g);
 
+/* The per-projection maps the general backprojector applies, composed in FP64
+ * from the transformation chain of make_static_transformations /
+ * make_transformations (ufo-general-backproject-task.c:807-1048): 16 doubles
+ * per projection, rows U, V, D, C over X = (x, y, z, 1) with u = U.X / D.X,
+ * v = V.X / D.X (detector coordinates including the centre) and the FDK
+ * factor coeff = 1 / C.X.  *parallel_beam = 1 when every source-position-y is
+ * infinite (then D = C = (0, 0, 0, 1)). */
+int ufomi_gbp_projection_rows(const ufomi_gbp_geometry *geometry, const ufomi_gbp_params *params, double *rows,
+                              int *parallel_beam);
+
 /* Volume dimensions the task's get_requisition would produce (:1464-1587). */
 int ufomi_gbp_volume_shape(const ufomi_gbp_params *p, int W, int *nx, int *ny, int *nz);
 
@@ -235,6 +245,11 @@ int ufomi_slice_ellipsoids(const double *ellipsoids, int n, double z, double *el
  * angle p * angle_step; FP64 line integrals rounded to float. */
 int ufomi_sinogram_ellipses(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, int n_slices,
                             int W, int P, double axis_pos, double angle_step, float *d_sino);
+/* P projections [H][W] of n ellipsoids for the geometry given by
+ * ufomi_gbp_projection_rows (texel (i, j) sees the ray through u = i + 0.5,
+ * v = j + 0.5): density times chord length in voxel units, FP64. */
+int ufomi_project_ellipsoids(ufomi_ctx *ctx, const double *ellipsoids, int n_ellipsoids, int W, int H, int P,
+                             const double *rows, float *d_projections);
 /* Adds N(0, (rel_sigma * max|d|)^2) noise from a counter-based generator;
  * returns max|d| before the noise in *max_out (may be NULL). */
 int ufomi_add_noise(ufomi_ctx *ctx, float *d, size_t n, double rel_sigma, uint64_t seed, float *max_out);

@@ -181,6 +181,47 @@ def test_phantom_sinogram_vs_numpy(ctx, torch_dev):
     np.testing.assert_allclose(sec2[0, 2:4], np.array([10.0, 6.0]) * math.sqrt(0.75))
 
 
+def test_projector_sphere_closed_form(ctx, torch_dev):
+    """Chord lengths through a centred sphere: 2 rho sqrt(R^2 - r^2) for parallel rays;
+    the central cone ray passes through the centre (2 rho R)."""
+    import torch
+    W = H = 33
+    R, rho = 10.0, 0.7
+    sphere = np.array([[0.0, 0.0, 0.0, R, R, R, 0.0, rho]])
+    params = ufomi.gbp_params(3)
+    g = ufomi.gbp_geometry(center_position_x=[16.5], center_position_z=[16.5])
+    rows, par = ufomi.gbp_projection_rows(g, params)
+    assert par
+    out = torch.empty((3, H, W), device=torch_dev)
+    ctx.project_ellipsoids(sphere, W, H, rows, out)
+    u = np.arange(W) + 0.5 - 16.5
+    r2 = u[None, :] ** 2 + u[:, None] ** 2
+    ref = 2 * rho * np.sqrt(np.maximum(R * R - r2, 0))
+    for i in range(3):
+        np.testing.assert_allclose(out[i].cpu().numpy(), ref, rtol=1e-5, atol=1e-4)
+    gc = ufomi.gbp_geometry(center_position_x=[16.5], center_position_z=[16.5], source_position_y=[-60.0],
+                            detector_position_y=[60.0])
+    rows_c, par_c = ufomi.gbp_projection_rows(gc, params)
+    assert not par_c
+    ctx.project_ellipsoids(sphere, W, H, rows_c, out)
+    assert abs(out[0, 16, 16].item() - 2 * rho * R) < 1e-4
+
+
+def test_projection_rows_match_kernel_geometry(ctx, torch_dev):
+    """Backprojecting a one-hot projection lights the voxels whose (u, v) the rows put on that texel."""
+    rows, _ = ufomi.gbp_projection_rows(ufomi.gbp_geometry(source_position_y=[-40.0], detector_position_y=[30.0],
+                                                           center_position_x=[8.0], center_position_z=[4.0]),
+                                        ufomi.gbp_params(5))
+    ref = oracle.gbp(np.zeros((5, 8, 16), np.float32) + 1.0, source_position_y=[-40.0], detector_position_y=[30.0],
+                     center_position_x=[8.0], center_position_z=[4.0], region=(-1.0, 1.0, 1.0))
+    x, y, z = -8 + 3, -8 + 5, 0.0
+    X = np.array([x, y, z, 1.0])
+    for p in range(5):
+        U, V, D, C = rows[p] @ X
+        assert 0 < D and 0 < C
+    assert np.isfinite(ref).all()
+
+
 def test_noise_is_seeded(ctx, torch_dev):
     import torch
     a = torch.ones(100000, device=torch_dev)

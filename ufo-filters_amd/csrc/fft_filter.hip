@@ -1,11 +1,13 @@
-// Ramp filtering: `fft ! filter ! ifft` as LDS-resident batched FFTs.
+// Ramp filtering: `fft ! filter ! ifft` as batched FFTs on 256-thread
+// workgroups.
 //
-// Every kernel runs 256-thread workgroups.  A transform of N points is done by
-// TF = min(256, N/4) threads with a radix-4 Stockham autosort FFT (one radix-2
-// pass first when log2 N is odd) entirely in LDS; G = 256 / TF transforms
-// share a workgroup when N < 1024.  Each thread keeps its B = N / (4 TF)
-// butterflies in registers between the read and write halves of a pass, so a
-// single LDS buffer of N complex values per transform suffices.
+// 16 <= N <= 8192: the register radix-16 kernels of fft16.h (ramp16_kernel,
+// fft16_rows_kernel, ifft16_rows_kernel): first pass straight from HBM, filter
+// multiply in registers, last pass straight to HBM, passes exchanged through
+// one LDS buffer.
+// N = 4, 8: a radix-4 Stockham FFT in LDS (lds_fft): TF = N/4 threads per
+// transform, each keeping its butterflies in registers between the read and
+// write halves of a pass.
 // Twiddles come from the context's FP64-accurate table (read through L2).
 #include "ufomi_internal.h"
 #include "fft16.h"

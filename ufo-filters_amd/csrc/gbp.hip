@@ -108,7 +108,7 @@ void chain(const Geo &g, const Flags &f, V3 X, double out[4])
 }
 
 // Rows of the affine maps (U, V, D, C) from four evaluations of the chain.
-void compose(const Geo &g, const Flags &f, float m[16])
+void compose(const Geo &g, const Flags &f, double m[16])
 {
     double o[4], ex[4], ey[4], ez[4];
     chain(g, f, {0, 0, 0}, o);
@@ -116,76 +116,73 @@ void compose(const Geo &g, const Flags &f, float m[16])
     chain(g, f, {0, 1, 0}, ey);
     chain(g, f, {0, 0, 1}, ez);
     for (int r = 0; r < 4; r++) {
-        m[4 * r + 0] = (float) (ex[r] - o[r]);
-        m[4 * r + 1] = (float) (ey[r] - o[r]);
-        m[4 * r + 2] = (float) (ez[r] - o[r]);
-        m[4 * r + 3] = (float) o[r];
+        m[4 * r + 0] = ex[r] - o[r];
+        m[4 * r + 1] = ey[r] - o[r];
+        m[4 * r + 2] = ez[r] - o[r];
+        m[4 * r + 3] = o[r];
     }
 }
 
-template <typename T> struct StoreOp;
+// Projection mode of a launch: affine (parallel beam), projective with only
+// V depending on z (cone beam without tilts: U_z = D_z = C_z = 0, exactly, as
+// composed), or general projective.
+enum { MODE_PARALLEL = 0, MODE_CONE_ZSEP = 1, MODE_CONE = 2 };
 
-template <bool CONE, int ADDR, typename ST>
-__global__ __launch_bounds__(256) void gbp_kernel(
-    const float *__restrict__ proj, int W, int H, int P, const float4 *__restrict__ mats, float x0, float xs,
-    float y0, float ys, const float *__restrict__ zs, int nx, int ny, int nz, float norm, float gmin, float gscale,
-    float gmax, ST *__restrict__ vol)
-{
-    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int iz = blockIdx.z;
-    if (ix >= nx || iy >= ny)
-        return;
-    const float x = fmaf((float) ix, xs, x0), y = fmaf((float) iy, ys, y0), z = zs[iz];
-    float result = 0.0f;
-    const long plane = (long) W * H;
-    for (int p = 0; p < P; p++) {
-        const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1];
-        float u = fmaf(mu.x, x, fmaf(mu.y, y, fmaf(mu.z, z, mu.w)));
-        float v = fmaf(mv.x, x, fmaf(mv.y, y, fmaf(mv.z, z, mv.w)));
-        float w = 1.0f;
-        if constexpr (CONE) {
-            const float4 md = mats[4 * p + 2], mc = mats[4 * p + 3];
-            const float d = fmaf(md.x, x, fmaf(md.y, y, fmaf(md.z, z, md.w)));
-            const float c = fmaf(mc.x, x, fmaf(mc.y, y, fmaf(mc.z, z, mc.w)));
-            const float rd = __builtin_amdgcn_rcpf(d);
-            u *= rd;
-            v *= rd;
-            const float coeff = __builtin_amdgcn_rcpf(c);
-            w = coeff * coeff;
-        }
-        // read_imagef, unnormalised coordinates, linear filter
-        const float fu = u - 0.5f, fv = v - 0.5f;
-        const float flu = floorf(fu), flv = floorf(fv);
-        const float a = fu - flu, b = fv - flv;
-        int i0 = (int) flu, j0 = (int) flv;
-        int i1 = i0 + 1, j1 = j0 + 1;
-        float t00, t10, t01, t11;
-        const float *img = proj + (long) p * plane;
+// Horizontal half of a linear texture fetch: texel column i0 and weight a of
+// read_imagef at u (unnormalised: sample u - 0.5), with the border rule.
+template <int ADDR> struct Col {
+    int i0, i1;
+    float a;
+    bool c0, c1;
+    __device__ __forceinline__ void set(float u, int W)
+    {
+        const float fu = u - 0.5f, fl = floorf(fu);
+        a = fu - fl;
+        i0 = (int) fl;
+        i1 = i0 + 1;
         if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
-            const bool ci0 = i0 >= 0 && i0 < W, ci1 = i1 >= 0 && i1 < W;
-            const bool cj0 = j0 >= 0 && j0 < H, cj1 = j1 >= 0 && j1 < H;
-            t00 = (ci0 && cj0) ? img[(long) j0 * W + i0] : 0.0f;
-            t10 = (ci1 && cj0) ? img[(long) j0 * W + i1] : 0.0f;
-            t01 = (ci0 && cj1) ? img[(long) j1 * W + i0] : 0.0f;
-            t11 = (ci1 && cj1) ? img[(long) j1 * W + i1] : 0.0f;
+            c0 = i0 >= 0 && i0 < W;
+            c1 = i1 >= 0 && i1 < W;
         } else {
             i0 = min(max(i0, 0), W - 1);
             i1 = min(max(i1, 0), W - 1);
-            j0 = min(max(j0, 0), H - 1);
-            j1 = min(max(j1, 0), H - 1);
-            t00 = img[(long) j0 * W + i0];
-            t10 = img[(long) j0 * W + i1];
-            t01 = img[(long) j1 * W + i0];
-            t11 = img[(long) j1 * W + i1];
+            c0 = c1 = true;
         }
-        const float r0 = fmaf(a, t10 - t00, t00);
-        const float r1 = fmaf(a, t11 - t01, t01);
-        const float val = fmaf(b, r1 - r0, r0);
-        result = fmaf(val, w, result);
     }
-    const float r = norm * result;
-    const long o = ((long) iz * ny + iy) * nx + ix;
+};
+
+// Vertical half and the 4 taps; fv = v - 0.5.  Zero border (clamp) or clamp to edge.
+template <int ADDR>
+__device__ __forceinline__ float fetch(const float *__restrict__ img, int W, int H, const Col<ADDR> &c, float fv)
+{
+    const float fl = floorf(fv);
+    const float b = fv - fl;
+    int j0 = (int) fl, j1 = j0 + 1;
+    float t00, t10, t01, t11;
+    if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
+        const bool r0 = j0 >= 0 && j0 < H, r1 = j1 >= 0 && j1 < H;
+        const float *p0 = img + (long) j0 * W, *p1 = img + (long) j1 * W;
+        t00 = (r0 && c.c0) ? p0[c.i0] : 0.0f;
+        t10 = (r0 && c.c1) ? p0[c.i1] : 0.0f;
+        t01 = (r1 && c.c0) ? p1[c.i0] : 0.0f;
+        t11 = (r1 && c.c1) ? p1[c.i1] : 0.0f;
+    } else {
+        j0 = min(max(j0, 0), H - 1);
+        j1 = min(max(j1, 0), H - 1);
+        const float *p0 = img + (long) j0 * W, *p1 = img + (long) j1 * W;
+        t00 = p0[c.i0];
+        t10 = p0[c.i1];
+        t01 = p1[c.i0];
+        t11 = p1[c.i1];
+    }
+    const float r0 = fmaf(c.a, t10 - t00, t00);
+    const float r1 = fmaf(c.a, t11 - t01, t01);
+    return fmaf(b, r1 - r0, r0);
+}
+
+template <typename ST>
+__device__ __forceinline__ void store_voxel(ST *__restrict__ vol, long o, float r, float gmin, float gscale, float gmax)
+{
     if constexpr (std::is_same<ST, float>::value) {
         vol[o] = r;
     } else if constexpr (std::is_same<ST, double>::value) {
@@ -200,6 +197,77 @@ __global__ __launch_bounds__(256) void gbp_kernel(
             vol[o] = g >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t) g;
         else
             vol[o] = (ST) g;
+    }
+}
+
+constexpr int kZB = 8;  // z voxels per thread
+
+// One thread owns a column of kZB voxels (x, y, z0..z0+7); a workgroup covers
+// 64 x 4 columns, so a wave's 64 lanes sample 64 neighbouring detector columns
+// (coalesced).  Per projection the column shares everything that does not
+// depend on z: in MODE_CONE_ZSEP the depth, the FDK weight and the horizontal
+// texture coordinate, leaving one FMA, the vertical split and 4 taps per voxel.
+template <int MODE, int ADDR, typename ST>
+__global__ __launch_bounds__(256) void gbp_kernel(
+    const float *__restrict__ proj, int W, int H, int P, const float4 *__restrict__ mats, float x0, float xs,
+    float y0, float ys, const float *__restrict__ zs, int nx, int ny, int nz, float norm, float gmin, float gscale,
+    float gmax, ST *__restrict__ vol)
+{
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int iz0 = blockIdx.z * kZB;
+    if (ix >= nx || iy >= ny)
+        return;
+    const float x = fmaf((float) ix, xs, x0), y = fmaf((float) iy, ys, y0);
+    float z[kZB], acc[kZB];
+#pragma unroll
+    for (int k = 0; k < kZB; k++) {
+        z[k] = zs[min(iz0 + k, nz - 1)];
+        acc[k] = 0.0f;
+    }
+    const long plane = (long) W * H;
+    for (int p = 0; p < P; p++) {
+        const float *img = proj + (long) p * plane;
+        const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1];
+        const float ub = fmaf(mu.x, x, fmaf(mu.y, y, mu.w));
+        const float vb = fmaf(mv.x, x, fmaf(mv.y, y, mv.w));
+        if constexpr (MODE == MODE_PARALLEL) {
+#pragma unroll
+            for (int k = 0; k < kZB; k++) {
+                Col<ADDR> c;
+                c.set(fmaf(mu.z, z[k], ub), W);
+                acc[k] += fetch<ADDR>(img, W, H, c, fmaf(mv.z, z[k], vb) - 0.5f);
+            }
+        } else if constexpr (MODE == MODE_CONE_ZSEP) {
+            const float4 md = mats[4 * p + 2], mc = mats[4 * p + 3];
+            const float rd = __builtin_amdgcn_rcpf(fmaf(md.x, x, fmaf(md.y, y, md.w)));
+            const float coeff = __builtin_amdgcn_rcpf(fmaf(mc.x, x, fmaf(mc.y, y, mc.w)));
+            const float w = coeff * coeff;
+            Col<ADDR> c;
+            c.set(ub * rd, W);
+            const float dv = mv.z * rd, v0 = fmaf(vb, rd, -0.5f);
+#pragma unroll
+            for (int k = 0; k < kZB; k++)
+                acc[k] = fmaf(fetch<ADDR>(img, W, H, c, fmaf(dv, z[k], v0)), w, acc[k]);
+        } else {
+            const float4 md = mats[4 * p + 2], mc = mats[4 * p + 3];
+            const float db = fmaf(md.x, x, fmaf(md.y, y, md.w)), cb = fmaf(mc.x, x, fmaf(mc.y, y, mc.w));
+#pragma unroll
+            for (int k = 0; k < kZB; k++) {
+                const float rd = __builtin_amdgcn_rcpf(fmaf(md.z, z[k], db));
+                const float coeff = __builtin_amdgcn_rcpf(fmaf(mc.z, z[k], cb));
+                Col<ADDR> c;
+                c.set(fmaf(mu.z, z[k], ub) * rd, W);
+                const float val = fetch<ADDR>(img, W, H, c, fmaf(fmaf(mv.z, z[k], vb), rd, -0.5f));
+                acc[k] = fmaf(val, coeff * coeff, acc[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kZB; k++) {
+        const int iz = iz0 + k;
+        if (iz < nz)
+            store_voxel(vol, ((long) iz * ny + iy) * nx + ix, norm * acc[k], gmin, gscale, gmax);
     }
 }
 
@@ -220,6 +288,80 @@ int check_len(const char *name, int n, unsigned P)
     if (n > 1 && (unsigned) n < P)
         return ufomi_fail(UFOMI_ERR_ARG, "%s has %d values, needs 1 or %u", name, n, P);
     return UFOMI_OK;
+}
+
+int gbp_launch(ufomi_ctx *ctx, const float *d_proj, int W, int H, unsigned n, const std::vector<float> &mats,
+               const Flags &f, int kaddr, const ufomi_gbp_params *prm, int nx, int ny, int nz, void *d_vol)
+{
+    int rc;
+    const int st = prm->store_type;
+    const bool integer_store = st == UFOMI_ST_UCHAR || st == UFOMI_ST_USHORT || st == UFOMI_ST_UINT;
+    // compute_slice_region (:246-259): step 0 -> (-(float) length / 2, 1)
+    const float x0 = prm->x_region[2] ? (float) prm->x_region[0] : -((float) nx / 2);
+    const float xs = prm->x_region[2] ? (float) prm->x_region[2] : 1.0f;
+    const float y0 = prm->y_region[2] ? (float) prm->y_region[0] : -((float) ny / 2);
+    const float ys = prm->y_region[2] ? (float) prm->y_region[2] : 1.0f;
+    // create_regions (:38-76): z_k = start + k * step in FP64, rounded.
+    double zstart = 0.0, zstep = 1.0;
+    if (!(fabs(prm->region[2]) < 1e-7)) {
+        zstart = prm->region[0];
+        zstep = prm->region[2];
+    }
+    std::vector<float> zs(nz);
+    for (int k = 0; k < nz; k++)
+        zs[k] = (float) (zstart + k * zstep);
+
+    const float norm = (float) (fabs(prm->overall_angle) / n);
+    double maxval = st == UFOMI_ST_UCHAR ? 255.0 : st == UFOMI_ST_USHORT ? 65535.0 : 4294967295.0;
+    const float gmin = (float) prm->gray_map_min;
+    const float gscale = integer_store ? (float) (maxval / (prm->gray_map_max - prm->gray_map_min)) : 0.0f;
+    const float gmax = (float) maxval;
+
+    const size_t mbytes = mats.size() * sizeof(float), zbytes = zs.size() * sizeof(float);
+    char *scratch = nullptr;
+    if ((rc = ufomi_scratch(ctx, mbytes + zbytes + 256, (void **) &scratch))) return rc;
+    std::vector<char> host(mbytes + zbytes + 256);
+    memcpy(host.data(), mats.data(), mbytes);
+    const size_t zoff = (mbytes + 255) & ~(size_t) 255;
+    memcpy(host.data() + zoff, zs.data(), zbytes);
+    if ((rc = ufomi_upload(ctx, scratch, host.data(), zoff + zbytes))) return rc;
+
+    // cone beam whose U, D and C rows do not depend on z: share them along z
+    bool zsep = true;
+    for (size_t i = 0; i < n && zsep; i++)
+        zsep = mats[16 * i + 2] == 0.0f && mats[16 * i + 10] == 0.0f && mats[16 * i + 14] == 0.0f;
+    const int mode = f.parallel ? MODE_PARALLEL : (zsep ? MODE_CONE_ZSEP : MODE_CONE);
+    dim3 grid((nx + 63) / 64, (ny + 3) / 4, (nz + kZB - 1) / kZB);
+    auto go = [&](auto mode_c, auto addr_c, auto *vol) {
+        constexpr int C = decltype(mode_c)::value;
+        constexpr int A = decltype(addr_c)::value;
+        using ST = std::remove_pointer_t<decltype(vol)>;
+        hipLaunchKernelGGL((gbp_kernel<C, A, ST>), grid, dim3(256), 0, ctx->stream, d_proj, W, H, (int) n,
+                           (const float4 *) scratch, x0, xs, y0, ys, (const float *) (scratch + zoff), nx, ny, nz,
+                           norm, gmin, gscale, gmax, vol);
+        UFOMI_CHECK_LAUNCH();
+        return (int) UFOMI_OK;
+    };
+    auto by_store = [&](auto cone, auto addr_c) {
+        switch (st) {
+        case UFOMI_ST_HALF: return go(cone, addr_c, (__half *) d_vol);
+        case UFOMI_ST_DOUBLE: return go(cone, addr_c, (double *) d_vol);
+        case UFOMI_ST_UCHAR: return go(cone, addr_c, (uint8_t *) d_vol);
+        case UFOMI_ST_USHORT: return go(cone, addr_c, (uint16_t *) d_vol);
+        case UFOMI_ST_UINT: return go(cone, addr_c, (uint32_t *) d_vol);
+        default: return go(cone, addr_c, (float *) d_vol);
+        }
+    };
+    auto by_addr = [&](auto cone) {
+        if (kaddr == UFOMI_ADDR_CLAMP)
+            return by_store(cone, std::integral_constant<int, UFOMI_ADDR_CLAMP>());
+        return by_store(cone, std::integral_constant<int, UFOMI_ADDR_CLAMP_TO_EDGE>());
+    };
+    switch (mode) {
+    case MODE_PARALLEL: return by_addr(std::integral_constant<int, MODE_PARALLEL>());
+    case MODE_CONE_ZSEP: return by_addr(std::integral_constant<int, MODE_CONE_ZSEP>());
+    default: return by_addr(std::integral_constant<int, MODE_CONE>());
+    }
 }
 
 }  // namespace
@@ -280,6 +422,30 @@ extern "C" int ufomi_gbp(ufomi_ctx *ctx, const float *d_proj, int W, int H, int 
     // "none" leaves out-of-range reads undefined in OpenCL; clamp to the edge here.
     const int kaddr = addr == UFOMI_ADDR_CLAMP ? UFOMI_ADDR_CLAMP : UFOMI_ADDR_CLAMP_TO_EDGE;
 
+    std::vector<double> rows(16 * (size_t) n);
+    int parallel = 0;
+    if ((rc = ufomi_gbp_projection_rows(g, prm, rows.data(), &parallel))) return rc;
+    std::vector<float> mats(rows.size());
+    for (size_t i = 0; i < rows.size(); i++)
+        mats[i] = (float) rows[i];
+    const Flags f = {false, false, false, false, false, parallel != 0};
+
+    int nx, ny, nz;
+    if ((rc = ufomi_gbp_volume_shape(prm, W, &nx, &ny, &nz))) return rc;
+    if (nz > 65535)
+        return ufomi_fail(UFOMI_ERR_ARG, "at most 65535 slices per call");
+    return gbp_launch(ctx, d_proj, W, H, n, mats, f, kaddr, prm, nx, ny, nz, d_vol);
+}
+
+extern "C" int ufomi_gbp_projection_rows(const ufomi_gbp_geometry *g, const ufomi_gbp_params *prm, double *rows,
+                                         int *parallel_beam)
+{
+    if (!g || !prm || !rows)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid arguments");
+    const unsigned n = prm->num_projections;
+    if (!n)
+        return ufomi_fail(UFOMI_ERR_NUM_PROJECTIONS, "Number of projections not set");
+    int rc;
     struct { const char *name; int n; } lens[] = {
         {"axis-angle-x", g->n_axis_angle_x}, {"axis-angle-y", g->n_axis_angle_y},
         {"volume-angle-x", g->n_volume_angle_x}, {"volume-angle-y", g->n_volume_angle_y},
@@ -311,7 +477,6 @@ extern "C" int ufomi_gbp(ufomi_ctx *ctx, const float *d_proj, int W, int H, int 
 
     // Tomographic angles: axis-angle-z if it has n values, else i / n * overall (:1435-1446).
     const bool own_tomo = g->axis_angle_z && g->n_axis_angle_z == (int) n;
-    std::vector<float> mats(16 * (size_t) n);
     for (unsigned i = 0; i < n; i++) {
         Geo q;
         q.tomo = own_tomo ? g->axis_angle_z[i] : ((double) i) / n * prm->overall_angle;
@@ -333,68 +498,9 @@ extern "C" int ufomi_gbp(ufomi_ctx *ctx, const float *d_proj, int W, int H, int 
         q.dz = at(g->detector_position_z, g->n_detector_position_z, i, 0.0);
         if (!f.parallel && !(std::isfinite(q.sy) && std::isfinite(q.dy) && q.dy != q.sy))
             return ufomi_fail(UFOMI_ERR_ARG, "cone beam needs finite source and detector y positions");
-        compose(q, f, &mats[16 * (size_t) i]);
+        compose(q, f, rows + 16 * (size_t) i);
     }
-
-    int nx, ny, nz;
-    if ((rc = ufomi_gbp_volume_shape(prm, W, &nx, &ny, &nz))) return rc;
-    if (nz > 65535)
-        return ufomi_fail(UFOMI_ERR_ARG, "at most 65535 slices per call");
-    // compute_slice_region (:246-259): step 0 -> (-(float) length / 2, 1)
-    const float x0 = prm->x_region[2] ? (float) prm->x_region[0] : -((float) nx / 2);
-    const float xs = prm->x_region[2] ? (float) prm->x_region[2] : 1.0f;
-    const float y0 = prm->y_region[2] ? (float) prm->y_region[0] : -((float) ny / 2);
-    const float ys = prm->y_region[2] ? (float) prm->y_region[2] : 1.0f;
-    // create_regions (:38-76): z_k = start + k * step in FP64, rounded.
-    double zstart = 0.0, zstep = 1.0;
-    if (!(fabs(prm->region[2]) < 1e-7)) {
-        zstart = prm->region[0];
-        zstep = prm->region[2];
-    }
-    std::vector<float> zs(nz);
-    for (int k = 0; k < nz; k++)
-        zs[k] = (float) (zstart + k * zstep);
-
-    const float norm = (float) (fabs(prm->overall_angle) / n);
-    double maxval = st == UFOMI_ST_UCHAR ? 255.0 : st == UFOMI_ST_USHORT ? 65535.0 : 4294967295.0;
-    const float gmin = (float) prm->gray_map_min;
-    const float gscale = integer_store ? (float) (maxval / (prm->gray_map_max - prm->gray_map_min)) : 0.0f;
-    const float gmax = (float) maxval;
-
-    const size_t mbytes = mats.size() * sizeof(float), zbytes = zs.size() * sizeof(float);
-    char *scratch = nullptr;
-    if ((rc = ufomi_scratch(ctx, mbytes + zbytes + 256, (void **) &scratch))) return rc;
-    std::vector<char> host(mbytes + zbytes + 256);
-    memcpy(host.data(), mats.data(), mbytes);
-    const size_t zoff = (mbytes + 255) & ~(size_t) 255;
-    memcpy(host.data() + zoff, zs.data(), zbytes);
-    if ((rc = ufomi_upload(ctx, scratch, host.data(), zoff + zbytes))) return rc;
-
-    dim3 grid((nx + 15) / 16, (ny + 15) / 16, nz);
-    auto go = [&](auto cone, auto addr_c, auto *vol) {
-        constexpr bool C = decltype(cone)::value;
-        constexpr int A = decltype(addr_c)::value;
-        using ST = std::remove_pointer_t<decltype(vol)>;
-        hipLaunchKernelGGL((gbp_kernel<C, A, ST>), grid, dim3(256), 0, ctx->stream, d_proj, W, H, (int) n,
-                           (const float4 *) scratch, x0, xs, y0, ys, (const float *) (scratch + zoff), nx, ny, nz,
-                           norm, gmin, gscale, gmax, vol);
-        UFOMI_CHECK_LAUNCH();
-        return (int) UFOMI_OK;
-    };
-    auto by_store = [&](auto cone, auto addr_c) {
-        switch (st) {
-        case UFOMI_ST_HALF: return go(cone, addr_c, (__half *) d_vol);
-        case UFOMI_ST_DOUBLE: return go(cone, addr_c, (double *) d_vol);
-        case UFOMI_ST_UCHAR: return go(cone, addr_c, (uint8_t *) d_vol);
-        case UFOMI_ST_USHORT: return go(cone, addr_c, (uint16_t *) d_vol);
-        case UFOMI_ST_UINT: return go(cone, addr_c, (uint32_t *) d_vol);
-        default: return go(cone, addr_c, (float *) d_vol);
-        }
-    };
-    auto by_addr = [&](auto cone) {
-        if (kaddr == UFOMI_ADDR_CLAMP)
-            return by_store(cone, std::integral_constant<int, UFOMI_ADDR_CLAMP>());
-        return by_store(cone, std::integral_constant<int, UFOMI_ADDR_CLAMP_TO_EDGE>());
-    };
-    return f.parallel ? by_addr(std::false_type()) : by_addr(std::true_type());
+    if (parallel_beam)
+        *parallel_beam = f.parallel ? 1 : 0;
+    return UFOMI_OK;
 }

@@ -129,7 +129,87 @@ __global__ __launch_bounds__(256) void absmax_kernel(const float *__restrict__ d
         atomicMax(out, __float_as_uint(red[0]));  // non-negative floats order like their bits
 }
 
+// Projections of ellipsoids for a general geometry given by the same per-
+// projection affine rows the general backprojector uses (u = U.X / D.X,
+// v = V.X / D.X): the pixel (u, v) sees the line {X : (U - u D).X = 0,
+// (V - v D).X = 0}.  The line integral of each ellipsoid (8 doubles: x0, y0,
+// z0, a, b, c, phi, density) along it is density times the chord length, in
+// voxel units.  FP64 throughout.
+__global__ __launch_bounds__(256) void project_ellipsoids_kernel(const double *__restrict__ ell, int n_ell,
+                                                                 const double *__restrict__ rows, int W, int H,
+                                                                 float *__restrict__ out)
+{
+    __shared__ double e[kMaxEll * 10];
+    const int p = blockIdx.z;
+    for (int k = threadIdx.x; k < n_ell; k += 256) {
+        const double *s = ell + 8 * k;
+        double *o = e + 10 * k;
+        o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
+        o[3] = 1.0 / s[3]; o[4] = 1.0 / s[4]; o[5] = 1.0 / s[5];
+        o[6] = cos(s[6]); o[7] = sin(s[6]); o[8] = s[7];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= W || j >= H)
+        return;
+    const double *m = rows + 16 * (size_t) p;  // U, V, D rows of 4
+    const double u = i + 0.5, v = j + 0.5;     // texel centre (read_imagef samples at u - 0.5)
+    double n1[3], n2[3];
+    for (int c = 0; c < 3; c++) {
+        n1[c] = m[c] - u * m[8 + c];
+        n2[c] = m[4 + c] - v * m[8 + c];
+    }
+    const double b1 = u * m[11] - m[3], b2 = v * m[11] - m[7];
+    const double d[3] = {n1[1] * n2[2] - n1[2] * n2[1], n1[2] * n2[0] - n1[0] * n2[2], n1[0] * n2[1] - n1[1] * n2[0]};
+    const double n11 = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
+    const double n22 = n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2];
+    const double n12 = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+    const double det = n11 * n22 - n12 * n12;
+    const double ca = (b1 * n22 - b2 * n12) / det, cb = (b2 * n11 - b1 * n12) / det;
+    const double x0[3] = {ca * n1[0] + cb * n2[0], ca * n1[1] + cb * n2[1], ca * n1[2] + cb * n2[2]};
+    const double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double acc = 0.0;
+    for (int k = 0; k < n_ell; k++) {
+        const double *q = e + 10 * k;
+        // into the ellipsoid frame (rotation -phi about z), scaled to the unit sphere
+        const double px = x0[0] - q[0], py = x0[1] - q[1], pz = x0[2] - q[2];
+        const double ex = (q[6] * px + q[7] * py) * q[3], ey = (-q[7] * px + q[6] * py) * q[4], ez = pz * q[5];
+        const double fx = (q[6] * d[0] + q[7] * d[1]) * q[3], fy = (-q[7] * d[0] + q[6] * d[1]) * q[4],
+                     fz = d[2] * q[5];
+        const double A = fx * fx + fy * fy + fz * fz, B = ex * fx + ey * fy + ez * fz;
+        const double C = ex * ex + ey * ey + ez * ez - 1.0;
+        const double disc = B * B - A * C;
+        if (disc > 0.0)
+            acc += q[8] * 2.0 * sqrt(disc) / A * dl;
+    }
+    out[((size_t) p * H + j) * W + i] = (float) acc;
+}
+
 }  // namespace
+
+extern "C" int ufomi_project_ellipsoids(ufomi_ctx *ctx, const double *ellipsoids, int n_ellipsoids, int W, int H,
+                                        int P, const double *rows, float *d_out)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!ellipsoids || !rows || !d_out || n_ellipsoids <= 0 || n_ellipsoids > kMaxEll || W <= 0 || H <= 0 ||
+        P <= 0 || P > 65535)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid projection arguments");
+    const size_t eb = (size_t) n_ellipsoids * 8 * sizeof(double), mb = (size_t) P * 16 * sizeof(double);
+    char *d = nullptr;
+    if ((rc = ufomi_scratch(ctx, eb + mb + 256, (void **) &d))) return rc;
+    std::vector<char> h(eb + mb + 256);
+    const size_t moff = (eb + 255) & ~(size_t) 255;
+    memcpy(h.data(), ellipsoids, eb);
+    memcpy(h.data() + moff, rows, mb);
+    if ((rc = ufomi_upload(ctx, d, h.data(), moff + mb))) return rc;
+    dim3 grid((W + 63) / 64, (H + 3) / 4, P);
+    hipLaunchKernelGGL(project_ellipsoids_kernel, grid, dim3(256), 0, ctx->stream, (const double *) d, n_ellipsoids,
+                       (const double *) (d + moff), W, H, d_out);
+    UFOMI_CHECK_LAUNCH();
+    return UFOMI_OK;
+}
 
 extern "C" int ufomi_phantom_ellipses(uint64_t seed, int n, double radius_px, double *e)
 {

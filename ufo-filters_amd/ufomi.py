@@ -35,6 +35,8 @@ EXPORTS = [
     "ufomi_cone_weight", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
     "ufomi_gbp", "ufomi_phantom_ellipses", "ufomi_phantom_ellipsoids", "ufomi_slice_ellipsoids",
     "ufomi_sinogram_ellipses", "ufomi_add_noise", "ufomi_bench_lds_read", "ufomi_bench_copy",
+    "ufomi_gbp_projection_rows", "ufomi_project_ellipsoids", "ufomi_device_alloc", "ufomi_device_free",
+    "ufomi_memcpy", "ufomi_device_memory",
 ]
 
 
@@ -100,6 +102,9 @@ def load():
         "ufomi_slice_ellipsoids": [dp, ci, cd, dp],
         "ufomi_sinogram_ellipses": [vp, dp, ci, ci, ci, ci, cd, cd, vp],
         "ufomi_add_noise": [vp, vp, csz, cd, ctypes.c_uint64, ctypes.POINTER(cf)],
+        "ufomi_gbp_projection_rows": [ctypes.POINTER(GbpGeometry), ctypes.POINTER(GbpParams), dp,
+                                      ctypes.POINTER(ci)],
+        "ufomi_project_ellipsoids": [vp, dp, ci, ci, ci, ci, dp, vp],
         "ufomi_bench_lds_read": [vp, ci, ci, vp, ctypes.POINTER(cd)],
         "ufomi_bench_copy": [vp, vp, vp, csz],
     }
@@ -164,6 +169,15 @@ def gbp_volume_shape(params, width):
     check(load().ufomi_gbp_volume_shape(ctypes.byref(params), width, ctypes.byref(nx), ctypes.byref(ny),
                                         ctypes.byref(nz)))
     return nz.value, ny.value, nx.value
+
+
+def gbp_projection_rows(geometry, params):
+    """[P][4][4] FP64 rows (U, V, D, C) of the general backprojector's maps, and the parallel flag."""
+    rows = np.empty((params.num_projections, 4, 4), dtype=np.float64)
+    par = ctypes.c_int()
+    check(load().ufomi_gbp_projection_rows(ctypes.byref(geometry), ctypes.byref(params),
+                                           rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(par)))
+    return rows, bool(par.value)
 
 
 def gbp_params(num_projections, overall_angle=2 * math.pi, region=(0, 0, 0), x_region=(0, 0, 0),
@@ -274,6 +288,11 @@ class Context:
             e = e[None]
         check(self.L.ufomi_sinogram_ellipses(self.h, _dptr(e), e.shape[1], e.shape[0], W, P, axis_pos,
                                              angle_step, _ptr(d_out)))
+
+    def project_ellipsoids(self, ellipsoids, W, H, rows, d_out):
+        e = np.ascontiguousarray(ellipsoids, dtype=np.float64)
+        r = np.ascontiguousarray(rows, dtype=np.float64)
+        check(self.L.ufomi_project_ellipsoids(self.h, _dptr(e), len(e), W, H, r.shape[0], _dptr(r), _ptr(d_out)))
 
     def add_noise(self, d, rel_sigma, seed):
         m = ctypes.c_float()
