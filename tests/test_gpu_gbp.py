@@ -56,6 +56,16 @@ def _projs(seed, p, h, w):
     return np.random.default_rng(seed).standard_normal((p, h, w)).astype(np.float32)
 
 
+def _smooth_projs(seed, p, h, w, width=9):
+    """Noise box-filtered along the detector rows. With voxels several pixels apart, white
+    noise turns the float32 rounding of detector coordinates (~1e-5 px at u ~ 300) into
+    errors of the same relative size in each float32 evaluation, so two float32 results
+    differ by ~sqrt(2) times that; band-limited data keeps the comparison about geometry."""
+    a = np.random.default_rng(seed).standard_normal((p, h, w + width - 1))
+    k = np.ones(width) / width
+    return np.apply_along_axis(lambda r: np.convolve(r, k, mode="valid"), 2, a).astype(np.float32)
+
+
 def test_parallel_tomography(ctx, torch_dev):
     projs = _projs(1, 24, 8, 32)
     _check(ctx, torch_dev, projs, center_position_x=[16.0], center_position_z=[4.0], region=(-3.0, 3.0, 1.0))
@@ -119,6 +129,25 @@ def test_regions_and_clamp_to_edge(ctx, torch_dev):
     projs = _projs(8, 12, 16, 16)
     _check(ctx, torch_dev, projs, addressing_mode="clamp_to_edge", x_region=(-10.0, 10.0, 0.5),
            y_region=(-6.0, 9.0, 1.5), region=(-2.0, 2.0, 0.5), center_position_x=[8.0], center_position_z=[8.0])
+
+
+@pytest.mark.parametrize("cone", [False, True])
+def test_wide_footprint_global_path(ctx, torch_dev, cone):
+    """Voxels spaced 6 px apart: a 32-voxel tile spans ~190 detector columns, more than the
+    LDS box holds, so every projection takes the global-memory path of the kernel."""
+    projs = _smooth_projs(12, 10, 24, 300)
+    kw = dict(x_region=(-96.0, 96.0, 6.0), y_region=(-96.0, 96.0, 6.0), region=(-4.0, 4.0, 2.0),
+              center_position_x=[150.0], center_position_z=[12.0])
+    if cone:
+        kw.update(source_position_y=[-900.0], detector_position_y=[300.0])
+    _check(ctx, torch_dev, projs, **kw)
+
+
+def test_mixed_box_and_global_paths(ctx, torch_dev):
+    """A volume whose outer tiles sit where the cone magnifies them beyond the LDS box."""
+    projs = _smooth_projs(13, 16, 40, 160)
+    _check(ctx, torch_dev, projs, x_region=(-64.0, 64.0, 1.0), y_region=(-64.0, 64.0, 1.0), region=(-4.0, 4.0, 1.0),
+           center_position_x=[80.0], center_position_z=[20.0], source_position_y=[-70.0], detector_position_y=[90.0])
 
 
 @pytest.mark.parametrize("store", ["half", "double", "uchar", "ushort", "uint"])

@@ -126,7 +126,7 @@ void compose(const Geo &g, const Flags &f, double m[16])
 // Projection mode of a launch: affine (parallel beam), projective with only
 // V depending on z (cone beam without tilts: U_z = D_z = C_z = 0, exactly, as
 // composed), or general projective.
-enum { MODE_PARALLEL = 0, MODE_CONE_ZSEP = 1, MODE_CONE = 2 };
+enum { MODE_PARALLEL = 0, MODE_CONE_ZSEP = 1, MODE_CONE = 2, MODE_PARALLEL_ZSEP = 3 };
 
 // Horizontal half of a linear texture fetch: texel column i0 and weight a of
 // read_imagef at u (unnormalised: sample u - 0.5), with the border rule.
@@ -200,74 +200,242 @@ __device__ __forceinline__ void store_voxel(ST *__restrict__ vol, long o, float 
     }
 }
 
-constexpr int kZB = 8;  // z voxels per thread
+// ---------------------------------------------------------------------------
+// LDS-staged kernel.  A workgroup owns a 32 x 32 x 8 voxel tile; thread
+// (lx, ly) owns the columns (lx + 16 kx, ly + 16 ky), kx, ky in {0, 1}, of 8
+// voxels.  For every projection:
+//   A. wave 0 projects the 8 tile corners (the image of a box under a
+//      projective map with positive depth is the hull of the corner images)
+//      and writes the detector footprint box to LDS;
+//   B. the workgroup stages the box, one float4 per texel (t00, t10 - t00,
+//      t01 - t00, t11 - t10 - t01 + t00) with the border rule applied, so a
+//      bilinear sample is one ds_read_b128 and three FMAs;
+//   C. every voxel samples from LDS.
+// A footprint larger than the box capacity (or a tile reaching behind the
+// source) takes the global-memory path for that projection.
+// ---------------------------------------------------------------------------
+constexpr int kTX = 32, kTY = 32, kTZ = 8;
+constexpr int kBoxPitch = 64, kBoxRows = 40;  // 2560 float4 = 40 KiB
 
-// One thread owns a column of kZB voxels (x, y, z0..z0+7); a workgroup covers
-// 64 x 4 columns, so a wave's 64 lanes sample 64 neighbouring detector columns
-// (coalesced).  Per projection the column shares everything that does not
-// depend on z: in MODE_CONE_ZSEP the depth, the FDK weight and the horizontal
-// texture coordinate, leaving one FMA, the vertical split and 4 taps per voxel.
+struct BoxGeom {
+    int i_lo, j_lo, w, h;  // w = 0: global path
+};
+
+// Projective (or affine) map of one projection at X = (x, y, z).
+template <bool CONE>
+__device__ __forceinline__ void map_point(const float4 &mu, const float4 &mv, const float4 &md, float x, float y,
+                                          float z, float &u, float &v, float &d)
+{
+    u = fmaf(mu.x, x, fmaf(mu.y, y, fmaf(mu.z, z, mu.w)));
+    v = fmaf(mv.x, x, fmaf(mv.y, y, fmaf(mv.z, z, mv.w)));
+    d = 1.0f;
+    if constexpr (CONE) {
+        d = fmaf(md.x, x, fmaf(md.y, y, fmaf(md.z, z, md.w)));
+        u = u / d;
+        v = v / d;
+    }
+}
+
 template <int MODE, int ADDR, typename ST>
-__global__ __launch_bounds__(256) void gbp_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void gbp_lds_kernel(
     const float *__restrict__ proj, int W, int H, int P, const float4 *__restrict__ mats, float x0, float xs,
     float y0, float ys, const float *__restrict__ zs, int nx, int ny, int nz, float norm, float gmin, float gscale,
-    float gmax, ST *__restrict__ vol)
+    float gmax, ST *__restrict__ vol, int tiles_x, int tiles_y)
 {
-    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int iz0 = blockIdx.z * kZB;
-    if (ix >= nx || iy >= ny)
-        return;
-    const float x = fmaf((float) ix, xs, x0), y = fmaf((float) iy, ys, y0);
-    float z[kZB], acc[kZB];
+    constexpr bool CONE = MODE == MODE_CONE_ZSEP || MODE == MODE_CONE;
+    // U (and for cone D, C) independent of z: checked on the host for the ZSEP modes
+    constexpr bool ZSEP = MODE == MODE_CONE_ZSEP || MODE == MODE_PARALLEL_ZSEP;
+    __shared__ float4 box[kBoxRows * kBoxPitch];
+    __shared__ BoxGeom bg;
+
+    const int tile = blockIdx.x;
+    const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
+    const int X0 = tx * kTX, Y0 = ty * kTY, Z0 = tz * kTZ;
+    const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
+
+    float xv[2], yv[2], z[kTZ];
 #pragma unroll
-    for (int k = 0; k < kZB; k++) {
-        z[k] = zs[min(iz0 + k, nz - 1)];
-        acc[k] = 0.0f;
+    for (int k = 0; k < 2; k++) {
+        xv[k] = fmaf((float) (X0 + lx + 16 * k), xs, x0);
+        yv[k] = fmaf((float) (Y0 + ly + 16 * k), ys, y0);
     }
+#pragma unroll
+    for (int k = 0; k < kTZ; k++)
+        z[k] = zs[min(Z0 + k, nz - 1)];
+    // tile corners (all 32 x 32 x 8 positions, also those beyond the volume)
+    const float cxa = fmaf((float) X0, xs, x0), cxb = fmaf((float) (X0 + kTX - 1), xs, x0);
+    const float cya = fmaf((float) Y0, ys, y0), cyb = fmaf((float) (Y0 + kTY - 1), ys, y0);
+    const float cza = z[0], czb = z[kTZ - 1];
+
+    float acc[2][2][kTZ];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int k = 0; k < kTZ; k++)
+                acc[a][b][k] = 0.0f;
+
     const long plane = (long) W * H;
     for (int p = 0; p < P; p++) {
         const float *img = proj + (long) p * plane;
-        const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1];
-        const float ub = fmaf(mu.x, x, fmaf(mu.y, y, mu.w));
-        const float vb = fmaf(mv.x, x, fmaf(mv.y, y, mv.w));
-        if constexpr (MODE == MODE_PARALLEL) {
+        const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1], md = mats[4 * p + 2], mc = mats[4 * p + 3];
+        // A. footprint of the tile (wave 0)
+        if (tid < 64) {
+            const int c = tid & 7;
+            float u, v, d;
+            map_point<CONE>(mu, mv, md, (c & 1) ? cxb : cxa, (c & 2) ? cyb : cya, (c & 4) ? czb : cza, u, v, d);
+            float umin = u, umax = u, vmin = v, vmax = v, dmin = d;
 #pragma unroll
-            for (int k = 0; k < kZB; k++) {
-                Col<ADDR> c;
-                c.set(fmaf(mu.z, z[k], ub), W);
-                acc[k] += fetch<ADDR>(img, W, H, c, fmaf(mv.z, z[k], vb) - 0.5f);
+            for (int o = 1; o < 8; o <<= 1) {
+                umin = fminf(umin, __shfl_xor(umin, o, 8));
+                umax = fmaxf(umax, __shfl_xor(umax, o, 8));
+                vmin = fminf(vmin, __shfl_xor(vmin, o, 8));
+                vmax = fmaxf(vmax, __shfl_xor(vmax, o, 8));
+                dmin = fminf(dmin, __shfl_xor(dmin, o, 8));
             }
-        } else if constexpr (MODE == MODE_CONE_ZSEP) {
-            const float4 md = mats[4 * p + 2], mc = mats[4 * p + 3];
-            const float rd = __builtin_amdgcn_rcpf(fmaf(md.x, x, fmaf(md.y, y, md.w)));
-            const float coeff = __builtin_amdgcn_rcpf(fmaf(mc.x, x, fmaf(mc.y, y, mc.w)));
-            const float w = coeff * coeff;
-            Col<ADDR> c;
-            c.set(ub * rd, W);
-            const float dv = mv.z * rd, v0 = fmaf(vb, rd, -0.5f);
-#pragma unroll
-            for (int k = 0; k < kZB; k++)
-                acc[k] = fmaf(fetch<ADDR>(img, W, H, c, fmaf(dv, z[k], v0)), w, acc[k]);
-        } else {
-            const float4 md = mats[4 * p + 2], mc = mats[4 * p + 3];
-            const float db = fmaf(md.x, x, fmaf(md.y, y, md.w)), cb = fmaf(mc.x, x, fmaf(mc.y, y, mc.w));
-#pragma unroll
-            for (int k = 0; k < kZB; k++) {
-                const float rd = __builtin_amdgcn_rcpf(fmaf(md.z, z[k], db));
-                const float coeff = __builtin_amdgcn_rcpf(fmaf(mc.z, z[k], cb));
-                Col<ADDR> c;
-                c.set(fmaf(mu.z, z[k], ub) * rd, W);
-                const float val = fetch<ADDR>(img, W, H, c, fmaf(fmaf(mv.z, z[k], vb), rd, -0.5f));
-                acc[k] = fmaf(val, coeff * coeff, acc[k]);
+            if (tid == 0) {
+                BoxGeom g;
+                const bool finite = umin > -1e6f && umax < 1e6f && vmin > -1e6f && vmax < 1e6f;
+                g.i_lo = finite ? (int) floorf(umin - 0.5f) - 1 : 0;
+                g.j_lo = finite ? (int) floorf(vmin - 0.5f) - 1 : 0;
+                g.w = finite ? (int) floorf(umax - 0.5f) - g.i_lo + 2 : 0;
+                g.h = finite ? (int) floorf(vmax - 0.5f) - g.j_lo + 2 : 0;
+                if (!(dmin > 0.0f) || g.w > kBoxPitch || g.h > kBoxRows || g.w <= 0 || g.h <= 0)
+                    g.w = 0;
+                bg = g;
             }
         }
-    }
+        __syncthreads();  // box geometry ready; previous projection's samples done
+        const BoxGeom g = bg;
+        if (g.w > 0) {
+            // B. stage the footprint
+            const int c = tid & 63;
+            if (c < g.w) {
+                const int i = g.i_lo + c;
+                int i0 = i, i1 = i + 1;
+                bool ci0 = true, ci1 = true;
+                if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
+                    ci0 = i0 >= 0 && i0 < W;
+                    ci1 = i1 >= 0 && i1 < W;
+                } else {
+                    i0 = min(max(i0, 0), W - 1);
+                    i1 = min(max(i1, 0), W - 1);
+                }
+                for (int r = tid >> 6; r < g.h; r += 4) {
+                    const int j = g.j_lo + r;
+                    int j0 = j, j1 = j + 1;
+                    bool cj0 = true, cj1 = true;
+                    if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
+                        cj0 = j0 >= 0 && j0 < H;
+                        cj1 = j1 >= 0 && j1 < H;
+                    } else {
+                        j0 = min(max(j0, 0), H - 1);
+                        j1 = min(max(j1, 0), H - 1);
+                    }
+                    const float *p0 = img + (long) j0 * W, *p1 = img + (long) j1 * W;
+                    const float t00 = (cj0 && ci0) ? p0[i0] : 0.0f;
+                    const float t10 = (cj0 && ci1) ? p0[i1] : 0.0f;
+                    const float t01 = (cj1 && ci0) ? p1[i0] : 0.0f;
+                    const float t11 = (cj1 && ci1) ? p1[i1] : 0.0f;
+                    box[r * kBoxPitch + c] = make_float4(t00, t10 - t00, t01 - t00, (t11 - t10) - (t01 - t00));
+                }
+            }
+            __syncthreads();
+            // C. sample
+            const float ilo = (float) g.i_lo + 0.5f, jlo = (float) g.j_lo + 0.5f;
 #pragma unroll
-    for (int k = 0; k < kZB; k++) {
-        const int iz = iz0 + k;
-        if (iz < nz)
-            store_voxel(vol, ((long) iz * ny + iy) * nx + ix, norm * acc[k], gmin, gscale, gmax);
+            for (int a = 0; a < 2; a++) {
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const float x = xv[a], y = yv[b];
+                    const float ub = fmaf(mu.x, x, fmaf(mu.y, y, mu.w));
+                    const float vb = fmaf(mv.x, x, fmaf(mv.y, y, mv.w));
+                    if constexpr (ZSEP) {
+                        float rd = 1.0f, w = 1.0f;
+                        if constexpr (CONE) {
+                            rd = __builtin_amdgcn_rcpf(fmaf(md.x, x, fmaf(md.y, y, md.w)));
+                            const float cf = __builtin_amdgcn_rcpf(fmaf(mc.x, x, fmaf(mc.y, y, mc.w)));
+                            w = cf * cf;
+                        }
+                        const float uu = fmaf(ub, rd, -ilo);
+                        const int ci = (int) uu;
+                        const float fa = __builtin_amdgcn_fractf(uu);
+                        const float dv = mv.z * rd, v0 = fmaf(vb, rd, -jlo);
+#pragma unroll
+                        for (int k = 0; k < kTZ; k++) {
+                            const float vv = fmaf(dv, z[k], v0);
+                            const float4 q = box[(int) vv * kBoxPitch + ci];
+                            const float fb = __builtin_amdgcn_fractf(vv);
+                            const float val = fmaf(fb, fmaf(fa, q.w, q.z), fmaf(fa, q.y, q.x));
+                            if constexpr (CONE)
+                                acc[a][b][k] = fmaf(val, w, acc[a][b][k]);
+                            else
+                                acc[a][b][k] += val;
+                        }
+                    } else {
+                        const float db = fmaf(md.x, x, fmaf(md.y, y, md.w));
+                        const float cb = fmaf(mc.x, x, fmaf(mc.y, y, mc.w));
+#pragma unroll
+                        for (int k = 0; k < kTZ; k++) {
+                            const float rd = __builtin_amdgcn_rcpf(fmaf(md.z, z[k], db));
+                            const float cf = __builtin_amdgcn_rcpf(fmaf(mc.z, z[k], cb));
+                            const float uu = fmaf(fmaf(mu.z, z[k], ub), rd, -ilo);
+                            const float vv = fmaf(fmaf(mv.z, z[k], vb), rd, -jlo);
+                            const float4 q = box[(int) vv * kBoxPitch + (int) uu];
+                            const float fa = __builtin_amdgcn_fractf(uu), fb = __builtin_amdgcn_fractf(vv);
+                            const float val = fmaf(fb, fmaf(fa, q.w, q.z), fmaf(fa, q.y, q.x));
+                            acc[a][b][k] = fmaf(val, cf * cf, acc[a][b][k]);
+                        }
+                    }
+                }
+            }
+        } else {
+            // Global-memory path for this projection (footprint too large or
+            // behind the source).  One voxel per iteration of a rolled loop and
+            // a select chain into the accumulators, so this rare path does not
+            // inflate the register budget of the LDS path.
+#pragma unroll 1
+            for (int q = 0; q < 4 * kTZ; q++) {
+                const int a = q / (2 * kTZ), b = (q / kTZ) & 1, k = q % kTZ;
+                const float x = a ? xv[1] : xv[0], y = b ? yv[1] : yv[0];
+                const float zz = zs[min(Z0 + k, nz - 1)];
+                float u, v, d;
+                map_point<CONE>(mu, mv, md, x, y, zz, u, v, d);
+                Col<ADDR> col;
+                col.set(u, W);
+                float val = fetch<ADDR>(img, W, H, col, v - 0.5f);
+                if constexpr (CONE) {
+                    const float cf = 1.0f / fmaf(mc.x, x, fmaf(mc.y, y, fmaf(mc.z, zz, mc.w)));
+                    val *= cf * cf;
+                }
+#pragma unroll
+                for (int aa = 0; aa < 2; aa++)
+#pragma unroll
+                    for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+                        for (int kk = 0; kk < kTZ; kk++)
+                            acc[aa][bb][kk] += (q == (aa * 2 + bb) * kTZ + kk) ? val : 0.0f;
+            }
+        }
+        __syncthreads();  // all samples of this projection done before the next staging
+    }
+
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        const int ix = X0 + lx + 16 * a;
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int iy = Y0 + ly + 16 * b;
+            if (ix < nx && iy < ny) {
+#pragma unroll
+                for (int k = 0; k < kTZ; k++) {
+                    const int iz = Z0 + k;
+                    if (iz < nz)
+                        store_voxel(vol, ((long) iz * ny + iy) * nx + ix, norm * acc[a][b][k], gmin, gscale, gmax);
+                }
+            }
+        }
     }
 }
 
@@ -326,19 +494,23 @@ int gbp_launch(ufomi_ctx *ctx, const float *d_proj, int W, int H, unsigned n, co
     memcpy(host.data() + zoff, zs.data(), zbytes);
     if ((rc = ufomi_upload(ctx, scratch, host.data(), zoff + zbytes))) return rc;
 
-    // cone beam whose U, D and C rows do not depend on z: share them along z
+    // rows that do not depend on z (U; for cone also D and C) are shared along z
     bool zsep = true;
     for (size_t i = 0; i < n && zsep; i++)
-        zsep = mats[16 * i + 2] == 0.0f && mats[16 * i + 10] == 0.0f && mats[16 * i + 14] == 0.0f;
-    const int mode = f.parallel ? MODE_PARALLEL : (zsep ? MODE_CONE_ZSEP : MODE_CONE);
-    dim3 grid((nx + 63) / 64, (ny + 3) / 4, (nz + kZB - 1) / kZB);
+        zsep = mats[16 * i + 2] == 0.0f && (f.parallel || (mats[16 * i + 10] == 0.0f && mats[16 * i + 14] == 0.0f));
+    const int mode = f.parallel ? (zsep ? MODE_PARALLEL_ZSEP : MODE_PARALLEL) : (zsep ? MODE_CONE_ZSEP : MODE_CONE);
+    const int tiles_x = (nx + kTX - 1) / kTX, tiles_y = (ny + kTY - 1) / kTY, tiles_z = (nz + kTZ - 1) / kTZ;
+    const long n_tiles = (long) tiles_x * tiles_y * tiles_z;
+    if (n_tiles > 0x7fffffffL)
+        return ufomi_fail(UFOMI_ERR_ARG, "volume too large for one launch");
+    dim3 grid((unsigned) n_tiles);
     auto go = [&](auto mode_c, auto addr_c, auto *vol) {
         constexpr int C = decltype(mode_c)::value;
         constexpr int A = decltype(addr_c)::value;
         using ST = std::remove_pointer_t<decltype(vol)>;
-        hipLaunchKernelGGL((gbp_kernel<C, A, ST>), grid, dim3(256), 0, ctx->stream, d_proj, W, H, (int) n,
+        hipLaunchKernelGGL((gbp_lds_kernel<C, A, ST>), grid, dim3(256), 0, ctx->stream, d_proj, W, H, (int) n,
                            (const float4 *) scratch, x0, xs, y0, ys, (const float *) (scratch + zoff), nx, ny, nz,
-                           norm, gmin, gscale, gmax, vol);
+                           norm, gmin, gscale, gmax, vol, tiles_x, tiles_y);
         UFOMI_CHECK_LAUNCH();
         return (int) UFOMI_OK;
     };
@@ -359,6 +531,7 @@ int gbp_launch(ufomi_ctx *ctx, const float *d_proj, int W, int H, unsigned n, co
     };
     switch (mode) {
     case MODE_PARALLEL: return by_addr(std::integral_constant<int, MODE_PARALLEL>());
+    case MODE_PARALLEL_ZSEP: return by_addr(std::integral_constant<int, MODE_PARALLEL_ZSEP>());
     case MODE_CONE_ZSEP: return by_addr(std::integral_constant<int, MODE_CONE_ZSEP>());
     default: return by_addr(std::integral_constant<int, MODE_CONE>());
     }
