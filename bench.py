@@ -16,6 +16,10 @@ Other workloads (--workload):
           projections of 1024^2 into 1024^3 (SOD 2500 px, SDD 5000 px)
   lamino  ramp (N 2048) + laminographic backprojection, tilt 30 deg, 1500
           projections of 1024^2 into 1024^2 x 256
+  batch   256 single-slice requests of uneven ROI (log-uniform in [256,2048]^2)
+          and angle count ({360,720,1200,1800,3600}) as one HIP graph: one
+          segmented ramp launch per FFT size + one persistent backprojection
+          launch, tile / angle chunk picked by the autotuner
 
 Prints one JSON line.  The dominant kernel's average duration is measured
 inside the timed region with HIP events on the stream it runs on and set
@@ -107,16 +111,22 @@ def mean_ms(ev, a, b):
 
 # ----------------------------------------------------------------- workloads
 
-def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage):
-    """Batched parallel-beam FBP (the default workload at S=2048, W=2048, P=1800)."""
+STORAGE_NAMES = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}
+
+
+def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storages):
+    """Batched parallel-beam FBP (the default workload at S=2048, W=2048, P=1800).
+
+    Every storage in `storages` (filtered-sinogram type: f32, f16, bf16) runs the same
+    filter + backprojection steps on the same resident sinograms; the first one is the
+    reported value, the others are reported beside it."""
     dev = torch.device("cuda:0")
     N = 1
     while N < 2 * W:
         N *= 2
     axis = W / 2.0
     step_angle = math.pi / P
-    dtype = {"f32": ufomi.F32, "f16": ufomi.F16, "bf16": ufomi.BF16}[storage]
-    tdtype = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[storage]
+    probe = S // 2  # the slice compared with the FP64 oracle
 
     log("generating %d sinograms %dx%d" % (S, P, W))
     sino = torch.empty((S, P, W), device=dev)
@@ -133,40 +143,67 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
     if noise:
         ctx.add_noise(sino, noise, 1000)
     coef = torch.from_numpy(ufomi.filter_coefficients(2 * N, filt_name)).to(dev)
-    filt = torch.empty((S, P, W), dtype=tdtype, device=dev)
     vol = torch.empty((S, W, W), device=dev)
     torch.cuda.synchronize()
     log("inputs ready (%.2f GB sinograms, %.2f GB volume)" % (sino.numel() * 4 / 1e9, vol.numel() * 4 / 1e9))
 
-    def step(e):
-        if e: e[0].record()
-        ctx.ramp_filter(sino, W, S * P, N, coef, W, filt, dtype)
-        if e: e[1].record()
-        ctx.bp_parallel(filt, W, P, vol, n_slices=S, dtype=dtype, axis_pos=axis)
-        if e: e[2].record()
+    runs = {}
+    for storage in storages:
+        dtype = {"f32": ufomi.F32, "f16": ufomi.F16, "bf16": ufomi.BF16}[storage]
+        tdtype = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[storage]
+        filt = torch.empty((S, P, W), dtype=tdtype, device=dev)
 
-    wall, ev = run_timed(torch, args.steps, args.warmup, step, 3)
-    filt_ms, bp_ms = mean_ms(ev, 0, 1), mean_ms(ev, 1, 2)
-    log("timed %d steps: %.3f s wall; filter %.2f ms, backprojection %.2f ms per step"
-        % (args.steps, wall, filt_ms, bp_ms))
+        def step(e):
+            if e: e[0].record()
+            ctx.ramp_filter(sino, W, S * P, N, coef, W, filt, dtype)
+            if e: e[1].record()
+            ctx.bp_parallel(filt, W, P, vol, n_slices=S, dtype=dtype, axis_pos=axis)
+            if e: e[2].record()
+
+        log("storage %s" % storage)
+        wall, ev = run_timed(torch, args.steps, args.warmup, step, 3)
+        filt_ms, bp_ms = mean_ms(ev, 0, 1), mean_ms(ev, 1, 2)
+        log("timed %d steps: %.3f s wall; filter %.2f ms, backprojection %.2f ms per step"
+            % (args.steps, wall, filt_ms, bp_ms))
+        runs[storage] = dict(wall=wall, filt_ms=filt_ms, bp_ms=bp_ms, slice=vol[probe].cpu().numpy())
+        del filt
+        torch.cuda.empty_cache()
+
     updates = float(S) * W * W * P
-    sample_bytes = 4 if storage == "f32" else 2
-    kernel = "bp_texture_kernel<%d,%s>" % (4 if S >= 4 else (2 if S >= 2 else 1),
-                                           {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[storage])
+    first = storages[0]
+    S_k = 4 if S >= 4 else (2 if S >= 2 else 1)
+    kernel = "bp_texture_kernel<%d,%s>" % (S_k, STORAGE_NAMES[first])
+    sample_bytes = 4 if first == "f32" else 2
+    filt_ms = runs[first]["filt_ms"]
     extra = {
         "filter_kernel_ms": round(filt_ms, 3),
         "filter_hbm_GBps": round(S * P * W * (4 + sample_bytes) / (filt_ms * 1e-3) / 1e9, 1),
         "filter_hbm_frac_of_8TBps": round(S * P * W * (4 + sample_bytes) / (filt_ms * 1e-3) / 8e12, 4),
     }
-    cpu = None
+    variants = {st: {"value": round(updates / (r["wall"] / args.steps) / 1e9, 2),
+                     "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
+                     "bp_kernel_ms": round(r["bp_ms"], 3), "filter_kernel_ms": round(r["filt_ms"], 3),
+                     "bp_kernel": "bp_texture_kernel<%d,%s>" % (S_k, STORAGE_NAMES[st])}
+                for st, r in runs.items()}
+    cpu = accuracy = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         n_cpu = min(8 if S > 1 else 1, S)
-        some = sino[S // 2:S // 2 + n_cpu].cpu().numpy()
-        del sino, filt, vol
+        some = sino[probe:probe + n_cpu].cpu().numpy()
+        del sino, vol
         oracle = oracle_module(threads)
+        # error of every storage against the FP64 oracle (FP64 fft-filter-ifft + FP64 backprojection)
+        t0 = time.perf_counter()
+        ref = oracle.bp_texture_f64(oracle.ramp_chain(some[0], N, oracle.filter_coefficients(2 * N, filt_name), W),
+                                    axis_pos=axis)
+        accuracy = {"vs": "FP64 C oracle (FP64 %s fft-filter-ifft + FP64 backprojection) of slice %d"
+                          % (filt_name, probe)}
+        for st, r in runs.items():
+            variants[st]["rel_l2_vs_fp64"] = float(oracle.rel_l2(r["slice"], ref))
+            accuracy[st] = variants[st]["rel_l2_vs_fp64"]
+        log("FP64 oracle slice %.1f s; rel-L2 %s" % (time.perf_counter() - t0, accuracy))
         log("cpu baseline: oracle on %d slices (%d threads)" % (n_cpu, threads))
-        coef_o = oracle.filter_coefficients(2 * N, "ramp-fromreal")
+        coef_o = oracle.filter_coefficients(2 * N, filt_name)
         reps = 1 if S > 1 else 10
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -180,11 +217,15 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
                          "backprojection of the C oracle, %.1f s" % (n_cpu, S, W, W, P,
                                                                      " x %d repeats" % reps if reps > 1 else "", dt)}
     workload = ("parallel-beam FBP: %d sinogram%s x %d angles x %d px -> %dx%dx%d volume, %s ramp zero-padded "
-                "to %d, %s filtered-sinogram storage" % (S, "s" if S > 1 else "", P, W, S, W, W, filt_name, N, storage))
-    return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=2 * 4, extra=extra,
-                cpu=cpu, workload=workload, dtype="f32" if storage == "f32" else "f32 accumulate, %s storage" % storage,
-                config={"slices": S, "angles": P, "detector": W, "fft_size": N, "storage": storage, "filter": filt_name},
-                traffic_key="%s %dx%dx%d %s" % (kernel, S, P, W, storage),
+                "to %d, %s filtered-sinogram storage" % (S, "s" if S > 1 else "", P, W, S, W, W, filt_name, N, first))
+    if len(storages) > 1:
+        extra["storage_variants"] = variants
+    r = runs[first]
+    return dict(updates=updates, wall=r["wall"], bp_ms=r["bp_ms"], kernel=kernel, bytes_per_update=8,
+                extra=extra, cpu=cpu, accuracy=accuracy, workload=workload,
+                dtype="f32" if first == "f32" else "f32 accumulate, %s storage" % first,
+                config={"slices": S, "angles": P, "detector": W, "fft_size": N, "storage": first, "filter": filt_name},
+                traffic_key="%s %dx%dx%d %s" % (kernel, S, P, W, first),
                 metric="GUPS (voxel*proj/s) for FBP")
 
 
@@ -233,33 +274,159 @@ def general(args, ctx, torch, np, ufomi, kind):
     log("timed %d steps: %.3f s wall; weight+filter %.2f ms, backprojection %.2f ms per step"
         % (args.steps, wall, pre_ms, bp_ms))
     updates = float(nz) * W * W * P
-    mode = 1 if kind == "cone" else 0
-    kernel = "gbp_kernel<%d,2,float>" % mode
-    cpu = None
+    # the kernel instance gbp_launch picks: z-separable maps (no z term in U, D, C) take the 2-D path
+    zsep = bool(np.all(rows[:, 0, 2] == 0) and (kind != "cone" or (np.all(rows[:, 2, 2] == 0)
+                                                               and np.all(rows[:, 3, 2] == 0))))
+    mode = (1 if zsep else 2) if kind == "cone" else (3 if zsep else 0)
+    kernel = "gbp_lds_kernel<%d,2,float>" % mode
+    cpu = accuracy = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         oracle = oracle_module(threads)
-        sub = 8 if kind == "cone" else 4
+        # probe: the central 64 x 64 x 4 voxels and 256 detector rows of the filter
+        probe_z = 4
+        c = W // 2
+        got_sub = vol[nz // 2:nz // 2 + probe_z, c - 32:c + 32, c - 32:c + 32].cpu().numpy()
+        rows_idx = np.linspace(0, P * H - 1, 256).astype(np.int64)
+        src_rows = (weighted if kind == "cone" else projs).reshape(P * H, W)[rows_idx].cpu().numpy()
+        got_rows = filt.reshape(P * H, W)[rows_idx].cpu().numpy()
         f_np = filt.cpu().numpy()
         del projs, weighted, filt, vol
-        x_reg = (-W / 2.0, -W / 2.0 + 64, 1.0)
+        t0 = time.perf_counter()
+        ref_rows = oracle.ramp_chain(src_rows, N, oracle.filter_coefficients(2 * N, "ramp-fromreal"), W)
+        ref_sub = oracle.gbp(f_np, overall_angle=2 * math.pi, region=(0.0, float(probe_z), 1.0),
+                             x_region=(-32.0, 32.0, 1.0), y_region=(-32.0, 32.0, 1.0), **geom)
+        accuracy = {"vs": "FP64 C oracle: FP64 fft-filter-ifft of 256 detector rows; FP64 general backprojection "
+                          "of the central 64x64x%d voxels from the device-filtered projections" % probe_z,
+                    "filter_rel_l2": float(oracle.rel_l2(got_rows, ref_rows)),
+                    "backprojection_rel_l2": float(oracle.rel_l2(got_sub, ref_sub))}
+        log("FP64 oracle probe %.1f s: %s" % (time.perf_counter() - t0, accuracy))
+        sub = 64 if kind == "cone" else 16
+        x_reg = (-128.0, 128.0, 1.0)
         t0 = time.perf_counter()
         oracle.gbp(f_np, overall_angle=2 * math.pi, region=(0.0, float(sub), 1.0), x_region=x_reg,
                    y_region=x_reg, mode="ref32", **geom)
         dt = time.perf_counter() - t0
-        n_up = sub * 64 * 64 * P
+        n_up = sub * 256 * 256 * P
         cpu = {"value": round(n_up / dt / 1e9, 4), "unit": "GUPS", "cores": threads, "kind": "port",
-               "sample": "64x64x%d voxels, all %d projections, float32-semantics general backprojection of the "
-                         "C oracle, %.1f s" % (sub, P, dt)}
+               "sample": "central 256x256x%d voxels, all %d projections, float32-semantics general "
+                         "backprojection of the C oracle, %.1f s" % (sub, P, dt)}
     desc = ("cone-beam FDK: %d projections of %dx%d, SOD %.0f px, SDD %.0f px, cosine weight + Ram-Lak (N %d) -> "
             "%dx%dx%d volume" % (P, H, W, sod, 2 * sod, N, nz, W, W) if kind == "cone" else
             "laminography: %d projections of %dx%d, tilt 30 deg, Ram-Lak (N %d) -> %dx%dx%d volume"
             % (P, H, W, N, nz, W, W))
     return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=4 * 4,
-                extra={"weight_filter_ms": round(pre_ms, 3)}, cpu=cpu, workload=desc, dtype="f32",
+                extra={"weight_filter_ms": round(pre_ms, 3)}, cpu=cpu, accuracy=accuracy, workload=desc, dtype="f32",
                 config={"projections": P, "detector": [H, W], "volume": [nz, W, W], "fft_size": N},
                 traffic_key="%s %s" % (kernel, kind),
                 metric="GUPS (voxel*proj/s) for %s" % ("FDK cone-beam" if kind == "cone" else "laminography"))
+
+
+def batch(args, ctx, torch, np, ufomi):
+    """256 uneven single-slice FBP requests as one HIP graph: segmented ramp filters + one persistent
+    backprojection launch, tile / angle chunk chosen by the autotuner."""
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(2048)
+    n = 256
+    lo, hi = math.log(256), math.log(2048)
+    reqs, shapes = [], []
+    log("generating %d requests" % n)
+    for i in range(n):
+        rw = int(round(math.exp(rng.uniform(lo, hi))))
+        rh = int(round(math.exp(rng.uniform(lo, hi))))
+        P = int(rng.choice([360, 720, 1200, 1800, 3600]))
+        W = max(rw, rh)
+        sino = torch.empty((P, W), device=dev)
+        ctx.sinogram_ellipses(ufomi.phantom_ellipses(3000 + i, 10, W / 2.0), W, P, W / 2.0, math.pi / P, sino)
+        out = torch.empty((rh, rw), device=dev)
+        reqs.append(dict(sinogram=sino, slice=out, roi_x=(W - rw) // 2, roi_y=(W - rh) // 2, roi_width=rw,
+                         roi_height=rh))
+        shapes.append((W, P, rw, rh))
+    b = ufomi.FbpBatch(ctx, reqs, filter="ramp-fromreal")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tile, chunk, tune_ms = b.autotune(reps=2, force=True)
+    tune_s = time.perf_counter() - t0
+    info = b.info()
+    updates = info["voxel_updates"]
+    log("autotuner: tile %d, angle chunk %d (%.2f ms backprojection), %.2f s; %d items on %d workgroups, "
+        "%d filter launches, %.3g voxel updates" % (tile, chunk, tune_ms, tune_s, info["items"], info["workgroups"],
+                                                    info["filter_launches"], updates))
+
+    def step(e):
+        if e: e[0].record()
+        b.run()
+        if e: e[1].record()
+
+    wall, ev = run_timed(torch, args.steps, args.warmup, step, 2)
+    graph_ms = mean_ms(ev, 0, 1)
+    # the persistent kernel and the filters launched directly, for the kernel's own duration
+    ev2 = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    for e in ev2:
+        e[0].record()
+        b.launch(1)
+        e[1].record()
+        b.launch(2)
+        e[2].record()
+    torch.cuda.synchronize()
+    filt_ms, bp_ms = mean_ms(ev2, 0, 1), mean_ms(ev2, 1, 2)
+    log("graph %.2f ms per replay; direct launches: filters %.2f ms, persistent backprojection %.2f ms"
+        % (graph_ms, filt_ms, bp_ms))
+    cpu = accuracy = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        oracle = oracle_module(threads)
+        b.run()
+        torch.cuda.synchronize()
+        # FP64 oracle on the two cheapest requests; the CPU baseline on requests in index order up to ~30 G updates
+        cost = [rw * rh * P for (W, P, rw, rh) in shapes]
+        cheap = sorted(range(n), key=lambda i: cost[i])[:2]
+        accuracy = {"vs": "FP64 C oracle (FP64 ramp-fromreal fft-filter-ifft + FP64 backprojection) of requests %s"
+                          % cheap}
+        worst = 0.0
+        for i in cheap:
+            W, P, rw, rh = shapes[i]
+            N = 1
+            while N < 2 * W:
+                N *= 2
+            r = reqs[i]
+            f64 = oracle.ramp_chain(r["sinogram"].cpu().numpy(), N, oracle.filter_coefficients(2 * N), W)
+            ref = oracle.bp_texture_f64(f64, roi_x=r["roi_x"], roi_y=r["roi_y"], roi_width=rw, roi_height=rh)
+            worst = max(worst, float(oracle.rel_l2(r["slice"].cpu().numpy(), ref)))
+        accuracy["max_rel_l2"] = worst
+        log("accuracy %s" % accuracy)
+        sample, acc_up = [], 0.0
+        for i in range(n):
+            if acc_up >= 30e9:
+                break
+            sample.append(i)
+            acc_up += cost[i]
+        host = [(reqs[i]["sinogram"].cpu().numpy(), shapes[i], reqs[i]) for i in sample]
+        t0 = time.perf_counter()
+        for s_np, (W, P, rw, rh), r in host:
+            N = 1
+            while N < 2 * W:
+                N *= 2
+            f32 = oracle.ramp_chain(s_np, N, oracle.filter_coefficients(2 * N), W).astype(np.float32)
+            oracle.bp_texture_ref32(f32, roi_x=r["roi_x"], roi_y=r["roi_y"], roi_width=rw, roi_height=rh)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(acc_up / dt / 1e9, 3), "unit": "GUPS", "cores": threads, "kind": "port",
+               "sample": "requests 0..%d of %d (%.3g voxel updates): FP64 fft-filter-ifft + float32-semantics "
+                         "backprojection of the C oracle, %.1f s" % (len(sample) - 1, n, acc_up, dt)}
+    b.close()
+    kernel = "bp_batch_kernel<%d,%d,float>" % (tile, chunk)
+    extra = {"graph_ms": round(graph_ms, 3), "filter_kernels_ms": round(filt_ms, 3),
+             "autotune": {"tile": tile, "angle_chunk": chunk, "tuned_bp_ms": round(tune_ms, 3),
+                          "seconds": round(tune_s, 2)},
+             "items": info["items"], "workgroups": info["workgroups"], "filter_launches": info["filter_launches"]}
+    return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=8, extra=extra, cpu=cpu,
+                accuracy=accuracy, dtype="f32",
+                workload="uneven batched FBP: 256 requests, ROI log-uniform in [256,2048]^2, angles from "
+                         "{360,720,1200,1800,3600}, detector width max(ROI), Ram-Lak zero-padded to 2^k >= 2W, "
+                         "one HIP graph (segmented ramp launches + one persistent backprojection launch)",
+                config={"requests": n, "voxel_updates": updates, "seed": 2048},
+                traffic_key="%s batch" % kernel,
+                metric="GUPS (voxel*proj/s) for uneven batched FBP")
 
 
 def main():
@@ -267,9 +434,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["fbp2048", "fbp512", "cone", "lamino"], default="fbp2048")
+    ap.add_argument("--workload", choices=["fbp2048", "fbp512", "cone", "lamino", "batch"], default="fbp2048")
     ap.add_argument("--slices", type=int, default=2048)
-    ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32")
+    ap.add_argument("--storage", choices=["f32", "f16", "bf16", "all"], default=None,
+                    help="filtered-sinogram storage; 'all' (the fbp2048 default) reports f32 as the value "
+                         "and f16 / bf16 beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -287,10 +456,14 @@ def main():
     ctx = ufomi.Context(0)
     ctx.set_stream(torch.cuda.current_stream(0))
 
+    storage = args.storage or ("all" if args.workload == "fbp2048" else "f32")
+    storages = ["f32", "f16", "bf16"] if storage == "all" else [storage]
     if args.workload == "fbp2048":
-        r = fbp_parallel(args, ctx, torch, np, ufomi, args.slices, 2048, 1800, "shepp-logan", 0.0, args.storage)
+        r = fbp_parallel(args, ctx, torch, np, ufomi, args.slices, 2048, 1800, "shepp-logan", 0.0, storages)
     elif args.workload == "fbp512":
-        r = fbp_parallel(args, ctx, torch, np, ufomi, 1, 512, 512, "ramp-fromreal", 0.01, args.storage)
+        r = fbp_parallel(args, ctx, torch, np, ufomi, 1, 512, 512, "ramp-fromreal", 0.01, storages)
+    elif args.workload == "batch":
+        r = batch(args, ctx, torch, np, ufomi)
     else:
         r = general(args, ctx, torch, np, ufomi, args.workload)
 
@@ -315,7 +488,10 @@ def main():
         "peak_source": "measured: ufomi_bench_lds_read (ds_read_b128 on every CU)",
         "hbm_copy_GBps": round(hbm_copy / 1e9, 1),
     }
+    variants = r["extra"].pop("storage_variants", None)
     roof.update(r["extra"])
+    if traffic:
+        roof["hbm_achieved_GBps"] = round(traffic / bp_s / 1e9, 1)
     result = {
         "metric": r["metric"],
         "value": round(r["updates"] / (r["wall"] / args.steps) / 1e9, 2),
@@ -332,6 +508,10 @@ def main():
         "config": dict({"workload": r["workload"]}, **r["config"]),
         "roofline": roof,
     }
+    if variants:
+        result["storage_variants"] = variants
+    if r.get("accuracy"):
+        result["accuracy"] = r["accuracy"]
     if r["cpu"] is not None:
         result["cpu_baseline"] = r["cpu"]
     print(json.dumps(result), flush=True)

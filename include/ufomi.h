@@ -153,6 +153,71 @@ int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int 
                       int n_slices, size_t sino_slice_pitch, const ufomi_bp_params *params,
                       float *d_out, size_t out_slice_pitch);
 
+/* --------------------------------------------------- batched FBP requests */
+
+/*
+ * Uneven batches of single-slice FBP requests, run as one HIP graph: one
+ * segmented ramp-filter launch per FFT size, then ONE persistent
+ * backprojection launch whose workgroups take (request, voxel tile) items from
+ * a device counter, largest angle counts first.  No counterpart in the
+ * reference, which reconstructs one stream of equal-size sinograms per
+ * pipeline: each request computes what `fft size-x=fft_size ! filter !
+ * ifft crop-width=width ! backproject` (src/ufo-fft-task.c:226-352,
+ * src/ufo-filter-task.c:117-144, src/ufo-ifft-task.c:208-311,
+ * src/ufo-backproject-task.c:125-267) computes for its sinogram, with the
+ * properties in `params` (texture mode).  For callers that queue many small
+ * reconstructions (region-of-interest previews, parameter scans).
+ */
+typedef struct {
+    const float *sinogram;      /* device [num_angles][width] float32 */
+    float *slice;               /* device [roi_height][roi_width] float32 */
+    int width;                  /* detector pixels W */
+    int num_angles;             /* sinogram height P_in */
+    int fft_size;               /* power of two >= width, 16..8192; 0 -> smallest >= 2 width */
+    ufomi_bp_params params;     /* `backproject` properties; mode must be texture */
+} ufomi_fbp_request;
+
+typedef struct ufomi_fbp_batch ufomi_fbp_batch;
+
+typedef struct {
+    double voxel_updates;       /* sum over requests of roi_w * roi_h * P_in */
+    int tile;                   /* voxel tile side of the persistent kernel */
+    int angle_chunk;            /* angles staged per LDS pass */
+    int workgroups;             /* persistent grid size */
+    int items;                  /* (request, tile) work items */
+    int filter_launches;        /* segmented ramp launches (one per FFT size) */
+    size_t filtered_bytes;      /* device buffer of filtered rows owned by the batch */
+} ufomi_fbp_batch_info;
+
+/*
+ * Validates every request (the errors of ufomi_bp_parallel / ufomi_ramp_filter,
+ * naming the request index), computes the filter coefficients
+ * (filter_type, scale as ufomi_filter_coefficients, other parameters at their
+ * defaults) and the FP64 trig tables, allocates the filtered-row buffer in
+ * `storage_dtype` (f32, f16, bf16) and builds the graph with tile 32, angle
+ * chunk 16.  The batch keeps the request pointers: run it while they are valid.
+ */
+int ufomi_fbp_batch_create(ufomi_ctx *ctx, const ufomi_fbp_request *requests, int n, int filter_type,
+                           float filter_scale, int storage_dtype, ufomi_fbp_batch **out);
+/* Choose tile (32, 64) and angle chunk (8, 16, 32; 8, 16 for tile 64); rebuilds the graph. */
+int ufomi_fbp_batch_configure(ufomi_ctx *ctx, ufomi_fbp_batch *batch, int tile, int angle_chunk);
+/*
+ * Autotuner: filters once, then times every tile / angle-chunk configuration
+ * of the persistent backprojection `reps` times on this batch (HIP events on
+ * the context's stream) and configures the fastest.  Results are cached per
+ * process under the batch's shape signature; a cached choice is applied
+ * without timing unless `force`.  Outputs (may be NULL): the choice and its
+ * mean backprojection time in ms (0 when taken from the cache).
+ */
+int ufomi_fbp_batch_autotune(ufomi_ctx *ctx, ufomi_fbp_batch *batch, int reps, int force, int *tile,
+                             int *angle_chunk, float *ms);
+/* One replay of the graph on the context's stream (asynchronous). */
+int ufomi_fbp_batch_run(ufomi_ctx *ctx, ufomi_fbp_batch *batch);
+/* The same work launched directly, stage 1 = filters, 2 = backprojection, 3 = both. */
+int ufomi_fbp_batch_launch(ufomi_ctx *ctx, ufomi_fbp_batch *batch, int stages);
+int ufomi_fbp_batch_get_info(const ufomi_fbp_batch *batch, ufomi_fbp_batch_info *info);
+int ufomi_fbp_batch_destroy(ufomi_fbp_batch *batch);
+
 /* ---------------------------------------------------- cone-beam weighting */
 
 /*

@@ -13,7 +13,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-FILTERS = {"ramp": 0, "ramp-fromreal": 1, "butterworth": 2, "faris-byer": 3, "hamming": 4, "bh3": 5}
+FILTERS = {"ramp": 0, "ramp-fromreal": 1, "butterworth": 2, "faris-byer": 3, "hamming": 4, "bh3": 5,
+           "shepp-logan": 6}
 ADDR = {"clamp": 0, "clamp_to_edge": 1}
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

@@ -21,7 +21,9 @@
  *   orc_filter_coefficients closed forms of ufo-filter-task.c:171-276 evaluated
  *                          independently in numpy, ramp-fromreal against the
  *                          closed cosine sum H_k = 1/4 - 2/pi^2 sum_odd cos/n^2,
- *                          realness and evenness of H.
+ *                          realness and evenness of H; shepp-logan (an
+ *                          extension with no counterpart in the reference)
+ *                          against sin(pi |f|)/pi and its 2/pi Nyquist ratio.
  *   orc_ramp_chain         numpy rfft/irfft filtering of the same rows,
  *                          fft->ifft identity (test_fft.py:114-129 semantics).
  *   orc_bp_texture_*       brute-force numpy on tiny inputs, constant sinogram
@@ -111,7 +113,7 @@ void orc_dft_naive(const double *re, const double *im, double *out_re, double *o
 /* ------------------------------------------------------------------------- */
 
 enum { ORC_FILTER_RAMP = 0, ORC_FILTER_RAMP_FROMREAL, ORC_FILTER_BUTTERWORTH,
-       ORC_FILTER_FARIS_BYER, ORC_FILTER_HAMMING, ORC_FILTER_BH3 };
+       ORC_FILTER_FARIS_BYER, ORC_FILTER_HAMMING, ORC_FILTER_BH3, ORC_FILTER_SHEPP_LOGAN };
 
 /* ufo-filter-task.c:162-169 -- bin j in (N/2, N) copies bin N-j. */
 static void orc_mirror(float *f, unsigned width)
@@ -150,6 +152,7 @@ int orc_filter_coefficients(int type, unsigned width, float scale, float cutoff,
             }
             break;
         case ORC_FILTER_RAMP_FROMREAL:      /* :184-195 */
+        case ORC_FILTER_SHEPP_LOGAN:        /* no counterpart: ramp-fromreal, windowed below */
             out[0] = out[1] = (float) (0.25 * scale);
             for (unsigned k = 1; k < width / 4 + 1; k++) {
                 out[2 * k] = (k % 2) ? (float) (-scale / ((double) (k * k) * M_PI * M_PI)) : 0.0f;
@@ -199,7 +202,7 @@ int orc_filter_coefficients(int type, unsigned width, float scale, float cutoff,
     }
     orc_mirror (out, width);
 
-    if (type == ORC_FILTER_RAMP_FROMREAL) {
+    if (type == ORC_FILTER_RAMP_FROMREAL || type == ORC_FILTER_SHEPP_LOGAN) {
         const int n = (int) (width / 2);
         double *re = malloc (n * sizeof (double)), *im = malloc (n * sizeof (double));
         for (int k = 0; k < n; k++) {
@@ -207,6 +210,16 @@ int orc_filter_coefficients(int type, unsigned width, float scale, float cutoff,
             im[k] = out[2 * k + 1];
         }
         orc_fft_c2c (re, im, n, -1);
+        if (type == ORC_FILTER_SHEPP_LOGAN) {
+            /* Shepp-Logan window sin(pi f)/(pi f) with f the bin frequency in cycles per
+             * sample, |f| <= 1/2, so the window is 2/pi at Nyquist. */
+            for (int k = 1; k < n; k++) {
+                const double f = (double) (k <= n / 2 ? k : n - k) / n;
+                const double w = sin (M_PI * f) / (M_PI * f);
+                re[k] *= w;
+                im[k] *= w;
+            }
+        }
         for (int k = 0; k < n; k++) {
             out[2 * k] = (float) re[k];
             out[2 * k + 1] = (float) im[k];

@@ -45,7 +45,8 @@ def test_version_and_no_context_without_gpu():
     assert ufomi.load().ufomi_version().decode().startswith("ufomi")
 
 
-@pytest.mark.parametrize("filt", ["ramp", "ramp-fromreal", "butterworth", "faris-byer", "hamming", "bh3"])
+@pytest.mark.parametrize("filt", ["ramp", "ramp-fromreal", "butterworth", "faris-byer", "hamming", "bh3",
+                                  "shepp-logan"])
 @pytest.mark.parametrize("width", [8, 64, 1024, 8192])
 def test_filter_coefficients_match_oracle(filt, width):
     a = ufomi.filter_coefficients(width, filt)

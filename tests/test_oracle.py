@@ -75,6 +75,24 @@ def test_ramp_fromreal_closed_form(n):
     assert 0 < c[0] < 0.25 / n * 4                   # small positive DC
 
 
+@pytest.mark.parametrize("n", [64, 1024, 4096])
+def test_shepp_logan_closed_form(n):
+    """The Shepp-Logan-windowed ramp |f| sin(pi f)/(pi f) = sin(pi |f|)/pi, f in cycles per
+    sample: the oracle's spectrum is within the truncation error of the real-space Ram-Lak
+    kernel (~0.2/N, the same as ramp-fromreal's distance to |f|) of that closed form, and
+    exactly 2/pi times ramp-fromreal at Nyquist."""
+    sl = oracle.filter_coefficients(2 * n, "shepp-logan")
+    rf = oracle.filter_coefficients(2 * n, "ramp-fromreal")[0::2].astype(np.float64)
+    k = np.arange(n)
+    f = np.minimum(k, n - k) / n
+    assert np.abs(sl[0::2] - np.sin(np.pi * f) / np.pi).max() < 0.21 / n
+    assert np.abs(rf - f).max() < 0.21 / n
+    assert sl[n] == pytest.approx(rf[n // 2] * 2 / np.pi, rel=1e-6)
+    assert sl[0] == np.float32(rf[0])
+    assert np.array_equal(sl[0::2], sl[1::2])
+    _mirror_ok(sl)
+
+
 @pytest.mark.parametrize("scale", [1.0, 0.5])
 def test_ramp_closed_form(scale):
     n = 64

@@ -32,7 +32,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kTile = 32;       // voxels per tile side
 constexpr int kA = 16;          // angles per chunk
-constexpr int kStride = 64;     // LDS entries per staged angle
 constexpr float kPiF = 3.14159274101257324219f;  // M_PI_F
 
 template <int S> struct Entry;
@@ -48,18 +47,43 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return b;
 }
 
+// Raw stored values of a filtered sinogram and their conversion to float.
+template <typename T> struct Stage;
+// Loads go through the caller's pointer type, which may carry the global
+// address space.
+template <> struct Stage<float> {
+    using R = float;
+    template <typename Ptr> static __device__ __forceinline__ R load(Ptr p, long i) { return p[i]; }
+    static __device__ __forceinline__ float cvt(R v) { return v; }
+};
+template <> struct Stage<__half> {
+    using R = unsigned short;
+    template <typename Ptr> static __device__ __forceinline__ R load(Ptr p, long i)
+    {
+        return __builtin_bit_cast(R, p[i]);
+    }
+    static __device__ __forceinline__ float cvt(R v) { return (float) __builtin_bit_cast(_Float16, v); }
+};
+template <> struct Stage<__hip_bfloat16> {
+    using R = unsigned short;
+    template <typename Ptr> static __device__ __forceinline__ R load(Ptr p, long i)
+    {
+        return __builtin_bit_cast(R, p[i]);
+    }
+    static __device__ __forceinline__ float cvt(R v) { return __uint_as_float((unsigned) v << 16); }
+};
+
 // Per-angle staging values for this tile: (sin, cos, a' = axis - 0.5 - lo, lo).
 struct AngleSpan {
     float s, c, a;
     int lo;
 };
 
-__device__ __forceinline__ AngleSpan angle_span(const float2 *__restrict__ trig, int p, int P,
-                                                float bx0, float bx1, float by0, float by1, float am)
+__device__ __forceinline__ AngleSpan angle_span(float2 sc, bool valid, float bx0, float bx1, float by0, float by1,
+                                                float am)
 {
     AngleSpan r;
-    if (p < P) {
-        const float2 sc = trig[p];
+    if (valid) {
         r.s = sc.x;
         r.c = sc.y;
         const float umin = am + fminf(by0 * r.s, by1 * r.s) + fminf(bx0 * r.c, bx1 * r.c);
@@ -72,145 +96,169 @@ __device__ __forceinline__ AngleSpan angle_span(const float2 *__restrict__ trig,
     return r;
 }
 
-template <int S, typename InT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void bp_texture_kernel(
-    const InT *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
-    float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
-    float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y)
+// One tile of TILE x TILE voxels of S slices, angles in chunks of A.  Thread
+// (lx, ly) owns voxels (lx + 16 kx, ly + 16 ky), kx, ky < K = TILE / 16.
+// Every thread of the workgroup calls it (it synchronises); it leaves the LDS
+// buffers free for the next call.  The detector span of a tile at one angle is
+// at most (TILE - 1) sqrt 2 + 3 entries, inside STRIDE = 2 TILE.
+template <int S, int TILE, int A, typename InT, typename InPtr = const InT *, typename OutPtr = float *>
+__device__ __forceinline__ void bp_tile(
+    InPtr __restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
+    float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices, int slice0,
+    OutPtr __restrict__ out, long out_pitch, int x0, int y0)
 {
     using E = typename Entry<S>::T;
+    constexpr int K = TILE / 16;
+    constexpr int STRIDE = 2 * TILE;
+    constexpr int ROWS = kThreads / STRIDE;           // angles staged per pass
     constexpr int NPLANE = S == 4 ? 2 : 1;
-    constexpr int PASSES = kA * kStride / kThreads;  // staging passes per chunk (4)
-    __shared__ E lds[NPLANE][kA][kStride];
-    __shared__ float4 par[kA];
-
-    const int nb = tiles_x * tiles_y * ((n_slices + S - 1) / S);
-    const int b = xcd_remap(blockIdx.x, nb);
-    const int tiles = tiles_x * tiles_y;
-    const int g = b / tiles;
-    const int ty = (b % tiles) / tiles_x, tx = b % tiles_x;
-    const int x0 = tx * kTile, y0 = ty * kTile;
-    const int slice0 = g * S;
+    constexpr int PASSES = A * STRIDE / kThreads;     // staging passes per chunk
+    static_assert(A % ROWS == 0, "angle chunk must be a multiple of the staged rows per pass");
+    __shared__ E lds[NPLANE][A][STRIDE];
+    __shared__ float4 par[A];
 
     const int tid = threadIdx.x;
     const int lx = tid & 15, ly = tid >> 4;
 
     // Voxel-centre coordinates exactly as backproject.cl:63-64 writes them.
-    float bx[2], by[2];
+    float bx[K], by[K];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < K; k++) {
         bx[k] = (float) (x0 + lx + 16 * k) - axis + roi_x + 0.5f;
         by[k] = (float) (y0 + ly + 16 * k) - axis + roi_y + 0.5f;
     }
     // Tile corners for the span computation.
-    const float cbx0 = (float) x0 - axis + roi_x + 0.5f, cbx1 = (float) (x0 + kTile - 1) - axis + roi_x + 0.5f;
-    const float cby0 = (float) y0 - axis + roi_y + 0.5f, cby1 = (float) (y0 + kTile - 1) - axis + roi_y + 0.5f;
+    const float cbx0 = (float) x0 - axis + roi_x + 0.5f, cbx1 = (float) (x0 + TILE - 1) - axis + roi_x + 0.5f;
+    const float cby0 = (float) y0 - axis + roi_y + 0.5f, cby1 = (float) (y0 + TILE - 1) - axis + roi_y + 0.5f;
     const float am = axis - 0.5f;
 
-    const InT *src[S];
+    InPtr src[S];
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const int sl = slice0 + s < n_slices ? slice0 + s : n_slices - 1;
         src[s] = sino + (long) sl * sino_pitch;
     }
 
-    // Staging registers: this thread fills entry e of angles (tid >> 6) + 4 m.
-    const int e = tid & (kStride - 1);
-    const int arow = tid >> 6;
-    float sv[PASSES][S], sd[PASSES][S];
+    // Staging registers: this thread fills entry e of angles arow + ROWS m.
+    const int e = tid & (STRIDE - 1);
+    const int arow = tid / STRIDE;
+    float sv[S], sd[S];
     AngleSpan sp[PASSES];
 
+    // fetch() issues the loads of the next chunk (clamped addresses, no
+    // branches) and keeps the raw stored values; commit() converts them after
+    // the current chunk's interpolation, so the loads stay in flight meanwhile.
+    using R = typename Stage<InT>::R;
+    R r0[PASSES][S], r1[PASSES][S];
+    bool in0[PASSES], in1[PASSES];
+    // (sin, cos) of the angles the next fetch() stages, loaded one chunk ahead
+    // so that no fetch waits for the trig table before issuing its loads.
+    float2 tq[PASSES];
+    auto load_trig = [&](int p0) {
+#pragma unroll
+        for (int m = 0; m < PASSES; m++)
+            tq[m] = trig[min(p0 + arow + ROWS * m, P - 1)];
+    };
     auto fetch = [&](int p0) {
 #pragma unroll
         for (int m = 0; m < PASSES; m++) {
-            const int pa = arow + 4 * m;
+            const int pa = arow + ROWS * m;
             const int p = p0 + pa;
-            sp[m] = angle_span(trig, p, P, cbx0, cbx1, cby0, cby1, am);
+            sp[m] = angle_span(tq[m], p < P, cbx0, cbx1, cby0, cby1, am);
             const int j = sp[m].lo + e;
-            const bool in0 = p < P && j >= 0 && j < W;
-            const bool in1 = p < P && j + 1 >= 0 && j + 1 < W;
+            in0[m] = p < P && j >= 0 && j < W;
+            in1[m] = p < P && j + 1 >= 0 && j + 1 < W;
+            const int j0 = min(max(j, 0), W - 1), j1 = min(max(j + 1, 0), W - 1);
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                const InT *row = src[s] + (long) (p < P ? p : 0) * W;
-                const float v0 = in0 ? ufomi_load(row, j) : 0.0f;
-                const float v1 = in1 ? ufomi_load(row, j + 1) : 0.0f;
-                sv[m][s] = v0;
-                sd[m][s] = v1 - v0;
+                InPtr row = src[s] + (long) (p < P ? p : 0) * W;
+                r0[m][s] = Stage<InT>::load(row, j0);
+                r1[m][s] = Stage<InT>::load(row, j1);
             }
         }
+        load_trig(p0 + A);
     };
     auto commit = [&]() {
 #pragma unroll
         for (int m = 0; m < PASSES; m++) {
-            const int pa = arow + 4 * m;
+            const int pa = arow + ROWS * m;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const float v0 = in0[m] ? Stage<InT>::cvt(r0[m][s]) : 0.0f;
+                const float v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
+                sv[s] = v0;
+                sd[s] = v1 - v0;
+            }
             if constexpr (S == 1) {
-                lds[0][pa][e] = make_float2(sv[m][0], sd[m][0]);
+                lds[0][pa][e] = make_float2(sv[0], sd[0]);
             } else if constexpr (S == 2) {
-                lds[0][pa][e] = make_float4(sv[m][0], sd[m][0], sv[m][1], sd[m][1]);
+                lds[0][pa][e] = make_float4(sv[0], sd[0], sv[1], sd[1]);
             } else {
-                lds[0][pa][e] = make_float4(sv[m][0], sd[m][0], sv[m][1], sd[m][1]);
-                lds[1][pa][e] = make_float4(sv[m][2], sd[m][2], sv[m][3], sd[m][3]);
+                lds[0][pa][e] = make_float4(sv[0], sd[0], sv[1], sd[1]);
+                lds[1][pa][e] = make_float4(sv[2], sd[2], sv[3], sd[3]);
             }
             if (e == 0)
                 par[pa] = make_float4(sp[m].s, sp[m].c, sp[m].a, 0.0f);
         }
     };
 
-    float acc[2][2][S];
+    float acc[K][K][S];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < K; i++)
 #pragma unroll
-        for (int k = 0; k < 2; k++)
+        for (int k = 0; k < K; k++)
 #pragma unroll
             for (int s = 0; s < S; s++)
                 acc[i][k][s] = 0.0f;
 
-    const int nchunks = (P + kA - 1) / kA;
+    const int nchunks = (P + A - 1) / A;
+    load_trig(0);
     fetch(0);
     commit();
     __syncthreads();
     for (int ch = 0; ch < nchunks; ch++) {
         if (ch + 1 < nchunks)
-            fetch((ch + 1) * kA);
+            fetch((ch + 1) * A);
 #pragma unroll 2
-        for (int pa = 0; pa < kA; pa++) {
+        for (int pa = 0; pa < A; pa++) {
             const float4 pr = par[pa];
-            // Addresses and weights of the 4 voxels first, then all LDS reads,
-            // then the accumulation: keeps 4 (8) reads in flight per wave.
-            int iv[4];
-            float fv[4];
+            // Addresses and weights of all K*K voxels first, then all LDS reads,
+            // then the accumulation: keeps K*K (2 K*K) reads in flight per wave.
+            constexpr int KK = K * K;
+            int iv[KK];
+            float fv[KK];
 #pragma unroll
-            for (int ky = 0; ky < 2; ky++) {
+            for (int ky = 0; ky < K; ky++) {
                 const float t = fmaf(by[ky], pr.x, pr.z);
 #pragma unroll
-                for (int kx = 0; kx < 2; kx++) {
+                for (int kx = 0; kx < K; kx++) {
                     const float u = fmaf(bx[kx], pr.y, t);
-                    iv[2 * ky + kx] = (int) u;
-                    fv[2 * ky + kx] = __builtin_amdgcn_fractf(u);
+                    iv[K * ky + kx] = (int) u;
+                    fv[K * ky + kx] = __builtin_amdgcn_fractf(u);
                 }
             }
-            E q[4];
+            E q[KK];
 #pragma unroll
-            for (int v = 0; v < 4; v++)
+            for (int v = 0; v < KK; v++)
                 q[v] = lds[0][pa][iv[v]];
             if constexpr (S == 1) {
 #pragma unroll
-                for (int v = 0; v < 4; v++)
-                    acc[v >> 1][v & 1][0] += fmaf(fv[v], q[v].y, q[v].x);
+                for (int v = 0; v < KK; v++)
+                    acc[v / K][v % K][0] += fmaf(fv[v], q[v].y, q[v].x);
             } else {
-                float4 q2[4];
+                float4 q2[KK];
                 if constexpr (S == 4) {
 #pragma unroll
-                    for (int v = 0; v < 4; v++)
+                    for (int v = 0; v < KK; v++)
                         q2[v] = lds[1][pa][iv[v]];
                 }
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    acc[v >> 1][v & 1][0] += fmaf(fv[v], q[v].y, q[v].x);
-                    acc[v >> 1][v & 1][1] += fmaf(fv[v], q[v].w, q[v].z);
+                for (int v = 0; v < KK; v++) {
+                    acc[v / K][v % K][0] += fmaf(fv[v], q[v].y, q[v].x);
+                    acc[v / K][v % K][1] += fmaf(fv[v], q[v].w, q[v].z);
                     if constexpr (S == 4) {
-                        acc[v >> 1][v & 1][2] += fmaf(fv[v], q2[v].y, q2[v].x);
-                        acc[v >> 1][v & 1][3] += fmaf(fv[v], q2[v].w, q2[v].z);
+                        acc[v / K][v % K][2] += fmaf(fv[v], q2[v].y, q2[v].x);
+                        acc[v / K][v % K][3] += fmaf(fv[v], q2[v].w, q2[v].z);
                     }
                 }
             }
@@ -224,10 +272,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 
     const float fp = (float) P;
 #pragma unroll
-    for (int ky = 0; ky < 2; ky++) {
+    for (int ky = 0; ky < K; ky++) {
         const int y = y0 + ly + 16 * ky;
 #pragma unroll
-        for (int kx = 0; kx < 2; kx++) {
+        for (int kx = 0; kx < K; kx++) {
             const int x = x0 + lx + 16 * kx;
             if (x < roi_w && y < roi_h) {
 #pragma unroll
@@ -240,6 +288,57 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             }
         }
     }
+}
+
+template <int S, typename InT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void bp_texture_kernel(
+    const InT *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
+    float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
+    float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y)
+{
+    const int nb = tiles_x * tiles_y * ((n_slices + S - 1) / S);
+    const int b = xcd_remap(blockIdx.x, nb);
+    const int tiles = tiles_x * tiles_y;
+    const int g = b / tiles;
+    const int ty = (b % tiles) / tiles_x, tx = b % tiles_x;
+    bp_tile<S, kTile, kA, InT>(sino, sino_pitch, W, P, trig, axis, roi_x, roi_y, roi_w, roi_h, n_slices, g * S,
+                               out, out_pitch, tx * kTile, ty * kTile);
+}
+
+// Persistent backprojection of a batch of single-slice requests of uneven
+// size: each workgroup takes (request, tile) items from a global counter until
+// the list is exhausted.  The list is sorted by decreasing cost, so the tail
+// of the launch is made of the cheapest tiles.
+template <int TILE, int A, typename InT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void bp_batch_kernel(
+    const ufomi_bp_batch_req *__restrict__ reqs, const int2 *__restrict__ items, int n_items,
+    int *__restrict__ counter, const float2 *__restrict__ trig)
+{
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0)
+            next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = next;
+        if (it >= n_items)
+            break;
+        const int2 w = items[it];
+        const ufomi_bp_batch_req &r = reqs[w.x];
+        const int ty = w.y / r.tiles_x, tx = w.y % r.tiles_x;
+        // The request's buffers are global memory: say so, so that the loads and
+        // stores are global (not flat) instructions and do not hold up LDS waits.
+        using GIn = __attribute__((address_space(1))) const InT;
+        using GOut = __attribute__((address_space(1))) float;
+        bp_tile<1, TILE, A, InT, GIn *, GOut *>((GIn *) r.sino, 0, r.W, r.P, trig + r.trig0, r.axis, r.roi_x,
+                                                r.roi_y, r.roi_w, r.roi_h, 1, 0, (GOut *) r.out, 0, tx * TILE,
+                                                ty * TILE);
+    }
+}
+
+__global__ void reset_counter_kernel(int *counter)
+{
+    if (threadIdx.x == 0)
+        *counter = 0;
 }
 
 // backproject_nearest (backproject.cl:24-48): flat index (int)(p * roi_w + h)
@@ -395,4 +494,62 @@ extern "C" int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, 
     case UFOMI_BF16: return by_s((const __hip_bfloat16 *) d_sino);
     default: return by_s((const float *) d_sino);
     }
+}
+
+// ---- persistent batch kernel launchers
+
+namespace {
+
+template <typename F>
+int by_batch_config(int tile, int achunk, int dtype, F f)
+{
+    auto with_t = [&](auto *tag) {
+        using InT = std::remove_pointer_t<decltype(tag)>;
+        if (tile == 32 && achunk == 8) return f(bp_batch_kernel<32, 8, InT>);
+        if (tile == 32 && achunk == 16) return f(bp_batch_kernel<32, 16, InT>);
+        if (tile == 32 && achunk == 32) return f(bp_batch_kernel<32, 32, InT>);
+        if (tile == 64 && achunk == 8) return f(bp_batch_kernel<64, 8, InT>);
+        if (tile == 64 && achunk == 16) return f(bp_batch_kernel<64, 16, InT>);
+        return ufomi_fail(UFOMI_ERR_ARG, "no persistent backprojection kernel for tile %d, angle chunk %d",
+                          tile, achunk);
+    };
+    switch (dtype) {
+    case UFOMI_F16: return with_t((__half *) nullptr);
+    case UFOMI_BF16: return with_t((__hip_bfloat16 *) nullptr);
+    case UFOMI_F32: return with_t((float *) nullptr);
+    default: return ufomi_fail(UFOMI_ERR_ARG, "unknown sinogram dtype %d", dtype);
+    }
+}
+
+}  // namespace
+
+bool ufomi_bp_batch_config_ok(int tile, int achunk)
+{
+    return (tile == 32 && (achunk == 8 || achunk == 16 || achunk == 32)) ||
+           (tile == 64 && (achunk == 8 || achunk == 16));
+}
+
+int ufomi_bp_batch_blocks_per_cu(int tile, int achunk, int dtype, int *blocks)
+{
+    return by_batch_config(tile, achunk, dtype, [&](auto kern) {
+        int n = 0;
+        UFOMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kThreads, 0));
+        *blocks = n > 0 ? n : 1;
+        return (int) UFOMI_OK;
+    });
+}
+
+int ufomi_bp_batch_launch(hipStream_t stream, int tile, int achunk, int dtype, int grid,
+                          const ufomi_bp_batch_req *d_reqs, const int2 *d_items, int n_items, int *d_counter,
+                          const float2 *d_trig)
+{
+    if (grid <= 0 || n_items <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid persistent launch (%d workgroups, %d items)", grid, n_items);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, d_counter);
+    UFOMI_CHECK_LAUNCH();
+    return by_batch_config(tile, achunk, dtype, [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, stream, d_reqs, d_items, n_items, d_counter, d_trig);
+        UFOMI_CHECK_LAUNCH();
+        return (int) UFOMI_OK;
+    });
 }

@@ -260,20 +260,21 @@ int launch_by_b(int B, K k)
 // remain for N = 4 and 8.
 // ------------------------------------------------------------------------
 
+// Block `blk` of a ramp-filter launch over `rows` rows of width W: G transforms
+// (2 G rows when `pair`, the real-even coefficient case) of the rows' block.
 template <int N, typename OutT>
-__global__ __launch_bounds__(kThreads) void ramp16_kernel(
-    const float *__restrict__ in, int W, long rows, const float *__restrict__ coef, const int *__restrict__ pair_flag,
-    int crop, OutT *__restrict__ out, float scale, const float2 *__restrict__ tw)
+__device__ __forceinline__ void ramp16_block(
+    const float *__restrict__ in, int W, long rows, const float *__restrict__ coef, bool pair,
+    int crop, OutT *__restrict__ out, float scale, const float2 *__restrict__ tw, long blk)
 {
     using P = fft16::Plan<N>;
     constexpr int R0 = P::radix(0), RL = P::radix(P::NPASS - 1);
     extern __shared__ float2 lds[];
     const int f = threadIdx.x / P::TF, t = threadIdx.x % P::TF;
     float2 *x = lds + (size_t) f * N;
-    const bool pair = *pair_flag != 0;
-    if (pair && 2 * (long) blockIdx.x * P::G >= rows)
+    if (pair && 2 * blk * P::G >= rows)
         return;  // whole block beyond the last pair
-    const long unit = (long) blockIdx.x * P::G + f;
+    const long unit = blk * P::G + f;
     const long r0 = pair ? 2 * unit : unit, r1 = r0 + 1;
     const bool v0 = r0 < rows, v1 = pair && r1 < rows;
     const float *a = in + r0 * W, *b = in + r1 * W;
@@ -306,6 +307,35 @@ __global__ __launch_bounds__(kThreads) void ramp16_kernel(
             if (v1) ufomi_store(out, r1 * crop + e, v[s].y * scale);
         }
     }
+}
+
+template <int N, typename OutT>
+__global__ __launch_bounds__(kThreads) void ramp16_kernel(
+    const float *__restrict__ in, int W, long rows, const float *__restrict__ coef, const int *__restrict__ pair_flag,
+    int crop, OutT *__restrict__ out, float scale, const float2 *__restrict__ tw)
+{
+    ramp16_block<N, OutT>(in, W, rows, coef, *pair_flag != 0, crop, out, scale, tw, blockIdx.x);
+}
+
+// Ramp filter over segments of uneven width sharing N and the coefficients:
+// block b belongs to the last segment whose block0 <= b; each segment's rows
+// are cropped to its own width.
+template <int N, typename OutT>
+__global__ __launch_bounds__(kThreads) void ramp16_segments_kernel(
+    const ufomi_ramp_seg *__restrict__ segs, int nseg, const float *__restrict__ coef, int pair, float scale,
+    const float2 *__restrict__ tw)
+{
+    const long b = blockIdx.x;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].block0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const ufomi_ramp_seg sg = segs[lo];
+    ramp16_block<N, OutT>(sg.in, sg.W, sg.rows, coef, pair != 0, sg.W, (OutT *) sg.out, scale, tw, b - sg.block0);
 }
 
 template <int N>
@@ -534,3 +564,31 @@ extern "C" int ufomi_ramp_filter(ufomi_ctx *ctx, const float *d_in, int W, int P
     });
 }
 
+long ufomi_ramp_rows_per_block(int N, int pair)
+{
+    const int g = launch_by_n(N, [](auto n) { return fft16::Plan<decltype(n)::value>::G; });
+    return g < 0 ? g : (long) g * (pair ? 2 : 1);
+}
+
+int ufomi_ramp_segments_launch(hipStream_t stream, const float2 *tw, int N, int out_dtype,
+                               const ufomi_ramp_seg *d_segs, int nseg, long nblocks, const float *d_coef, int pair)
+{
+    if (nblocks <= 0 || nblocks > 0x7fffffffL || nseg <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid segmented ramp launch (%ld blocks, %d segments)", nblocks, nseg);
+    const float scale = 1.0f / (float) N;
+    return launch_by_n(N, [&](auto n) {
+        constexpr int NN = decltype(n)::value;
+        auto go = [&](auto *tag) {
+            using OutT = std::remove_pointer_t<decltype(tag)>;
+            hipLaunchKernelGGL((ramp16_segments_kernel<NN, OutT>), dim3((unsigned) nblocks), dim3(kThreads),
+                               lds16_bytes<NN>(), stream, d_segs, nseg, d_coef, pair, scale, tw);
+            UFOMI_CHECK_LAUNCH();
+            return (int) UFOMI_OK;
+        };
+        switch (out_dtype) {
+        case UFOMI_F16: return go((__half *) nullptr);
+        case UFOMI_BF16: return go((__hip_bfloat16 *) nullptr);
+        default: return go((float *) nullptr);
+        }
+    });
+}

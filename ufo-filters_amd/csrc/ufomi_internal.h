@@ -64,6 +64,40 @@ static inline int ufomi_log2(long x)
     return l;
 }
 
+// ---- batched FBP (batch.hip) over the kernels of fft_filter.hip and bp_parallel.hip
+
+// One segment of a segmented ramp-filter launch: `rows` rows of width W at `in`
+// (float32), written cropped to W at `out` (the launch's storage type); the
+// segment owns blocks [block0, next segment's block0).
+struct ufomi_ramp_seg {
+    const float *in;
+    void *out;
+    int W;
+    int rows;
+    long block0;
+};
+
+// Rows covered by one block of the radix-16 ramp kernel for transform size N
+// (2 G when `pair`); < 0 (with the error set) when N has no such kernel.
+long ufomi_ramp_rows_per_block(int N, int pair);
+int ufomi_ramp_segments_launch(hipStream_t stream, const float2 *tw, int N, int out_dtype,
+                               const ufomi_ramp_seg *d_segs, int nseg, long nblocks, const float *d_coef, int pair);
+
+// One single-slice request of a persistent backprojection launch.
+struct ufomi_bp_batch_req {
+    const void *sino;   // filtered rows [P][W] in the batch's storage type
+    float *out;         // [roi_h][roi_w]
+    int W, P, roi_w, roi_h, tiles_x, trig0;
+    float axis, roi_x, roi_y, pad;
+};
+
+// Tile / angle-chunk configurations of the persistent kernel.
+bool ufomi_bp_batch_config_ok(int tile, int achunk);
+int ufomi_bp_batch_blocks_per_cu(int tile, int achunk, int dtype, int *blocks);
+int ufomi_bp_batch_launch(hipStream_t stream, int tile, int achunk, int dtype, int grid,
+                          const ufomi_bp_batch_req *d_reqs, const int2 *d_items, int n_items, int *d_counter,
+                          const float2 *d_trig);
+
 // Storage conversions used by the producing / consuming kernels.
 __device__ __forceinline__ float ufomi_load(const float *p, long i) { return p[i]; }
 __device__ __forceinline__ float ufomi_load(const __half *p, long i) { return __half2float(p[i]); }

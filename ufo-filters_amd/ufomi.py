@@ -36,7 +36,9 @@ EXPORTS = [
     "ufomi_gbp", "ufomi_phantom_ellipses", "ufomi_phantom_ellipsoids", "ufomi_slice_ellipsoids",
     "ufomi_sinogram_ellipses", "ufomi_add_noise", "ufomi_bench_lds_read", "ufomi_bench_copy",
     "ufomi_gbp_projection_rows", "ufomi_project_ellipsoids", "ufomi_device_alloc", "ufomi_device_free",
-    "ufomi_memcpy", "ufomi_device_memory",
+    "ufomi_memcpy", "ufomi_device_memory", "ufomi_fbp_batch_create", "ufomi_fbp_batch_configure",
+    "ufomi_fbp_batch_autotune", "ufomi_fbp_batch_run", "ufomi_fbp_batch_launch", "ufomi_fbp_batch_get_info",
+    "ufomi_fbp_batch_destroy",
 ]
 
 
@@ -51,6 +53,17 @@ class BpParams(ctypes.Structure):
                 ("angle_step", ctypes.c_double), ("angle_offset", ctypes.c_double), ("mode", ctypes.c_int),
                 ("roi_x", ctypes.c_uint), ("roi_y", ctypes.c_uint), ("roi_width", ctypes.c_uint),
                 ("roi_height", ctypes.c_uint)]
+
+
+class FbpRequest(ctypes.Structure):
+    _fields_ = [("sinogram", ctypes.c_void_p), ("slice", ctypes.c_void_p), ("width", ctypes.c_int),
+                ("num_angles", ctypes.c_int), ("fft_size", ctypes.c_int), ("params", BpParams)]
+
+
+class FbpBatchInfo(ctypes.Structure):
+    _fields_ = [("voxel_updates", ctypes.c_double), ("tile", ctypes.c_int), ("angle_chunk", ctypes.c_int),
+                ("workgroups", ctypes.c_int), ("items", ctypes.c_int), ("filter_launches", ctypes.c_int),
+                ("filtered_bytes", ctypes.c_size_t)]
 
 
 _GEOM_NAMES = ["axis_angle_x", "axis_angle_y", "axis_angle_z", "volume_angle_x", "volume_angle_y",
@@ -107,6 +120,11 @@ def load():
         "ufomi_project_ellipsoids": [vp, dp, ci, ci, ci, ci, dp, vp],
         "ufomi_bench_lds_read": [vp, ci, ci, vp, ctypes.POINTER(cd)],
         "ufomi_bench_copy": [vp, vp, vp, csz],
+        "ufomi_fbp_batch_create": [vp, ctypes.POINTER(FbpRequest), ci, ci, cf, ci, ctypes.POINTER(vp)],
+        "ufomi_fbp_batch_configure": [vp, vp, ci, ci],
+        "ufomi_fbp_batch_autotune": [vp, vp, ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(cf)],
+        "ufomi_fbp_batch_run": [vp, vp], "ufomi_fbp_batch_launch": [vp, vp, ci],
+        "ufomi_fbp_batch_get_info": [vp, ctypes.POINTER(FbpBatchInfo)], "ufomi_fbp_batch_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -307,6 +325,66 @@ class Context:
 
     def bench_copy(self, d_in, d_out):
         check(self.L.ufomi_bench_copy(self.h, _ptr(d_in), _ptr(d_out), d_in.numel()))
+
+
+class FbpBatch:
+    """Uneven single-slice FBP requests as one HIP graph (ufomi_fbp_batch_*).
+
+    requests: dicts with torch device tensors `sinogram` ([P][W] float32) and
+    `slice` ([roi_height][roi_width] float32), optional `fft_size` and any
+    `backproject` property of bp_params().  The tensors must outlive the batch.
+    """
+
+    def __init__(self, ctx, requests, filter="ramp-fromreal", scale=1.0, storage=F32):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.h = ctypes.c_void_p()
+        arr = (FbpRequest * len(requests))()
+        self._keep = []
+        for i, r in enumerate(requests):
+            sino, out = r["sinogram"], r["slice"]
+            p, w = sino.shape
+            props = {k: v for k, v in r.items() if k not in ("sinogram", "slice", "fft_size")}
+            arr[i].sinogram = sino.data_ptr()
+            arr[i].slice = out.data_ptr()
+            arr[i].width = w
+            arr[i].num_angles = p
+            arr[i].fft_size = r.get("fft_size", 0)
+            arr[i].params = bp_params(**props)
+            self._keep += [sino, out]
+        check(self.L.ufomi_fbp_batch_create(ctx.h, arr, len(requests), FILTERS[filter], scale, storage,
+                                            ctypes.byref(self.h)))
+
+    def configure(self, tile, angle_chunk):
+        check(self.L.ufomi_fbp_batch_configure(self.ctx.h, self.h, tile, angle_chunk))
+
+    def autotune(self, reps=2, force=False):
+        t, a, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        check(self.L.ufomi_fbp_batch_autotune(self.ctx.h, self.h, reps, int(force), ctypes.byref(t),
+                                              ctypes.byref(a), ctypes.byref(ms)))
+        return t.value, a.value, ms.value
+
+    def run(self):
+        check(self.L.ufomi_fbp_batch_run(self.ctx.h, self.h))
+
+    def launch(self, stages=3):
+        check(self.L.ufomi_fbp_batch_launch(self.ctx.h, self.h, stages))
+
+    def info(self):
+        i = FbpBatchInfo()
+        check(self.L.ufomi_fbp_batch_get_info(self.h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in FbpBatchInfo._fields_}
+
+    def close(self):
+        if self.h:
+            self.L.ufomi_fbp_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bp_params(num_projections=0, offset=0, axis_pos=-1.0, angle_step=-1.0, angle_offset=0.0, mode="texture",
