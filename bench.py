@@ -16,6 +16,9 @@ Other workloads (--workload):
           projections of 1024^2 into 1024^3 (SOD 2500 px, SDD 5000 px)
   lamino  ramp (N 2048) + laminographic backprojection, tilt 30 deg, 1500
           projections of 1024^2 into 1024^2 x 256
+  learned 512^2 textured phantoms: train the residual U-Net on (120-view FBP,
+          1800-view FBP) pairs, then slices/s and RMSE to the phantom of 120-view
+          FBP + U-Net against plain 1800-view FBP (--views 60 for 60 views)
   batch   256 single-slice requests of uneven ROI (log-uniform in [256,2048]^2)
           and angle count ({360,720,1200,1800,3600}) as one HIP graph: one
           segmented ramp launch per FFT size + one persistent backprojection
@@ -429,17 +432,162 @@ def batch(args, ctx, torch, np, ufomi):
                 metric="GUPS (voxel*proj/s) for uneven batched FBP")
 
 
+def unet_flops(size):
+    """Multiply-add flops (x2) of one U-Net forward pass on a size^2 image."""
+    f = 2 * size * size * 9 * 1 * 32  # first conv
+    for lev in range(4):
+        hw = (size >> lev) ** 2
+        C = 32 << lev
+        if lev > 0:
+            f += 2 * hw * 9 * (C // 2) * C
+        f += 2 * hw * 9 * C * C
+    for lev in range(3):
+        hw = (size >> lev) ** 2
+        C = 32 << lev
+        f += 2 * (hw // 4) * (2 * C) * 4 * C  # transposed conv
+        f += 2 * hw * 9 * (2 * C) * C + 2 * hw * 9 * C * C
+    return f + 2 * size * size * 32
+
+
+def learned_fbp(args, ctx, torch, np, ufomi):
+    """Learned sparse-view FBP of 512^2 textured phantoms: train the U-Net on pairs
+    (sparse-view FBP, 1800-view FBP), then compare slices/s and RMSE to the phantom of
+    (sparse FBP + U-Net) against plain 1800-view FBP."""
+    import learned
+    dev = torch.device("cuda:0")
+    size, full, sparse = 512, 1800, args.views
+    n_train, n_test, batch = args.train_pairs, 256, 64
+    steps = args.train_steps if args.train_steps > 0 else max(1, n_train // batch)
+    pm = learned.PairMaker(ctx, size, sparse, full, chunk=256)
+    t0 = time.perf_counter()
+    X = torch.empty((n_train, size, size), device=dev)
+    Y = torch.empty_like(X)
+    pm.make(np.arange(n_train) + 100000, X, Y)
+    Xt = torch.empty((n_test, size, size), device=dev)
+    Yt, Pt = torch.empty_like(Xt), torch.empty_like(Xt)
+    pm.make(np.arange(n_test) + 900000, Xt, Yt, Pt)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    log("generated %d training pairs + %d test phantoms (%d / %d views) in %.1f s"
+        % (n_train, n_test, sparse, full, gen_s))
+
+    net = ufomi.Unet(ctx, batch, size, size, training=True, seed=2024)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = learned.train(net, X, Y, steps=steps, batch=batch, lr=1e-3, log=log, log_every=max(1, steps // 10))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log("trained %d steps (batch %d) in %.1f s" % (steps, batch, train_s))
+    params = net.get_params()
+    net.close()
+    del X, Y
+    torch.cuda.empty_cache()
+
+    inf = ufomi.Unet(ctx, n_test, size, size)
+    inf.set_params(params)
+    out = torch.empty_like(Xt)
+    learned.infer(inf, Xt, out, n_test)
+    torch.cuda.synchronize()
+    mask = learned.disk_mask(size)
+    acc = {"rmse_to_phantom": {"plain_%d_view_fbp" % full: learned.rmse(Yt, Pt, mask),
+                               "plain_%d_view_fbp" % sparse: learned.rmse(Xt, Pt, mask),
+                               "learned_%d_view" % sparse: learned.rmse(out, Pt, mask)},
+           "rmse_to_%d_view_fbp" % full: {"plain_%d_view_fbp" % sparse: learned.rmse(Xt, Yt, mask),
+                                          "learned_%d_view" % sparse: learned.rmse(out, Yt, mask)},
+           "test_set": "%d unseen phantoms, RMSE inside the reconstruction disk" % n_test}
+    log("accuracy %s" % acc)
+
+    # throughput: 256 slices end to end, sinograms resident in HBM
+    B = n_test
+    ell_w = [learned.phantom_params(900000 + i, size) for i in range(B)]
+    sino_full = torch.empty((B, full, size), device=dev)
+    ctx.sinogram_phantom(np.stack([e for e, _, _ in ell_w]), np.stack([w for _, w, _ in ell_w]), size / 2.0, size,
+                         full, size / 2.0, math.pi / full, sino_full)
+    sino_sparse = sino_full[:, ::full // sparse].contiguous()
+    filt_full = torch.empty_like(sino_full)
+    filt_sparse = torch.empty_like(sino_sparse)
+    rec = torch.empty((B, size, size), device=dev)
+    rec2 = torch.empty_like(rec)
+    N = learned.fft_size(size)
+    coef = torch.from_numpy(ufomi.filter_coefficients(2 * N, "ramp-fromreal")).to(dev)
+
+    def plain(e):
+        if e: e[0].record()
+        ctx.ramp_filter(sino_full, size, B * full, N, coef, size, filt_full)
+        ctx.bp_parallel(filt_full, size, full, rec, n_slices=B, axis_pos=size / 2.0)
+        if e: e[1].record()
+
+    def learned_step(e):
+        if e: e[0].record()
+        ctx.ramp_filter(sino_sparse, size, B * sparse, N, coef, size, filt_sparse)
+        ctx.bp_parallel(filt_sparse, size, sparse, rec, n_slices=B, axis_pos=size / 2.0)
+        if e: e[1].record()
+        inf.forward(rec, rec2, batch=B)
+        if e: e[2].record()
+
+    wall_p, ev_p = run_timed(torch, args.steps, args.warmup, plain, 2)
+    wall_l, ev_l = run_timed(torch, args.steps, args.warmup, learned_step, 3)
+    fbp_sparse_ms, unet_ms = mean_ms(ev_l, 0, 1), mean_ms(ev_l, 1, 2)
+    plain_sps = B / (wall_p / args.steps)
+    learned_sps = B / (wall_l / args.steps)
+    log("plain %d-view FBP %.0f slices/s; learned %d-view %.0f slices/s (FBP %.2f ms + U-Net %.2f ms per %d)"
+        % (full, plain_sps, sparse, learned_sps, fbp_sparse_ms, unet_ms, B))
+    sink = torch.empty(4096 * 256, device=dev)
+    ctx.bench_mfma(1024, 64, sink)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    mf = ctx.bench_mfma(4096, 2048, sink)
+    e1.record()
+    torch.cuda.synchronize()
+    mfma_peak = mf / (e0.elapsed_time(e1) * 1e-3)
+    flops = unet_flops(size) * B
+    achieved = flops / (unet_ms * 1e-3)
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        oracle = oracle_module(threads)
+        import unet_oracle
+        named = inf.unpack(params)
+        cnet = unet_oracle.UNet({k: v for k, v in named.items()}, bf16_storage=False)
+        n_cpu = 2
+        s_np = sino_sparse[:n_cpu].cpu().numpy()
+        coef_o = oracle.filter_coefficients(2 * N, "ramp-fromreal")
+        t0 = time.perf_counter()
+        for i in range(n_cpu):
+            f32 = oracle.ramp_chain(s_np[i], N, coef_o, size).astype(np.float32)
+            img = oracle.bp_texture_ref32(f32, axis_pos=size / 2.0)
+            cnet.forward(img[None].astype(np.float64))
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(n_cpu / dt, 4), "unit": "slices/s", "cores": threads, "kind": "port",
+               "sample": "%d slices: C-oracle %d-view FBP + numpy float64 U-Net forward (same weights), %.1f s"
+                         % (n_cpu, sparse, dt)}
+    inf.close()
+    extra = {"kernel_ms": round(unet_ms, 3), "flops_per_forward": flops, "fbp_sparse_ms": round(fbp_sparse_ms, 3),
+             "mfma_peak_measured_TFLOPs": round(mfma_peak / 1e12, 1)}
+    return dict(learned_sps=learned_sps, plain_sps=plain_sps, wall=wall_l, kernel_ms=unet_ms, achieved=achieved,
+                extra=extra, cpu=cpu, accuracy=acc,
+                training={"pairs": n_train, "steps": steps, "batch": batch, "lr": 1e-3, "optimizer": "Adam",
+                          "seconds": round(train_s, 1), "data_seconds": round(gen_s, 1),
+                          "loss_first_last": [losses[0][1], losses[-1][1]]},
+                workload="learned sparse-view FBP, 512^2 textured ellipse phantoms: %d-view Ram-Lak FBP + residual "
+                         "U-Net (4 levels, 32 base channels, 1.93M params, BF16 MFMA, FP32 accumulation), batch %d, "
+                         "against plain %d-view FBP" % (sparse, B, full))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["fbp2048", "fbp512", "cone", "lamino", "batch"], default="fbp2048")
+    ap.add_argument("--workload", choices=["fbp2048", "fbp512", "cone", "lamino", "batch", "learned"], default="fbp2048")
     ap.add_argument("--slices", type=int, default=2048)
     ap.add_argument("--storage", choices=["f32", "f16", "bf16", "all"], default=None,
                     help="filtered-sinogram storage; 'all' (the fbp2048 default) reports f32 as the value "
                          "and f16 / bf16 beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--views", type=int, default=120, help="learned: sparse view count (divides 1800)")
+    ap.add_argument("--train-pairs", type=int, default=20000, help="learned: training pairs")
+    ap.add_argument("--train-steps", type=int, default=0, help="learned: Adam steps (0: one epoch)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -464,6 +612,28 @@ def main():
         r = fbp_parallel(args, ctx, torch, np, ufomi, 1, 512, 512, "ramp-fromreal", 0.01, storages)
     elif args.workload == "batch":
         r = batch(args, ctx, torch, np, ufomi)
+    elif args.workload == "learned":
+        r = learned_fbp(args, ctx, torch, np, ufomi)
+        result = {
+            "metric": "slices/s learned %d-view vs plain 1800-view FBP" % args.views,
+            "value": round(r["learned_sps"], 1), "unit": "slices/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 (U-Net, fp32 accumulation); f32 FBP",
+            "data": "synthetic: seeded ellipse phantoms with band-limited texture, analytic sinograms",
+            "config": {"workload": r["workload"]},
+            "plain_1800_view_slices_per_s": round(r["plain_sps"], 1),
+            "learned_vs_plain": round(r["learned_sps"] / r["plain_sps"], 3),
+            "roofline": dict({"kernel": "U-Net forward (gemm_kernel<...,Conv3|UpFwd> + VALU layers)", "bound": "mfma",
+                              "achieved": round(r["achieved"] / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                              "frac": round(r["achieved"] / 2.5e15, 4), "traffic": None}, **r["extra"]),
+            "accuracy": r["accuracy"], "training": r["training"],
+        }
+        if r["cpu"] is not None:
+            result["cpu_baseline"] = r["cpu"]
+        print(json.dumps(result), flush=True)
+        ctx.close()
+        return
     else:
         r = general(args, ctx, torch, np, ufomi, args.workload)
 

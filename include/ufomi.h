@@ -218,6 +218,52 @@ int ufomi_fbp_batch_launch(ufomi_ctx *ctx, ufomi_fbp_batch *batch, int stages);
 int ufomi_fbp_batch_get_info(const ufomi_fbp_batch *batch, ufomi_fbp_batch_info *info);
 int ufomi_fbp_batch_destroy(ufomi_fbp_batch *batch);
 
+/* ------------------------------------- learned sparse-view stage (U-Net) */
+
+/*
+ * A residual U-Net that maps a sparse-view FBP slice to an estimate of the
+ * full-view FBP slice: 4 levels of 32, 64, 128, 256 channels, two 3x3 convs +
+ * ReLU per level, 2x2 max pooling, 2x2 stride-2 transposed convs, skip
+ * concatenation, 1x1 output conv added to the input; about 1.93M parameters.
+ * BF16 weights and activations, FP32 accumulation (MFMA implicit GEMM), FP32
+ * master weights and Adam.  No counterpart in the reference: it follows
+ * `backproject` for callers that reconstruct from few views.
+ *
+ * Images are float32 [batch][height][width] device arrays, height and width
+ * powers of two (16..8192; width >= 64 with `training`, which allocates
+ * gradient and optimiser state).  Parameters are a flat float32 array in the order of
+ * ufomi_unet_tensor (index 0 .. ufomi_unet_num_tensors - 1), each tensor at
+ * its offset (16-byte aligned gaps are zero).  Weights: 3x3 conv
+ * [cout][3][3][cin], transposed conv [2x2][cout][1][cin], final [1][1][1][32].
+ */
+typedef struct ufomi_unet ufomi_unet;
+
+int ufomi_unet_create(ufomi_ctx *ctx, int max_batch, int height, int width, int training, uint64_t seed,
+                      ufomi_unet **out);
+int ufomi_unet_destroy(ufomi_unet *net);
+/* count: parameters; padded: length of the flat parameter array. */
+int ufomi_unet_num_params(const ufomi_unet *net, size_t *count, size_t *padded);
+int ufomi_unet_num_tensors(const ufomi_unet *net);
+int ufomi_unet_tensor(const ufomi_unet *net, int index, const char **name, size_t *offset, int *dims4);
+int ufomi_unet_get_params(ufomi_ctx *ctx, ufomi_unet *net, float *host_params);
+/* Also resets the optimiser state. */
+int ufomi_unet_set_params(ufomi_ctx *ctx, ufomi_unet *net, const float *host_params);
+/* d_out = net(d_in), batch <= max_batch (asynchronous). */
+int ufomi_unet_forward(ufomi_ctx *ctx, ufomi_unet *net, const float *d_in, int batch, float *d_out);
+/* Mean squared error against d_target and its gradient for every parameter
+ * (host_grads, the flat layout; may be NULL).  loss may be NULL (then no
+ * synchronisation). */
+int ufomi_unet_gradients(ufomi_ctx *ctx, ufomi_unet *net, const float *d_in, const float *d_target, int batch,
+                         float *loss, float *host_grads);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) on the MSE loss. */
+int ufomi_unet_train_step(ufomi_ctx *ctx, ufomi_unet *net, const float *d_in, const float *d_target, int batch,
+                          float lr, float *loss);
+/* One 3x3 convolution of the network's GEMM core (test and benchmark hook):
+ * NHWC bf16 [batch][height][width][cin] -> [..][cout], weights [cout][3][3][cin]
+ * bf16, optional float bias, optional ReLU; cin a power of two >= 32. */
+int ufomi_nn_conv3x3(ufomi_ctx *ctx, const void *d_in, int batch, int height, int width, int cin, const void *d_w,
+                     const float *d_bias, int relu, int cout, void *d_out);
+
 /* ---------------------------------------------------- cone-beam weighting */
 
 /*
@@ -315,6 +361,24 @@ int ufomi_sinogram_ellipses(ufomi_ctx *ctx, const double *ellipses, int n_ellips
  * v = j + 0.5): density times chord length in voxel units, FP64. */
 int ufomi_project_ellipsoids(ufomi_ctx *ctx, const double *ellipsoids, int n_ellipsoids, int W, int H, int P,
                              const double *rows, float *d_projections);
+/*
+ * Band-limited texture: n (<= 64) plane waves (kx, ky, phase, amplitude) with
+ * uniform directions, periods log-uniform in [period_min_px, period_max_px],
+ * uniform phases and amplitude rms * sqrt(2 / n) (RMS `rms`), restricted to
+ * the disk of radius radius_px around the axis (a random-phase spectral
+ * synthesis of the band, the Fourier counterpart of Perlin noise).
+ */
+int ufomi_phantom_texture(uint64_t seed, int n, double period_min_px, double period_max_px, double rms,
+                          double *waves);
+/* Sinograms of ellipses plus texture waves ([n_slices][n_waves][4], may be
+ * NULL with n_waves 0): exact FP64 line integrals of both. */
+int ufomi_sinogram_phantom(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, const double *waves,
+                           int n_waves, double radius_px, int n_slices, int W, int P, double axis_pos,
+                           double angle_step, float *d_sino);
+/* The same phantoms sampled at the backprojector's pixel centres
+ * (i + 0.5 - axis_pos, j + 0.5 - axis_pos): n_slices images [N][N]. */
+int ufomi_phantom_image(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, const double *waves, int n_waves,
+                        double radius_px, int n_slices, int N, double axis_pos, float *d_images);
 /* Adds N(0, (rel_sigma * max|d|)^2) noise from a counter-based generator;
  * returns max|d| before the noise in *max_out (may be NULL). */
 int ufomi_add_noise(ufomi_ctx *ctx, float *d, size_t n, double rel_sigma, uint64_t seed, float *max_out);
@@ -328,6 +392,9 @@ int ufomi_add_noise(ufomi_ctx *ctx, float *d, size_t n, double rel_sigma, uint64
  * LDS and HBM bandwidth. */
 int ufomi_bench_lds_read(ufomi_ctx *ctx, int blocks, int iters, float *d_sink, double *bytes);
 int ufomi_bench_copy(ufomi_ctx *ctx, const float *d_in, float *d_out, size_t n);
+/* Dense BF16 MFMA: blocks x 256 threads, each wave `iters` x 8 independent
+ * v_mfma_f32_16x16x32_bf16; *flops = the work issued (2 flops per MAC). */
+int ufomi_bench_mfma(ufomi_ctx *ctx, int blocks, int iters, float *d_sink, double *flops);
 
 #ifdef __cplusplus
 }

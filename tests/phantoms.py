@@ -41,3 +41,29 @@ def random_ellipses(rng, n, radius):
     return np.stack([r * np.cos(psi) * radius, r * np.sin(psi) * radius,
                      rng.uniform(0.05, 0.4, n) * radius, rng.uniform(0.05, 0.4, n) * radius,
                      rng.uniform(0, np.pi, n), rng.uniform(-0.5, 1.0, n)], axis=1)
+
+
+def texture_sinogram(waves, radius, W, P, axis, step):
+    """Line integrals of plane waves (kx, ky, phase, amp) restricted to the disk |x| < radius."""
+    th = np.arange(P)[:, None] * step
+    c, s = np.cos(th), np.sin(th)
+    t = np.arange(W)[None, :] + 0.5 - axis
+    L2 = radius ** 2 - t ** 2
+    L = np.sqrt(np.maximum(L2, 0.0))
+    out = np.zeros((P, W))
+    for kx, ky, ph, amp in np.atleast_2d(waves):
+        alpha = (kx * c + ky * s) * t + ph
+        beta = -kx * s + ky * c
+        with np.errstate(divide="ignore", invalid="ignore"):
+            f = np.where(np.abs(beta * L) < 1e-8, 2 * L, 2 * np.sin(beta * L) / beta)
+        out += np.where(L2 > 0, amp * np.cos(alpha) * f, 0.0)
+    return out
+
+
+def texture_image(waves, radius, n, axis):
+    x = np.arange(n)[None, :] + 0.5 - axis
+    y = np.arange(n)[:, None] + 0.5 - axis
+    out = np.zeros((n, n))
+    for kx, ky, ph, amp in np.atleast_2d(waves):
+        out += amp * np.cos(kx * x + ky * y + ph)
+    return np.where(x ** 2 + y ** 2 < radius ** 2, out, 0.0)

@@ -1,6 +1,7 @@
 // Roofline denominators measured on the running GPU: LDS read bandwidth
-// (ds_read_b128, conflict-free, every CU busy) and HBM copy bandwidth
-// (streaming float4 copy).  Used by bench.py to express kernel throughput as a
+// (ds_read_b128, conflict-free, every CU busy), HBM copy bandwidth
+// (streaming float4 copy) and dense BF16 MFMA throughput
+// (v_mfma_f32_16x16x32_bf16 chains held in registers).  Used by bench.py to express kernel throughput as a
 // fraction of what this device sustains.
 #include "ufomi_internal.h"
 
@@ -40,7 +41,46 @@ __global__ __launch_bounds__(256) void copy_kernel(const float4 *__restrict__ in
         out[i] = in[i];
 }
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// 8 independent accumulator chains per wave, operands in registers.
+__global__ __launch_bounds__(256) void mfma_kernel(int iters, float *__restrict__ sink)
+{
+    bf16x8_t a, b;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        a[i] = (__bf16) (1.0f + 0.001f * (float) (threadIdx.x + i));
+        b[i] = (__bf16) (0.5f - 0.001f * (float) i);
+    }
+    f32x4_t acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) acc[c] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[c], 0, 0, 0);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+    if (s == -1.0f)  // never true; keeps the chains alive
+        sink[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 }  // namespace
+
+extern "C" int ufomi_bench_mfma(ufomi_ctx *ctx, int blocks, int iters, float *d_sink, double *flops)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (blocks <= 0 || iters <= 0 || !d_sink)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid microbenchmark arguments");
+    hipLaunchKernelGGL(mfma_kernel, dim3(blocks), dim3(256), 0, ctx->stream, iters, d_sink);
+    UFOMI_CHECK_LAUNCH();
+    if (flops)  // 4 waves x 8 chains x 16 x 16 x 32 x 2 per iteration
+        *flops = (double) blocks * 4.0 * iters * 8.0 * 16384.0;
+    return UFOMI_OK;
+}
 
 extern "C" int ufomi_bench_lds_read(ufomi_ctx *ctx, int blocks, int iters, float *d_sink, double *bytes)
 {

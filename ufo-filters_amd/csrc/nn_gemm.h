@@ -1,0 +1,418 @@
+// BF16 MFMA implicit-GEMM core of the learned sparse-view stage (unet.hip).
+//
+// One kernel template computes C[M][N] = sum_k A[m][k] B[k][n] with
+// v_mfma_f32_16x16x32_bf16 (fp32 accumulation).  The operands are never
+// materialised: an operation type `Op` supplies
+//   Op::RowA row_a(m)            per-row context (pixel coordinates, ...)
+//   Op::ColB col_b(n)            per-column context
+//   bf16x8 load_a(RowA, k)       A[m][k .. k+7]
+//   bf16x8 load_b(ColB, k)       B[k .. k+7][n]
+//   void k_range(z, &k0, &k1)    the K slice of split index z (grid z)
+//   void store(m, n, float, z)   epilogue for C[m][n]
+// Convolutions (NHWC activations), their data gradients, the 2x2 transposed
+// convolution and the weight gradients are all such operations.
+//
+// Lane layout of v_mfma_f32_16x16x32_bf16 (wave of 64, r = lane % 16,
+// q = lane / 16): lane supplies A[r][8q .. 8q+7] and B[8q .. 8q+7][r] and
+// receives D[4q + e][r], e < 4.  The order of the 8 k values inside a lane
+// only has to agree between A and B.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nn {
+
+using bf16 = __bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 zero8()
+{
+    bf16x8 z;
+#pragma unroll
+    for (int i = 0; i < 8; i++) z[i] = (bf16) 0.0f;
+    return z;
+}
+
+__device__ __forceinline__ bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+// Workgroup of 256 threads = WM x WN waves; each wave owns MT x NT tiles of
+// 16 x 16.  Fragments of step k+32 are loaded before the MFMAs of step k.
+template <int MT, int NT, int WM, int WN, typename Op>
+__global__ __launch_bounds__(256) void gemm_kernel(Op op)
+{
+    static_assert(WM * WN == 4, "four waves per workgroup");
+    constexpr int BM = WM * MT * 16, BN = WN * NT * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const long m0 = (long) blockIdx.x * BM + (wave % WM) * MT * 16;
+    const long n0 = (long) blockIdx.y * BN + (wave / WM) * NT * 16;
+    long kb, ke;
+    op.k_range(blockIdx.z, kb, ke);
+
+    typename Op::RowA ra[MT];
+    typename Op::ColB cb[NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++) ra[i] = op.row_a(m0 + i * 16 + r);
+#pragma unroll
+    for (int j = 0; j < NT; j++) cb[j] = op.col_b(n0 + j * 16 + r);
+
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    if (kb < ke) {
+        bf16x8 a[MT], b[NT];
+#pragma unroll
+        for (int i = 0; i < MT; i++) a[i] = op.load_a(ra[i], kb + 8 * q);
+#pragma unroll
+        for (int j = 0; j < NT; j++) b[j] = op.load_b(cb[j], kb + 8 * q);
+        for (long k = kb; k < ke; k += 32) {
+            const bool more = k + 32 < ke;
+            bf16x8 an[MT], bn[NT];
+            if (more) {
+#pragma unroll
+                for (int i = 0; i < MT; i++) an[i] = op.load_a(ra[i], k + 32 + 8 * q);
+#pragma unroll
+                for (int j = 0; j < NT; j++) bn[j] = op.load_b(cb[j], k + 32 + 8 * q);
+            }
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            if (more) {
+#pragma unroll
+                for (int i = 0; i < MT; i++) a[i] = an[i];
+#pragma unroll
+                for (int j = 0; j < NT; j++) b[j] = bn[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                op.store(m0 + i * 16 + 4 * q + e, n0 + j * 16 + r, acc[i][j][e], blockIdx.z);
+}
+
+// A tensor of pixels in NHWC order: element (pixel p, channel c) at
+// ptr[p * cs + c0 + c].
+template <typename T>
+struct View {
+    T *ptr;
+    int cs, c0;
+    __device__ __forceinline__ T *at(long p, int c) const { return ptr + p * cs + c0 + c; }
+};
+
+// Pixel p of a [n][H][W] grid with power-of-two H and W.
+struct Pix {
+    int y, x;
+};
+__device__ __forceinline__ Pix pix_of(long p, int lh, int lw)
+{
+    Pix r;
+    r.x = (int) (p & ((1 << lw) - 1));
+    r.y = (int) ((p >> lw) & ((1 << lh) - 1));
+    return r;
+}
+
+// ---------------------------------------------------------------- 3x3 conv
+// out[p][n] = act(sum_{tap, c} in[p + d(tap)][c] w[n][tap][c] + bias[n]),
+// zero padding, K ordered (tap, c); cin a multiple of 32.  With mask, channels
+// n >= mask_from are zeroed where mask[p][n] <= 0 (a ReLU derivative).
+struct Conv3 {
+    View<const bf16> in;
+    int lh, lw, lcin, cout;
+    long npix;
+    const bf16 *w;        // [cout][9][cin]
+    const float *bias;    // nullptr: none
+    int relu;
+    View<const bf16> mask;  // mask.ptr nullptr: none
+    int mask_from;
+    View<bf16> out;
+
+    struct RowA {
+        long p;
+        int y, x;
+        bool ok;
+    };
+    struct ColB {
+        const bf16 *w;
+        bool ok;
+    };
+    __device__ __forceinline__ RowA row_a(long m) const
+    {
+        RowA r;
+        r.p = m;
+        r.ok = m < npix;
+        const Pix px = pix_of(m, lh, lw);
+        r.y = px.y;
+        r.x = px.x;
+        return r;
+    }
+    __device__ __forceinline__ ColB col_b(long n) const
+    {
+        ColB c;
+        c.ok = n < cout;
+        c.w = w + (c.ok ? n : 0) * (9L << lcin);
+        return c;
+    }
+    __device__ __forceinline__ void k_range(int, long &k0, long &k1) const
+    {
+        k0 = 0;
+        k1 = 9L << lcin;
+    }
+    __device__ __forceinline__ bf16x8 load_a(const RowA &r, long k) const
+    {
+        const int tap = (int) (k >> lcin), c = (int) (k & ((1 << lcin) - 1));
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int yy = r.y + dy, xx = r.x + dx;
+        if (!r.ok || yy < 0 || yy >= (1 << lh) || xx < 0 || xx >= (1 << lw)) return zero8();
+        return ld8(in.at(r.p + (long) dy * (1L << lw) + dx, c));
+    }
+    __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const
+    {
+        return c.ok ? ld8(c.w + k) : zero8();
+    }
+    __device__ __forceinline__ void store(long m, long n, float v, int) const
+    {
+        if (m >= npix || n >= cout) return;
+        if (bias) v += bias[n];
+        if (relu) v = fmaxf(v, 0.0f);
+        if (mask.ptr && n >= mask_from && !((float) *mask.at(m, (int) n) > 0.0f)) v = 0.0f;
+        *out.at(m, (int) n) = (bf16) v;
+    }
+};
+
+// ------------------------------------------------ 2x2 stride-2 transposed conv
+// out[child(p, ab)][co] = sum_c in[p][c] w[ab][co][c] + bias[co]; M = input
+// pixels, N = 4 cout ((ab, co)), K = cin (multiple of 32).  Output grid is
+// (2h, 2w).
+struct UpFwd {
+    View<const bf16> in;
+    int lh, lw, lcin, lcout;
+    long npix;
+    const bf16 *w;      // [4][cout][cin]
+    const float *bias;  // [cout]
+    View<bf16> out;
+
+    struct RowA {
+        long p;
+        bool ok;
+    };
+    struct ColB {
+        const bf16 *w;
+        bool ok;
+    };
+    __device__ __forceinline__ RowA row_a(long m) const { return RowA{m, m < npix}; }
+    __device__ __forceinline__ ColB col_b(long n) const
+    {
+        const bool ok = n < (4L << lcout);
+        return ColB{w + ((ok ? n : 0) << lcin), ok};
+    }
+    __device__ __forceinline__ void k_range(int, long &k0, long &k1) const
+    {
+        k0 = 0;
+        k1 = 1L << lcin;
+    }
+    __device__ __forceinline__ bf16x8 load_a(const RowA &r, long k) const
+    {
+        return r.ok ? ld8(in.at(r.p, (int) k)) : zero8();
+    }
+    __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const { return c.ok ? ld8(c.w + k) : zero8(); }
+    __device__ __forceinline__ void store(long m, long n, float v, int) const
+    {
+        if (m >= npix || n >= (4L << lcout)) return;
+        const int ab = (int) (n >> lcout), co = (int) (n & ((1 << lcout) - 1));
+        const long img = m >> (lh + lw);
+        const Pix px = pix_of(m, lh, lw);
+        const long child = (((img << (lh + 1)) + 2 * px.y + (ab >> 1)) << (lw + 1)) + 2 * px.x + (ab & 1);
+        *out.at(child, co) = (bf16) (v + bias[co]);
+    }
+};
+
+// Data gradient of UpFwd: gin[p][c] = sum_{ab, co} g[child(p, ab)][co] wT[c][ab][co],
+// masked by the ReLU of the forward input.  M = input pixels, N = cin,
+// K = 4 cout ((ab, co), cout a multiple of 32).
+struct UpDgrad {
+    View<const bf16> g;
+    int lh, lw, lcout, cin;
+    long npix;
+    const bf16 *wt;  // [cin][4][cout]
+    View<const bf16> mask;
+    View<bf16> out;
+
+    struct RowA {
+        long child0;
+        bool ok;
+    };
+    struct ColB {
+        const bf16 *w;
+        bool ok;
+    };
+    __device__ __forceinline__ RowA row_a(long m) const
+    {
+        RowA r;
+        r.ok = m < npix;
+        const long img = m >> (lh + lw);
+        const Pix px = pix_of(m, lh, lw);
+        r.child0 = (((img << (lh + 1)) + 2 * px.y) << (lw + 1)) + 2 * px.x;
+        return r;
+    }
+    __device__ __forceinline__ ColB col_b(long n) const
+    {
+        ColB c;
+        c.ok = n < cin;
+        c.w = wt + (c.ok ? n : 0) * (4L << lcout);
+        return c;
+    }
+    __device__ __forceinline__ void k_range(int, long &k0, long &k1) const
+    {
+        k0 = 0;
+        k1 = 4L << lcout;
+    }
+    __device__ __forceinline__ bf16x8 load_a(const RowA &r, long k) const
+    {
+        if (!r.ok) return zero8();
+        const int ab = (int) (k >> lcout), co = (int) (k & ((1 << lcout) - 1));
+        return ld8(g.at(r.child0 + ((long) (ab >> 1) << (lw + 1)) + (ab & 1), co));
+    }
+    __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const
+    {
+        return c.ok ? ld8(c.w + k) : zero8();
+    }
+    __device__ __forceinline__ void store(long m, long n, float v, int) const
+    {
+        if (m >= npix || n >= cin) return;
+        if (!((float) *mask.at(m, (int) n) > 0.0f)) v = 0.0f;
+        *out.at(m, (int) n) = (bf16) v;
+    }
+};
+
+// Split-K partial sums of a weight gradient: ws[z][m][n] (fp32).
+struct Partial {
+    float *ws;
+    long M, N;
+    long kchunk, K;
+    __device__ __forceinline__ void range(int z, long &k0, long &k1) const
+    {
+        k0 = z * kchunk;
+        k1 = k0 + kchunk < K ? k0 + kchunk : K;
+    }
+    __device__ __forceinline__ void put(long m, long n, float v, int z) const
+    {
+        if (m < M && n < N) ws[(z * M + m) * N + n] = v;
+    }
+};
+
+// Weight gradient of Conv3: dW[co][tap][c] = sum_p g[p][co] in[p + d(tap)][c].
+// M = cout, N = 9 cin ((tap, c)), K = pixels in groups of 8 along a row
+// (W a multiple of 8).  Pixel-strided operands: 8 scalar loads per fragment.
+struct Conv3Wgrad {
+    View<const bf16> g;
+    View<const bf16> in;
+    int lh, lw, cin, cout;
+    long npix;
+    Partial part;
+
+    struct RowA {
+        int co;
+        bool ok;
+    };
+    struct ColB {
+        int c, dy, dx;
+        bool ok;
+    };
+    __device__ __forceinline__ RowA row_a(long m) const { return RowA{(int) m, m < cout}; }
+    __device__ __forceinline__ ColB col_b(long n) const
+    {
+        ColB c;
+        c.ok = n < 9L * cin;
+        const int tap = c.ok ? (int) (n / cin) : 0;
+        c.c = c.ok ? (int) (n % cin) : 0;
+        c.dy = tap / 3 - 1;
+        c.dx = tap % 3 - 1;
+        return c;
+    }
+    __device__ __forceinline__ void k_range(int z, long &k0, long &k1) const { part.range(z, k0, k1); }
+    __device__ __forceinline__ bf16x8 load_a(const RowA &r, long k) const
+    {
+        bf16x8 v = zero8();
+        if (!r.ok || k >= npix) return v;
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = *g.at(k + i, r.co);
+        return v;
+    }
+    __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const
+    {
+        bf16x8 v = zero8();
+        if (!c.ok || k >= npix) return v;
+        const Pix px = pix_of(k, lh, lw);
+        const int yy = px.y + c.dy;
+        if (yy < 0 || yy >= (1 << lh)) return v;
+        const long base = k + (long) c.dy * (1L << lw) + c.dx;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int xx = px.x + i + c.dx;
+            if (xx >= 0 && xx < (1 << lw)) v[i] = *in.at(base + i, c.c);
+        }
+        return v;
+    }
+    __device__ __forceinline__ void store(long m, long n, float v, int z) const { part.put(m, n, v, z); }
+};
+
+// Weight gradient of UpFwd: dW[ab][co][c] = sum_p g[child(p, ab)][co] in[p][c].
+// M = 4 cout ((ab, co)), N = cin, K = input pixels (groups of 8 along a row).
+struct UpWgrad {
+    View<const bf16> g;
+    View<const bf16> in;
+    int lh, lw, lcout, cin;
+    long npix;
+    Partial part;
+
+    struct RowA {
+        int ab, co;
+        bool ok;
+    };
+    struct ColB {
+        int c;
+        bool ok;
+    };
+    __device__ __forceinline__ RowA row_a(long m) const
+    {
+        RowA r;
+        r.ok = m < (4L << lcout);
+        r.ab = (int) (m >> lcout);
+        r.co = (int) (m & ((1 << lcout) - 1));
+        return r;
+    }
+    __device__ __forceinline__ ColB col_b(long n) const { return ColB{(int) n, n < cin}; }
+    __device__ __forceinline__ void k_range(int z, long &k0, long &k1) const { part.range(z, k0, k1); }
+    __device__ __forceinline__ bf16x8 load_a(const RowA &r, long k) const
+    {
+        bf16x8 v = zero8();
+        if (!r.ok || k >= npix) return v;
+        const long img = k >> (lh + lw);
+        const Pix px = pix_of(k, lh, lw);
+        const long child0 = (((img << (lh + 1)) + 2 * px.y + (r.ab >> 1)) << (lw + 1)) + 2 * px.x + (r.ab & 1);
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = *g.at(child0 + 2 * i, r.co);
+        return v;
+    }
+    __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const
+    {
+        bf16x8 v = zero8();
+        if (!c.ok || k >= npix) return v;
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = *in.at(k + i, c.c);
+        return v;
+    }
+    __device__ __forceinline__ void store(long m, long n, float v, int z) const { part.put(m, n, v, z); }
+};
+
+}  // namespace nn

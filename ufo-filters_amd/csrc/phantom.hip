@@ -60,10 +60,30 @@ __device__ double ellipse_line_integral(const double *e, double c, double s, dou
 
 constexpr int kMaxEll = 64;
 
-__global__ __launch_bounds__(256) void sinogram_kernel(const double *__restrict__ ell, int n_ell, int W, int P,
-                                                       double axis, double step, float *__restrict__ out)
+constexpr int kMaxWave = 64;
+
+// Line integral of the texture wave (kx, ky, phase, amplitude) restricted to
+// the disk |x| < R along x cos(theta) + y sin(theta) = t: the chord is
+// t n + s d, |s| <= L = sqrt(R^2 - t^2), and
+// int_{-L}^{L} a cos(k.n t + phi + (k.d) s) ds = 2 a cos(k.n t + phi) sin(b L) / b, b = k.d.
+__device__ double wave_line_integral(const double *w, double c, double s, double t, double R)
+{
+    const double L2 = R * R - t * t;
+    if (L2 <= 0.0) return 0.0;
+    const double L = sqrt(L2);
+    const double alpha = (w[0] * c + w[1] * s) * t + w[2];
+    const double beta = -w[0] * s + w[1] * c;
+    const double bl = beta * L;
+    const double f = fabs(bl) < 1e-8 ? 2.0 * L : 2.0 * sin(bl) / beta;
+    return w[3] * cos(alpha) * f;
+}
+
+__global__ __launch_bounds__(256) void sinogram_kernel(const double *__restrict__ ell, int n_ell,
+                                                       const double *__restrict__ wav, int n_wav, double R, int W,
+                                                       int P, double axis, double step, float *__restrict__ out)
 {
     __shared__ double e[kMaxEll * 7];
+    __shared__ double wv[kMaxWave * 4];
     const long sl = blockIdx.z;
     for (int k = threadIdx.x; k < n_ell; k += 256) {
         const double *src = ell + (sl * n_ell + k) * 6;
@@ -75,6 +95,7 @@ __global__ __launch_bounds__(256) void sinogram_kernel(const double *__restrict_
         e[7 * k + 5] = sin(src[4]);
         e[7 * k + 6] = src[5];
     }
+    for (int k = threadIdx.x; k < 4 * n_wav; k += 256) wv[k] = wav[sl * n_wav * 4 + k];
     __syncthreads();
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int p = blockIdx.y;
@@ -86,7 +107,33 @@ __global__ __launch_bounds__(256) void sinogram_kernel(const double *__restrict_
     double acc = 0.0;
     for (int k = 0; k < n_ell; k++)
         acc += ellipse_line_integral(e + 7 * k, c, s, t);
+    for (int k = 0; k < n_wav; k++)
+        acc += wave_line_integral(wv + 4 * k, c, s, t, R);
     out[(sl * P + p) * W + j] = (float) acc;
+}
+
+// The same phantom sampled at pixel centres (x, y) = (i + 0.5 - axis, j + 0.5 - axis).
+__global__ __launch_bounds__(256) void phantom_image_kernel(const double *__restrict__ ell, int n_ell,
+                                                            const double *__restrict__ wav, int n_wav, double R,
+                                                            int N, double axis, float *__restrict__ out)
+{
+    const long sl = blockIdx.z;
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= N) return;
+    const double x = i + 0.5 - axis, y = j + 0.5 - axis;
+    double v = 0.0;
+    for (int k = 0; k < n_ell; k++) {
+        const double *q = ell + (sl * n_ell + k) * 6;
+        const double dx = x - q[0], dy = y - q[1], cp = cos(q[4]), sp = sin(q[4]);
+        const double u = (dx * cp + dy * sp) / q[2], w = (-dx * sp + dy * cp) / q[3];
+        if (u * u + w * w <= 1.0) v += q[5];
+    }
+    if (x * x + y * y < R * R)
+        for (int k = 0; k < n_wav; k++) {
+            const double *q = wav + (sl * n_wav + k) * 4;
+            v += q[3] * cos(q[0] * x + q[1] * y + q[2]);
+        }
+    out[(sl * N + j) * (long) N + i] = (float) v;
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -279,21 +326,97 @@ extern "C" int ufomi_slice_ellipsoids(const double *ell, int n, double z, double
     return UFOMI_OK;
 }
 
-extern "C" int ufomi_sinogram_ellipses(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, int n_slices,
-                                       int W, int P, double axis_pos, double angle_step, float *d_sino)
+extern "C" int ufomi_phantom_texture(uint64_t seed, int n, double period_min_px, double period_max_px, double rms,
+                                     double *waves)
+{
+    if (!waves || n <= 0 || n > kMaxWave || !(period_min_px > 0.0) || !(period_max_px >= period_min_px))
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid texture arguments");
+    Rng rng(seed);
+    const double amp = rms * sqrt(2.0 / (double) n);
+    for (int k = 0; k < n; k++) {
+        // direction uniform, period log-uniform in [period_min, period_max], phase uniform
+        const double psi = 2.0 * M_PI * rng.uniform();
+        const double period = exp(rng.uniform(log(period_min_px), log(period_max_px)));
+        const double kk = 2.0 * M_PI / period;
+        waves[4 * k + 0] = kk * cos(psi);
+        waves[4 * k + 1] = kk * sin(psi);
+        waves[4 * k + 2] = 2.0 * M_PI * rng.uniform();
+        waves[4 * k + 3] = amp;
+    }
+    return UFOMI_OK;
+}
+
+namespace {
+
+int upload_phantom(ufomi_ctx *ctx, const double *ellipses, int n_ell, const double *waves, int n_wav, int n_slices,
+                   double **d_e, double **d_w)
+{
+    const size_t eb = (size_t) n_slices * n_ell * 6 * sizeof(double);
+    const size_t wb = (size_t) n_slices * n_wav * 4 * sizeof(double);
+    const size_t woff = (eb + 255) & ~(size_t) 255;
+    char *d = nullptr;
+    int rc;
+    if ((rc = ufomi_scratch(ctx, woff + wb + 8, (void **) &d))) return rc;
+    std::vector<char> h(woff + wb + 8, 0);
+    if (eb) memcpy(h.data(), ellipses, eb);
+    if (wb) memcpy(h.data() + woff, waves, wb);
+    if ((rc = ufomi_upload(ctx, d, h.data(), h.size()))) return rc;
+    *d_e = (double *) d;
+    *d_w = (double *) (d + woff);
+    return UFOMI_OK;
+}
+
+int check_phantom(const double *ellipses, int n_ell, const double *waves, int n_wav, int n_slices)
+{
+    if (n_ell < 0 || n_ell > kMaxEll || (n_ell && !ellipses) || n_wav < 0 || n_wav > kMaxWave || (n_wav && !waves) ||
+        n_slices <= 0 || n_slices > 65535)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid phantom arguments (%d ellipses, %d waves, %d slices)", n_ell, n_wav,
+                          n_slices);
+    return UFOMI_OK;
+}
+
+}  // namespace
+
+extern "C" int ufomi_sinogram_phantom(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, const double *waves,
+                                      int n_waves, double radius_px, int n_slices, int W, int P, double axis_pos,
+                                      double angle_step, float *d_sino)
 {
     int rc = ufomi_enter(ctx);
     if (rc) return rc;
-    if (!ellipses || !d_sino || n_ellipses <= 0 || n_ellipses > kMaxEll || n_slices <= 0 || n_slices > 65535 ||
-        W <= 0 || P <= 0 || P > 65535)
+    if ((rc = check_phantom(ellipses, n_ellipses, waves, n_waves, n_slices))) return rc;
+    if (!d_sino || W <= 0 || P <= 0 || P > 65535)
         return ufomi_fail(UFOMI_ERR_ARG, "invalid sinogram arguments");
-    const size_t bytes = (size_t) n_slices * n_ellipses * 6 * sizeof(double);
-    double *d_e = nullptr;
-    if ((rc = ufomi_scratch(ctx, bytes, (void **) &d_e))) return rc;
-    if ((rc = ufomi_upload(ctx, d_e, ellipses, bytes))) return rc;
+    double *d_e, *d_w;
+    if ((rc = upload_phantom(ctx, ellipses, n_ellipses, waves, n_waves, n_slices, &d_e, &d_w))) return rc;
     dim3 grid((W + 255) / 256, P, n_slices);
-    hipLaunchKernelGGL(sinogram_kernel, grid, dim3(256), 0, ctx->stream, d_e, n_ellipses, W, P, axis_pos,
-                       angle_step, d_sino);
+    hipLaunchKernelGGL(sinogram_kernel, grid, dim3(256), 0, ctx->stream, d_e, n_ellipses, d_w, n_waves, radius_px, W,
+                       P, axis_pos, angle_step, d_sino);
+    UFOMI_CHECK_LAUNCH();
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_sinogram_ellipses(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, int n_slices,
+                                       int W, int P, double axis_pos, double angle_step, float *d_sino)
+{
+    if (!ellipses || n_ellipses <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid sinogram arguments");
+    return ufomi_sinogram_phantom(ctx, ellipses, n_ellipses, nullptr, 0, 0.0, n_slices, W, P, axis_pos, angle_step,
+                                  d_sino);
+}
+
+extern "C" int ufomi_phantom_image(ufomi_ctx *ctx, const double *ellipses, int n_ellipses, const double *waves,
+                                   int n_waves, double radius_px, int n_slices, int N, double axis_pos, float *d_img)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_phantom(ellipses, n_ellipses, waves, n_waves, n_slices))) return rc;
+    if (!d_img || N <= 0 || N > 65535)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid image arguments");
+    double *d_e, *d_w;
+    if ((rc = upload_phantom(ctx, ellipses, n_ellipses, waves, n_waves, n_slices, &d_e, &d_w))) return rc;
+    dim3 grid((N + 255) / 256, N, n_slices);
+    hipLaunchKernelGGL(phantom_image_kernel, grid, dim3(256), 0, ctx->stream, d_e, n_ellipses, d_w, n_waves,
+                       radius_px, N, axis_pos, d_img);
     UFOMI_CHECK_LAUNCH();
     return UFOMI_OK;
 }

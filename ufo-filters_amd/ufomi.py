@@ -38,7 +38,10 @@ EXPORTS = [
     "ufomi_gbp_projection_rows", "ufomi_project_ellipsoids", "ufomi_device_alloc", "ufomi_device_free",
     "ufomi_memcpy", "ufomi_device_memory", "ufomi_fbp_batch_create", "ufomi_fbp_batch_configure",
     "ufomi_fbp_batch_autotune", "ufomi_fbp_batch_run", "ufomi_fbp_batch_launch", "ufomi_fbp_batch_get_info",
-    "ufomi_fbp_batch_destroy",
+    "ufomi_fbp_batch_destroy", "ufomi_phantom_texture", "ufomi_sinogram_phantom", "ufomi_phantom_image",
+    "ufomi_unet_create", "ufomi_unet_destroy", "ufomi_unet_num_params", "ufomi_unet_num_tensors", "ufomi_unet_tensor",
+    "ufomi_unet_get_params", "ufomi_unet_set_params", "ufomi_unet_forward", "ufomi_unet_gradients",
+    "ufomi_unet_train_step", "ufomi_nn_conv3x3", "ufomi_bench_mfma",
 ]
 
 
@@ -120,11 +123,23 @@ def load():
         "ufomi_project_ellipsoids": [vp, dp, ci, ci, ci, ci, dp, vp],
         "ufomi_bench_lds_read": [vp, ci, ci, vp, ctypes.POINTER(cd)],
         "ufomi_bench_copy": [vp, vp, vp, csz],
+        "ufomi_bench_mfma": [vp, ci, ci, vp, ctypes.POINTER(cd)],
         "ufomi_fbp_batch_create": [vp, ctypes.POINTER(FbpRequest), ci, ci, cf, ci, ctypes.POINTER(vp)],
         "ufomi_fbp_batch_configure": [vp, vp, ci, ci],
         "ufomi_fbp_batch_autotune": [vp, vp, ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(cf)],
         "ufomi_fbp_batch_run": [vp, vp], "ufomi_fbp_batch_launch": [vp, vp, ci],
         "ufomi_fbp_batch_get_info": [vp, ctypes.POINTER(FbpBatchInfo)], "ufomi_fbp_batch_destroy": [vp],
+        "ufomi_phantom_texture": [ctypes.c_uint64, ci, cd, cd, cd, dp],
+        "ufomi_sinogram_phantom": [vp, dp, ci, dp, ci, cd, ci, ci, ci, cd, cd, vp],
+        "ufomi_phantom_image": [vp, dp, ci, dp, ci, cd, ci, ci, cd, vp],
+        "ufomi_unet_create": [vp, ci, ci, ci, ci, ctypes.c_uint64, ctypes.POINTER(vp)], "ufomi_unet_destroy": [vp],
+        "ufomi_unet_num_params": [vp, ctypes.POINTER(csz), ctypes.POINTER(csz)], "ufomi_unet_num_tensors": [vp],
+        "ufomi_unet_tensor": [vp, ci, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(csz), ctypes.POINTER(ci)],
+        "ufomi_unet_get_params": [vp, vp, vp], "ufomi_unet_set_params": [vp, vp, vp],
+        "ufomi_unet_forward": [vp, vp, vp, ci, vp],
+        "ufomi_unet_gradients": [vp, vp, vp, vp, ci, ctypes.POINTER(cf), vp],
+        "ufomi_unet_train_step": [vp, vp, vp, vp, ci, cf, ctypes.POINTER(cf)],
+        "ufomi_nn_conv3x3": [vp, vp, ci, ci, ci, ci, vp, vp, ci, ci, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -166,6 +181,14 @@ def filter_coefficients(width, filter="ramp-fromreal", scale=1.0, cutoff=0.5, or
 def phantom_ellipses(seed, n, radius_px):
     out = np.empty((n, 6), dtype=np.float64)
     check(load().ufomi_phantom_ellipses(seed, n, radius_px, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def phantom_texture(seed, n, period_min_px, period_max_px, rms):
+    """[n][4] plane waves (kx, ky, phase, amplitude) of a band-limited texture."""
+    out = np.empty((n, 4), dtype=np.float64)
+    check(load().ufomi_phantom_texture(seed, n, period_min_px, period_max_px, rms,
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     return out
 
 
@@ -300,6 +323,29 @@ class Context:
                                _ptr(d_vol)))
 
     # -- synthetic data
+    def sinogram_phantom(self, ellipses, waves, radius, W, P, axis_pos, angle_step, d_out):
+        """ellipses [S][n][6] (or [n][6]), waves [S][m][4] (or [m][4], or None)."""
+        e = np.ascontiguousarray(ellipses, dtype=np.float64)
+        n_sl = 1 if e.ndim == 2 else e.shape[0]
+        n_ell = e.shape[-2]
+        w = None if waves is None else np.ascontiguousarray(waves, dtype=np.float64)
+        n_w = 0 if w is None else w.shape[-2]
+        check(self.L.ufomi_sinogram_phantom(self.h, _dptr(e), n_ell, None if w is None else _dptr(w), n_w, radius,
+                                            n_sl, W, P, axis_pos, angle_step, _ptr(d_out)))
+
+    def phantom_image(self, ellipses, waves, radius, N, axis_pos, d_out):
+        e = np.ascontiguousarray(ellipses, dtype=np.float64)
+        n_sl = 1 if e.ndim == 2 else e.shape[0]
+        n_ell = e.shape[-2]
+        w = None if waves is None else np.ascontiguousarray(waves, dtype=np.float64)
+        n_w = 0 if w is None else w.shape[-2]
+        check(self.L.ufomi_phantom_image(self.h, _dptr(e), n_ell, None if w is None else _dptr(w), n_w, radius,
+                                         n_sl, N, axis_pos, _ptr(d_out)))
+
+    def nn_conv3x3(self, d_in, batch, height, width, cin, d_w, d_bias, relu, cout, d_out):
+        check(self.L.ufomi_nn_conv3x3(self.h, _ptr(d_in), batch, height, width, cin, _ptr(d_w), _ptr(d_bias),
+                                      int(relu), cout, _ptr(d_out)))
+
     def sinogram_ellipses(self, ellipses, W, P, axis_pos, angle_step, d_out):
         e = np.ascontiguousarray(ellipses, dtype=np.float64)
         if e.ndim == 2:
@@ -325,6 +371,11 @@ class Context:
 
     def bench_copy(self, d_in, d_out):
         check(self.L.ufomi_bench_copy(self.h, _ptr(d_in), _ptr(d_out), d_in.numel()))
+
+    def bench_mfma(self, blocks, iters, d_sink):
+        f = ctypes.c_double()
+        check(self.L.ufomi_bench_mfma(self.h, blocks, iters, _ptr(d_sink), ctypes.byref(f)))
+        return f.value
 
 
 class FbpBatch:
@@ -378,6 +429,72 @@ class FbpBatch:
     def close(self):
         if self.h:
             self.L.ufomi_fbp_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Unet:
+    """The learned sparse-view stage (ufomi_unet_*): images are float32 torch tensors [B][H][W]."""
+
+    def __init__(self, ctx, max_batch, height, width, training=False, seed=0):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.h = ctypes.c_void_p()
+        check(self.L.ufomi_unet_create(ctx.h, max_batch, height, width, int(training), seed, ctypes.byref(self.h)))
+        c, p = ctypes.c_size_t(), ctypes.c_size_t()
+        check(self.L.ufomi_unet_num_params(self.h, ctypes.byref(c), ctypes.byref(p)))
+        self.count, self.padded = c.value, p.value
+        self.tensors = []
+        for i in range(self.L.ufomi_unet_num_tensors(self.h)):
+            name, off, dims = ctypes.c_char_p(), ctypes.c_size_t(), (ctypes.c_int * 4)()
+            check(self.L.ufomi_unet_tensor(self.h, i, ctypes.byref(name), ctypes.byref(off), dims))
+            self.tensors.append((name.value.decode(), off.value, tuple(dims)))
+
+    def get_params(self):
+        out = np.empty(self.padded, dtype=np.float32)
+        check(self.L.ufomi_unet_get_params(self.ctx.h, self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.padded
+        check(self.L.ufomi_unet_set_params(self.ctx.h, self.h, p.ctypes.data_as(ctypes.c_void_p)))
+
+    def unpack(self, flat):
+        """{name: array of the tensor's dims} views of a flat parameter array."""
+        out = {}
+        for name, off, dims in self.tensors:
+            n = int(np.prod(dims))
+            out[name] = flat[off:off + n].reshape(dims)
+        return out
+
+    def forward(self, d_in, d_out, batch=None):
+        batch = d_in.shape[0] if batch is None else batch
+        check(self.L.ufomi_unet_forward(self.ctx.h, self.h, _ptr(d_in), batch, _ptr(d_out)))
+
+    def gradients(self, d_in, d_target, batch=None):
+        batch = d_in.shape[0] if batch is None else batch
+        loss = ctypes.c_float()
+        g = np.empty(self.padded, dtype=np.float32)
+        check(self.L.ufomi_unet_gradients(self.ctx.h, self.h, _ptr(d_in), _ptr(d_target), batch, ctypes.byref(loss),
+                                          g.ctypes.data_as(ctypes.c_void_p)))
+        return loss.value, g
+
+    def train_step(self, d_in, d_target, lr, batch=None, want_loss=True):
+        batch = d_in.shape[0] if batch is None else batch
+        loss = ctypes.c_float()
+        check(self.L.ufomi_unet_train_step(self.ctx.h, self.h, _ptr(d_in), _ptr(d_target), batch, lr,
+                                           ctypes.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def close(self):
+        if self.h:
+            self.L.ufomi_unet_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
