@@ -585,6 +585,8 @@ def main():
                     help="filtered-sinogram storage; 'all' (the fbp2048 default) reports f32 as the value "
                          "and f16 / bf16 beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--bp-config", default="", help="S,A,H: slices per tile, angle chunk, f16 half staging "
+                                                     "(ufomi_bp_configure; default automatic)")
     ap.add_argument("--views", type=int, default=120, help="learned: sparse view count (divides 1800)")
     ap.add_argument("--train-pairs", type=int, default=20000, help="learned: training pairs")
     ap.add_argument("--train-steps", type=int, default=0, help="learned: Adam steps (0: one epoch)")
@@ -603,6 +605,8 @@ def main():
     torch.cuda.set_device(0)
     ctx = ufomi.Context(0)
     ctx.set_stream(torch.cuda.current_stream(0))
+    if args.bp_config:
+        ctx.bp_configure(*[int(v) for v in args.bp_config.split(",")])
 
     storage = args.storage or ("all" if args.workload == "fbp2048" else "f32")
     storages = ["f32", "f16", "bf16"] if storage == "all" else [storage]

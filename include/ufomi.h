@@ -142,6 +142,18 @@ typedef struct {
 void ufomi_bp_params_init(ufomi_bp_params *p);
 
 /*
+ * Tile configuration of ufomi_bp_parallel for this context: slices sharing
+ * one voxel tile (0 = automatic: 8, 4, 2 or 1 as n_slices allows; or 1, 2, 4,
+ * 8), angles per LDS chunk (0 = automatic, 8, 16), and, for f16 storage,
+ * half-precision (v, d) staging interpolated with v_dot2_f32_f16 (half the
+ * LDS bytes; adds one f16 rounding of the sample difference and of the
+ * weight): 0 or positive = on (the default), negative = off.  Slices and
+ * chunk do not change results; half staging does.  No counterpart in the
+ * reference.
+ */
+int ufomi_bp_configure(ufomi_ctx *ctx, int slices_per_tile, int angle_chunk, int half_staging);
+
+/*
  * `backproject`: n_slices sinograms [P_in][W] (slice pitch in elements, 0 =
  * P_in * W) -> n_slices slices [roi_h][roi_w] (out slice pitch 0 = roi_w *
  * roi_h).  Replaces backproject_tex / backproject_nearest

@@ -119,6 +119,41 @@ def test_nearest_vs_oracle(ctx, torch_dev):
     assert rel_l2(got, ref) < 1e-6
 
 
+@pytest.mark.parametrize("slices,chunk", [(1, 8), (2, 8), (4, 8), (8, 8), (8, 16)])
+def test_tile_configurations_bitwise(ctx, torch_dev, slices, chunk):
+    """Slices per tile and angles per LDS chunk change the schedule, not the arithmetic."""
+    rng = np.random.default_rng(12)
+    sinos = rng.standard_normal((11, 70, 64)).astype(np.float32)
+    ref = _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75)
+    try:
+        ctx.bp_configure(slices, chunk, -1)
+        got = _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75)
+        got16 = _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75, dtype=ufomi.BF16)
+    finally:
+        ctx.bp_configure(0, 0, 0)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(got16, _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75, dtype=ufomi.BF16))
+
+
+@pytest.mark.parametrize("slices", [4, 8])
+def test_half_staging(ctx, torch_dev, slices):
+    """f16 storage with half (v, d) staging and v_dot2 interpolation: one more f16
+    rounding (the sample difference and the weight), still within the f16 bar."""
+    rng = np.random.default_rng(13)
+    sinos = rng.standard_normal((9, 64, 80)).astype(np.float32)
+    ref = np.stack([oracle.bp_texture_f64(s) for s in sinos])
+    try:
+        ctx.bp_configure(slices, 8, 1)
+        got = _bp(ctx, torch_dev, sinos, n_slices=9, dtype=ufomi.F16)
+    finally:
+        ctx.bp_configure(0, 0, 0)
+    assert rel_l2(got, ref) < 1e-3
+    with pytest.raises(ufomi.UfomiError):
+        ctx.bp_configure(3, 0, 0)
+    with pytest.raises(ufomi.UfomiError):
+        ctx.bp_configure(0, 32, 0)
+
+
 @pytest.mark.parametrize("dtype,tol", [(ufomi.F16, 1e-3), (ufomi.BF16, 8e-3)])
 def test_low_precision_storage(ctx, torch_dev, dtype, tol):
     rng = np.random.default_rng(2)

@@ -9,7 +9,7 @@ sum) compound layer after layer, so the oracle is evaluated twice, with f64 and
 with f32 accumulation; the distance between those two correct evaluations is
 the floor, and the device must stay within 3x that floor (+2e-3) of the f64
 oracle: for the output's learned part (out - img) and for every parameter
-gradient tensor.  The loss agrees within 1e-4 relative.
+gradient tensor; the loss within 5x its floor + 1e-4 relative.
 """
 import numpy as np
 import pytest
@@ -90,8 +90,8 @@ def test_gradients_vs_oracle(ctx, torch_dev):
     tgt = (img + 0.5 * rng.standard_normal((B, H, W))).astype(np.float32)
     loss, g = net.gradients(torch.from_numpy(img).to(torch_dev), torch.from_numpy(tgt).to(torch_dev))
     ref_loss, ref = U.UNet(params).gradients(img.astype(np.float64), tgt.astype(np.float64))
-    _, r32 = U.UNet(params, acc=np.float32).gradients(img.astype(np.float64), tgt.astype(np.float64))
-    assert loss == pytest.approx(ref_loss, rel=1e-4)
+    loss32, r32 = U.UNet(params, acc=np.float32).gradients(img.astype(np.float64), tgt.astype(np.float64))
+    assert abs(loss - ref_loss) <= 5 * abs(loss32 - ref_loss) + 1e-4 * ref_loss
     got = net.unpack(g)
     for name, _ in U.layout():
         floor = rel_l2(r32[name], ref[name])

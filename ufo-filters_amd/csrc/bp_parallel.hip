@@ -24,6 +24,7 @@
 #include "ufomi_internal.h"
 
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -34,10 +35,30 @@ constexpr int kTile = 32;       // voxels per tile side
 constexpr int kA = 16;          // angles per chunk
 constexpr float kPiF = 3.14159274101257324219f;  // M_PI_F
 
-template <int S> struct Entry;
-template <> struct Entry<1> { using T = float2; };
-template <> struct Entry<2> { using T = float4; };
-template <> struct Entry<4> { using T = float4; };  // two float4 planes
+// Layout of one staged detector entry for S slices in LDS.
+//  f32 (v, d) pairs: S = 1 one float2; S >= 2 S/2 float4 planes, each the
+//  (v, d) of two slices.  H16 (f16 storage, S >= 4): S/4 uint4 planes, each
+//  the half2 (v, d) of four slices, consumed by v_dot2 with (1, f).
+template <int S, bool H16> struct LdsT {
+    using E = float4;
+    static constexpr int NP = S / 2;
+};
+template <> struct LdsT<1, false> {
+    using E = float2;
+    static constexpr int NP = 1;
+};
+template <int S> struct LdsT<S, true> {
+    using E = uint4;
+    static constexpr int NP = S / 4;
+};
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_vd(float v, float d)
+{
+    const _Float16 hv = (_Float16) v, hd = (_Float16) d;
+    return (unsigned) __builtin_bit_cast(unsigned short, hv) | ((unsigned) __builtin_bit_cast(unsigned short, hd) << 16);
+}
 
 __device__ __forceinline__ int xcd_remap(int b, int nb)
 {
@@ -101,17 +122,22 @@ __device__ __forceinline__ AngleSpan angle_span(float2 sc, bool valid, float bx0
 // Every thread of the workgroup calls it (it synchronises); it leaves the LDS
 // buffers free for the next call.  The detector span of a tile at one angle is
 // at most (TILE - 1) sqrt 2 + 3 entries, inside STRIDE = 2 TILE.
-template <int S, int TILE, int A, typename InT, typename InPtr = const InT *, typename OutPtr = float *>
+// H16 (f16 storage only) stages (v, d) as half2 and interpolates with
+// v_dot2_f32_f16: half the LDS bytes and one VALU op per update.
+template <int S, int TILE, int A, typename InT, bool H16 = false, typename InPtr = const InT *,
+          typename OutPtr = float *>
 __device__ __forceinline__ void bp_tile(
     InPtr __restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
     float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices, int slice0,
     OutPtr __restrict__ out, long out_pitch, int x0, int y0)
 {
-    using E = typename Entry<S>::T;
+    static_assert(!H16 || (S % 4 == 0 && std::is_same<InT, __half>::value), "half staging: f16, 4 | S");
+    using L = LdsT<S, H16>;
+    using E = typename L::E;
     constexpr int K = TILE / 16;
     constexpr int STRIDE = 2 * TILE;
     constexpr int ROWS = kThreads / STRIDE;           // angles staged per pass
-    constexpr int NPLANE = S == 4 ? 2 : 1;
+    constexpr int NPLANE = L::NP;
     constexpr int PASSES = A * STRIDE / kThreads;     // staging passes per chunk
     static_assert(A % ROWS == 0, "angle chunk must be a multiple of the staged rows per pass");
     __shared__ E lds[NPLANE][A][STRIDE];
@@ -189,13 +215,18 @@ __device__ __forceinline__ void bp_tile(
                 sv[s] = v0;
                 sd[s] = v1 - v0;
             }
-            if constexpr (S == 1) {
+            if constexpr (H16) {
+#pragma unroll
+                for (int pl = 0; pl < NPLANE; pl++)
+                    lds[pl][pa][e] = make_uint4(pack_vd(sv[4 * pl], sd[4 * pl]), pack_vd(sv[4 * pl + 1], sd[4 * pl + 1]),
+                                                pack_vd(sv[4 * pl + 2], sd[4 * pl + 2]),
+                                                pack_vd(sv[4 * pl + 3], sd[4 * pl + 3]));
+            } else if constexpr (S == 1) {
                 lds[0][pa][e] = make_float2(sv[0], sd[0]);
-            } else if constexpr (S == 2) {
-                lds[0][pa][e] = make_float4(sv[0], sd[0], sv[1], sd[1]);
             } else {
-                lds[0][pa][e] = make_float4(sv[0], sd[0], sv[1], sd[1]);
-                lds[1][pa][e] = make_float4(sv[2], sd[2], sv[3], sd[3]);
+#pragma unroll
+                for (int pl = 0; pl < NPLANE; pl++)
+                    lds[pl][pa][e] = make_float4(sv[2 * pl], sd[2 * pl], sv[2 * pl + 1], sd[2 * pl + 1]);
             }
             if (e == 0)
                 par[pa] = make_float4(sp[m].s, sp[m].c, sp[m].a, 0.0f);
@@ -237,30 +268,38 @@ __device__ __forceinline__ void bp_tile(
                     fv[K * ky + kx] = __builtin_amdgcn_fractf(u);
                 }
             }
-            E q[KK];
+            E q[NPLANE][KK];
 #pragma unroll
-            for (int v = 0; v < KK; v++)
-                q[v] = lds[0][pa][iv[v]];
-            if constexpr (S == 1) {
+            for (int pl = 0; pl < NPLANE; pl++)
 #pragma unroll
                 for (int v = 0; v < KK; v++)
-                    acc[v / K][v % K][0] += fmaf(fv[v], q[v].y, q[v].x);
-            } else {
-                float4 q2[KK];
-                if constexpr (S == 4) {
-#pragma unroll
-                    for (int v = 0; v < KK; v++)
-                        q2[v] = lds[1][pa][iv[v]];
-                }
+                    q[pl][v] = lds[pl][pa][iv[v]];
+            if constexpr (H16) {
 #pragma unroll
                 for (int v = 0; v < KK; v++) {
-                    acc[v / K][v % K][0] += fmaf(fv[v], q[v].y, q[v].x);
-                    acc[v / K][v % K][1] += fmaf(fv[v], q[v].w, q[v].z);
-                    if constexpr (S == 4) {
-                        acc[v / K][v % K][2] += fmaf(fv[v], q2[v].y, q2[v].x);
-                        acc[v / K][v % K][3] += fmaf(fv[v], q2[v].w, q2[v].z);
+                    const half2_t w = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(1.0f, fv[v]));
+#pragma unroll
+                    for (int pl = 0; pl < NPLANE; pl++) {
+                        const uint4 qq = q[pl][v];
+                        float *a = &acc[v / K][v % K][4 * pl];
+                        a[0] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, qq.x), w, a[0], false);
+                        a[1] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, qq.y), w, a[1], false);
+                        a[2] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, qq.z), w, a[2], false);
+                        a[3] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, qq.w), w, a[3], false);
                     }
                 }
+            } else if constexpr (S == 1) {
+#pragma unroll
+                for (int v = 0; v < KK; v++)
+                    acc[v / K][v % K][0] += fmaf(fv[v], q[0][v].y, q[0][v].x);
+            } else {
+#pragma unroll
+                for (int v = 0; v < KK; v++)
+#pragma unroll
+                    for (int pl = 0; pl < NPLANE; pl++) {
+                        acc[v / K][v % K][2 * pl] += fmaf(fv[v], q[pl][v].y, q[pl][v].x);
+                        acc[v / K][v % K][2 * pl + 1] += fmaf(fv[v], q[pl][v].w, q[pl][v].z);
+                    }
             }
         }
         __syncthreads();
@@ -290,19 +329,20 @@ __device__ __forceinline__ void bp_tile(
     }
 }
 
-template <int S, typename InT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void bp_texture_kernel(
-    const InT *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
-    float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
-    float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y)
+// S = 8 holds about twice the registers of S = 4: fewer waves per SIMD.
+template <int S, int A, typename InT, bool H16>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(S == 8 ? (H16 ? 3 : 2) : 4))) void
+bp_texture_kernel(const InT *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
+                  float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
+                  float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y)
 {
     const int nb = tiles_x * tiles_y * ((n_slices + S - 1) / S);
     const int b = xcd_remap(blockIdx.x, nb);
     const int tiles = tiles_x * tiles_y;
     const int g = b / tiles;
     const int ty = (b % tiles) / tiles_x, tx = b % tiles_x;
-    bp_tile<S, kTile, kA, InT>(sino, sino_pitch, W, P, trig, axis, roi_x, roi_y, roi_w, roi_h, n_slices, g * S,
-                               out, out_pitch, tx * kTile, ty * kTile);
+    bp_tile<S, kTile, A, InT, H16>(sino, sino_pitch, W, P, trig, axis, roi_x, roi_y, roi_w, roi_h, n_slices, g * S,
+                                   out, out_pitch, tx * kTile, ty * kTile);
 }
 
 // Persistent backprojection of a batch of single-slice requests of uneven
@@ -329,7 +369,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         // stores are global (not flat) instructions and do not hold up LDS waits.
         using GIn = __attribute__((address_space(1))) const InT;
         using GOut = __attribute__((address_space(1))) float;
-        bp_tile<1, TILE, A, InT, GIn *, GOut *>((GIn *) r.sino, 0, r.W, r.P, trig + r.trig0, r.axis, r.roi_x,
+        bp_tile<1, TILE, A, InT, false, GIn *, GOut *>((GIn *) r.sino, 0, r.W, r.P, trig + r.trig0, r.axis, r.roi_x,
                                                 r.roi_y, r.roi_w, r.roi_h, 1, 0, (GOut *) r.out, 0, tx * TILE,
                                                 ty * TILE);
     }
@@ -407,6 +447,20 @@ int get_trig(ufomi_ctx *ctx, double angle_offset, double step, unsigned offset, 
 
 }  // namespace
 
+extern "C" int ufomi_bp_configure(ufomi_ctx *ctx, int slices_per_tile, int angle_chunk, int half_staging)
+{
+    if (!ctx) return ufomi_fail(UFOMI_ERR_ARG, "context is NULL");
+    if (slices_per_tile != 0 && slices_per_tile != 1 && slices_per_tile != 2 && slices_per_tile != 4 &&
+        slices_per_tile != 8)
+        return ufomi_fail(UFOMI_ERR_ARG, "slices per tile must be 0 (automatic), 1, 2, 4 or 8");
+    if (angle_chunk != 0 && angle_chunk != 8 && angle_chunk != 16)
+        return ufomi_fail(UFOMI_ERR_ARG, "angle chunk must be 0 (automatic), 8 or 16");
+    ctx->bp_slices = slices_per_tile;
+    ctx->bp_chunk = angle_chunk;
+    ctx->bp_half = half_staging;
+    return UFOMI_OK;
+}
+
 extern "C" void ufomi_bp_params_init(ufomi_bp_params *p)
 {
     memset(p, 0, sizeof(*p));
@@ -468,25 +522,46 @@ extern "C" int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, 
     if (prm->mode != UFOMI_BP_TEXTURE)
         return ufomi_fail(UFOMI_ERR_ARG, "unknown backprojection mode %d", prm->mode);
 
-    const int S = n_slices >= 4 ? 4 : (n_slices >= 2 ? 2 : 1);
+    // Tile configuration: slices sharing a tile (S), angles per LDS chunk (A),
+    // half-precision staging for f16 storage; ufomi_bp_configure overrides.
+    // Measured on MI355X (2048 slices of 1800 x 2048): S = 8, A = 8 is the
+    // fastest f32 configuration; half staging speeds f16 storage up further.
+    int S = n_slices >= 8 ? 8 : (n_slices >= 4 ? 4 : (n_slices >= 2 ? 2 : 1));
+    if (ctx->bp_slices && n_slices >= ctx->bp_slices) S = ctx->bp_slices;
+    const bool h16 = dtype == UFOMI_F16 && S >= 4 && ctx->bp_half >= 0;
+    // S = 8 with half staging keeps its registers only with 8-angle chunks
+    const int A = (S == 8 && h16) ? 8 : (ctx->bp_chunk ? ctx->bp_chunk : (S >= 4 ? 8 : 16));
     const int tiles_x = (roi_w + kTile - 1) / kTile, tiles_y = (roi_h + kTile - 1) / kTile;
     const long nb = (long) tiles_x * tiles_y * ((n_slices + S - 1) / S);
     if (nb > 0x7fffffffL)
         return ufomi_fail(UFOMI_ERR_ARG, "too many tiles");
-    auto go = [&](auto s_const, auto *in) {
-        constexpr int SS = decltype(s_const)::value;
+    auto go = [&](auto s_const, auto a_const, auto h_const, auto *in) {
+        constexpr int SS = decltype(s_const)::value, AA = decltype(a_const)::value;
+        constexpr bool HH = decltype(h_const)::value;
         using InT = std::remove_const_t<std::remove_pointer_t<decltype(in)>>;
-        hipLaunchKernelGGL((bp_texture_kernel<SS, InT>), dim3((unsigned) nb), dim3(kThreads), 0, ctx->stream,
-                           in, (long) sino_pitch, W, P_in, trig, axis, (float) prm->roi_x, (float) prm->roi_y,
-                           roi_w, roi_h, n_slices, d_out, (long) out_pitch, tiles_x, tiles_y);
-        UFOMI_CHECK_LAUNCH();
-        return (int) UFOMI_OK;
+        if constexpr (HH && !(std::is_same<InT, __half>::value && SS % 4 == 0)) {
+            return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "half staging needs f16 storage and 4 | S");
+        } else {
+            hipLaunchKernelGGL((bp_texture_kernel<SS, AA, InT, HH>), dim3((unsigned) nb), dim3(kThreads), 0,
+                               ctx->stream, in, (long) sino_pitch, W, P_in, trig, axis, (float) prm->roi_x,
+                               (float) prm->roi_y, roi_w, roi_h, n_slices, d_out, (long) out_pitch, tiles_x, tiles_y);
+            UFOMI_CHECK_LAUNCH();
+            return (int) UFOMI_OK;
+        }
+    };
+    using I8 = std::integral_constant<int, 8>;
+    using I16 = std::integral_constant<int, 16>;
+    auto by_a = [&](auto s_const, auto h_const, auto *in) {
+        return A == 8 ? go(s_const, I8(), h_const, in) : go(s_const, I16(), h_const, in);
     };
     auto by_s = [&](auto *in) {
+        using F = std::false_type;
+        using T = std::true_type;
         switch (S) {
-        case 4: return go(std::integral_constant<int, 4>(), in);
-        case 2: return go(std::integral_constant<int, 2>(), in);
-        default: return go(std::integral_constant<int, 1>(), in);
+        case 8: return h16 ? by_a(std::integral_constant<int, 8>(), T(), in) : by_a(std::integral_constant<int, 8>(), F(), in);
+        case 4: return h16 ? by_a(std::integral_constant<int, 4>(), T(), in) : by_a(std::integral_constant<int, 4>(), F(), in);
+        case 2: return by_a(std::integral_constant<int, 2>(), F(), in);
+        default: return by_a(std::integral_constant<int, 1>(), F(), in);
         }
     };
     switch (dtype) {

@@ -25,6 +25,8 @@ struct ufomi_ctx {
     int lut_n;
     double lut_offset, lut_step;
     unsigned lut_first;
+    // parallel backprojection tile configuration (0 = automatic), ufomi_bp_configure
+    int bp_slices, bp_chunk, bp_half;
 };
 
 #define UFOMI_FFT_MAX 16384  // twiddle table length

@@ -41,7 +41,7 @@ EXPORTS = [
     "ufomi_fbp_batch_destroy", "ufomi_phantom_texture", "ufomi_sinogram_phantom", "ufomi_phantom_image",
     "ufomi_unet_create", "ufomi_unet_destroy", "ufomi_unet_num_params", "ufomi_unet_num_tensors", "ufomi_unet_tensor",
     "ufomi_unet_get_params", "ufomi_unet_set_params", "ufomi_unet_forward", "ufomi_unet_gradients",
-    "ufomi_unet_train_step", "ufomi_nn_conv3x3", "ufomi_bench_mfma",
+    "ufomi_unet_train_step", "ufomi_nn_conv3x3", "ufomi_bench_mfma", "ufomi_bp_configure",
 ]
 
 
@@ -124,6 +124,7 @@ def load():
         "ufomi_bench_lds_read": [vp, ci, ci, vp, ctypes.POINTER(cd)],
         "ufomi_bench_copy": [vp, vp, vp, csz],
         "ufomi_bench_mfma": [vp, ci, ci, vp, ctypes.POINTER(cd)],
+        "ufomi_bp_configure": [vp, ci, ci, ci],
         "ufomi_fbp_batch_create": [vp, ctypes.POINTER(FbpRequest), ci, ci, cf, ci, ctypes.POINTER(vp)],
         "ufomi_fbp_batch_configure": [vp, vp, ci, ci],
         "ufomi_fbp_batch_autotune": [vp, vp, ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(cf)],
@@ -304,6 +305,10 @@ class Context:
         check(self.L.ufomi_ramp_filter(self.h, _ptr(d_in), W, P, N, _ptr(d_coef), crop, _ptr(d_out), dtype))
 
     # -- backprojection
+    def bp_configure(self, slices_per_tile=0, angle_chunk=0, half_staging=0):
+        """Tile configuration of bp_parallel (0 = automatic)."""
+        check(self.L.ufomi_bp_configure(self.h, slices_per_tile, angle_chunk, int(half_staging)))
+
     def bp_parallel(self, d_sino, W, P_in, d_out, n_slices=1, dtype=F32, sino_pitch=0, out_pitch=0, **props):
         p = bp_params(**props)
         check(self.L.ufomi_bp_parallel(self.h, _ptr(d_sino), dtype, W, P_in, n_slices, sino_pitch,
