@@ -49,14 +49,19 @@ def log(msg):
 
 
 def measure_lds_peak(ctx, torch):
-    sink = torch.empty(4096 * 256, device="cuda")
-    ctx.bench_lds_read(4096, 64, sink)  # warm-up
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    nbytes = ctx.bench_lds_read(4096, 4096, sink)
-    e1.record()
-    torch.cuda.synchronize()
-    return nbytes / (e0.elapsed_time(e1) * 1e-3)
+    """Best of 5 runs of the conflict-free ds_read_b128 microbenchmark (single runs vary by ~20%
+    with clocks and neighbours on the node)."""
+    sink = torch.empty(8192 * 256, device="cuda")
+    ctx.bench_lds_read(8192, 64, sink)  # warm-up
+    best = 0.0
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        nbytes = ctx.bench_lds_read(8192, 4096, sink)
+        e1.record()
+        torch.cuda.synchronize()
+        best = max(best, nbytes / (e0.elapsed_time(e1) * 1e-3))
+    return best
 
 
 def measure_hbm_copy(ctx, torch):
@@ -174,8 +179,13 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
 
     updates = float(S) * W * W * P
     first = storages[0]
-    S_k = 4 if S >= 4 else (2 if S >= 2 else 1)
-    kernel = "bp_texture_kernel<%d,%s>" % (S_k, STORAGE_NAMES[first])
+
+    def kname(st):
+        # the instance ufomi_bp_parallel picks: slices per tile, angle chunk, half staging
+        sk = 8 if S >= 8 else (4 if S >= 4 else (2 if S >= 2 else 1))
+        return "bp_texture_kernel<%d,%d,%s,%s>" % (sk, 8 if sk >= 4 else 16, STORAGE_NAMES[st],
+                                                   "true" if st == "f16" and sk >= 4 else "false")
+    kernel = kname(first)
     sample_bytes = 4 if first == "f32" else 2
     filt_ms = runs[first]["filt_ms"]
     extra = {
@@ -186,7 +196,7 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
     variants = {st: {"value": round(updates / (r["wall"] / args.steps) / 1e9, 2),
                      "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
                      "bp_kernel_ms": round(r["bp_ms"], 3), "filter_kernel_ms": round(r["filt_ms"], 3),
-                     "bp_kernel": "bp_texture_kernel<%d,%s>" % (S_k, STORAGE_NAMES[st])}
+                     "bp_kernel": kname(st)}
                 for st, r in runs.items()}
     cpu = accuracy = None
     if not args.no_cpu_baseline:
@@ -224,7 +234,9 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
     if len(storages) > 1:
         extra["storage_variants"] = variants
     r = runs[first]
-    return dict(updates=updates, wall=r["wall"], bp_ms=r["bp_ms"], kernel=kernel, bytes_per_update=8,
+    # LDS bytes read per update: (v, d) float pairs, or half pairs with f16 half staging
+    bpu = 4 if "true" in kernel else 8
+    return dict(updates=updates, wall=r["wall"], bp_ms=r["bp_ms"], kernel=kernel, bytes_per_update=bpu,
                 extra=extra, cpu=cpu, accuracy=accuracy, workload=workload,
                 dtype="f32" if first == "f32" else "f32 accumulate, %s storage" % first,
                 config={"slices": S, "angles": P, "detector": W, "fft_size": N, "storage": first, "filter": filt_name},

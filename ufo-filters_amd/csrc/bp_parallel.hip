@@ -54,6 +54,12 @@ template <int S> struct LdsT<S, true> {
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
+// The value of lane + 1 of the wave (DPP wave_shl:1, a VALU move; lane 63 gets 0).
+__device__ __forceinline__ float next_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
 __device__ __forceinline__ unsigned pack_vd(float v, float d)
 {
     const _Float16 hv = (_Float16) v, hd = (_Float16) d;
@@ -175,8 +181,20 @@ __device__ __forceinline__ void bp_tile(
     // branches) and keeps the raw stored values; commit() converts them after
     // the current chunk's interpolation, so the loads stay in flight meanwhile.
     using R = typename Stage<InT>::R;
-    R r0[PASSES][S], r1[PASSES][S];
+    // Each lane loads sample j = lo + e; sample j + 1 comes from the next lane
+    // of the wave (a wave covers 64 consecutive entries of one angle).  The
+    // interpolation reads entries below (TILE - 1) sqrt 2 + 2 < STRIDE - 1, so
+    // the last entry of an angle is never read; with STRIDE = 128 the entry at
+    // the wave boundary (e = 63) is, and that lane loads j + 1 itself.
+    // Float staging of 8 slices keeps both loads: its register budget (3 waves
+    // per SIMD) is what the extra lane moves would cost (measured: 1345 vs 1488 ms
+    // for 2048 x 1800 x 2048).
+    static_assert(STRIDE == 64 || STRIDE == 128, "a staged angle spans one or two waves");
+    constexpr bool kDpp = H16 || S < 8;
+    constexpr bool kSeam = kDpp && STRIDE == 128;
+    R r0[PASSES][S], r1[(kSeam || !kDpp) ? PASSES : 1][S];
     bool in0[PASSES], in1[PASSES];
+    const bool wave_last = kSeam && e == 63;
     // (sin, cos) of the angles the next fetch() stages, loaded one chunk ahead
     // so that no fetch waits for the trig table before issuing its loads.
     float2 tq[PASSES];
@@ -195,11 +213,17 @@ __device__ __forceinline__ void bp_tile(
             in0[m] = p < P && j >= 0 && j < W;
             in1[m] = p < P && j + 1 >= 0 && j + 1 < W;
             const int j0 = min(max(j, 0), W - 1), j1 = min(max(j + 1, 0), W - 1);
+            const long roff = (long) (p < P ? p : 0) * W;
 #pragma unroll
-            for (int s = 0; s < S; s++) {
-                InPtr row = src[s] + (long) (p < P ? p : 0) * W;
-                r0[m][s] = Stage<InT>::load(row, j0);
-                r1[m][s] = Stage<InT>::load(row, j1);
+            for (int s = 0; s < S; s++) r0[m][s] = Stage<InT>::load(src[s] + roff, j0);
+            if constexpr (!kDpp) {
+#pragma unroll
+                for (int s = 0; s < S; s++) r1[m][s] = Stage<InT>::load(src[s] + roff, j1);
+            } else if constexpr (kSeam) {
+                if (wave_last) {
+#pragma unroll
+                    for (int s = 0; s < S; s++) r1[m][s] = Stage<InT>::load(src[s] + roff, j1);
+                }
             }
         }
         load_trig(p0 + A);
@@ -211,7 +235,16 @@ __device__ __forceinline__ void bp_tile(
 #pragma unroll
             for (int s = 0; s < S; s++) {
                 const float v0 = in0[m] ? Stage<InT>::cvt(r0[m][s]) : 0.0f;
-                const float v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
+                // entry e + 1 of the same angle sits in the next lane of this wave
+                float v1;
+                if constexpr (!kDpp) {
+                    v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
+                } else {
+                    v1 = next_lane(v0);
+                    if constexpr (kSeam) {
+                        if (wave_last) v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
+                    }
+                }
                 sv[s] = v0;
                 sd[s] = v1 - v0;
             }
@@ -329,9 +362,9 @@ __device__ __forceinline__ void bp_tile(
     }
 }
 
-// S = 8 holds about twice the registers of S = 4: fewer waves per SIMD.
+// S = 8 (float staging) holds about twice the registers of S = 4: two waves per SIMD.
 template <int S, int A, typename InT, bool H16>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(S == 8 ? (H16 ? 3 : 2) : 4))) void
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(S == 8 && !H16 ? 2 : 4))) void
 bp_texture_kernel(const InT *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
                   float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
                   float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y)

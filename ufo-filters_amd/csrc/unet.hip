@@ -154,6 +154,18 @@ __global__ __launch_bounds__(256) void reduce_kernel(const float *__restrict__ s
     dst[i] = s;
 }
 
+// dst[g * len + i] = sum over z in [32 g, 32 g + 32) of src[z * len + i].
+__global__ __launch_bounds__(256) void reduce_groups_kernel(const float *__restrict__ src, int nz, long len,
+                                                            float *__restrict__ dst)
+{
+    const long i = (long) blockIdx.x * 256 + threadIdx.x;
+    if (i >= len) return;
+    const int z0 = blockIdx.y * 32, z1 = z0 + 32 < nz ? z0 + 32 : nz;
+    float s = 0.0f;
+    for (int z = z0; z < z1; z++) s += src[(long) z * len + i];
+    dst[(long) blockIdx.y * len + i] = s;
+}
+
 // 2x2 max pooling of C channels (C % 8 == 0); out grid (H/2, W/2).
 __global__ __launch_bounds__(256) void maxpool_kernel(View<const bf16> in, View<bf16> out, long nout, int lh, int lw,
                                                       int C)
@@ -447,7 +459,7 @@ int wgrad_splits(const ufomi_unet *n, long M, long N, long pixels, long *kchunk)
     const long tiles = ((M + 63) / 64) * ((N + 127) / 128);
     long splits = 2048 / tiles;
     if (splits < 1) splits = 1;
-    const long max_ws = (long) (n->ws_floats / (size_t) (M * N));
+    const long max_ws = (long) (n->ws_floats / (size_t) (M * N)) * 32 / 34;  // room for the group sums
     if (splits > max_ws) splits = max_ws;
     const long groups = (pixels + 255) / 256;  // at least 256 pixels per split
     if (splits > groups) splits = groups;
@@ -458,8 +470,19 @@ int wgrad_splits(const ufomi_unet *n, long M, long N, long pixels, long *kchunk)
     return (int) ((pixels + chunk - 1) / chunk);
 }
 
+// dst[i] = sum_z ws[z * len + i].  Many partials are first summed in groups of
+// 32 (one thread per (i, group), in parallel) into the workspace after them, so
+// no thread walks a long dependent chain; the order is fixed either way.
 int reduce_into(hipStream_t s, const float *ws, int nz, long len, float *dst)
 {
+    if (nz > 32) {
+        const int ng = (nz + 31) / 32;
+        float *tmp = const_cast<float *>(ws) + (long) nz * len;
+        hipLaunchKernelGGL(reduce_groups_kernel, dim3(blocks_for(len), ng), dim3(256), 0, s, ws, nz, len, tmp);
+        UFOMI_CHECK_LAUNCH();
+        ws = tmp;
+        nz = ng;
+    }
     hipLaunchKernelGGL(reduce_kernel, dim3(blocks_for(len)), dim3(256), 0, s, ws, nz, len, dst);
     UFOMI_CHECK_LAUNCH();
     return UFOMI_OK;
