@@ -26,11 +26,13 @@ def _bf16_tensor(a, torch_dev):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(torch.bfloat16).to(torch_dev)
 
 
-@pytest.mark.parametrize("cin,cout", [(32, 32), (64, 64), (32, 128), (128, 64)])
-def test_conv3x3_mfma_vs_oracle(ctx, torch_dev, cin, cout):
+@pytest.mark.parametrize("cin,cout,H,W", [(32, 32, 16, 32), (64, 64, 16, 32), (32, 128, 16, 32), (128, 64, 16, 32),
+                                          (32, 32, 8, 128), (64, 64, 4, 64), (32, 128, 16, 64), (256, 64, 8, 64)])
+def test_conv3x3_mfma_vs_oracle(ctx, torch_dev, cin, cout, H, W):
+    """W >= 64 runs the LDS-staged kernel (4 x 64 pixel patches), narrower images the generic GEMM."""
     import torch
     rng = np.random.default_rng(cin + cout)
-    B, H, W = 2, 16, 32
+    B = 2
     x = U.bf16(rng.standard_normal((B, H, W, cin)))
     w = U.bf16(rng.standard_normal((cout, 3, 3, cin)) / np.sqrt(9 * cin))
     b = rng.standard_normal(cout).astype(np.float32)

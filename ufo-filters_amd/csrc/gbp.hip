@@ -201,24 +201,33 @@ __device__ __forceinline__ void store_voxel(ST *__restrict__ vol, long o, float 
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged kernel.  A workgroup owns a 32 x 32 x 8 voxel tile; thread
-// (lx, ly) owns the columns (lx + 16 kx, ly + 16 ky), kx, ky in {0, 1}, of 8
-// voxels.  For every projection:
-//   A. wave 0 projects the 8 tile corners (the image of a box under a
-//      projective map with positive depth is the hull of the corner images)
-//      and writes the detector footprint box to LDS;
-//   B. the workgroup stages the box, one float4 per texel (t00, t10 - t00,
-//      t01 - t00, t11 - t10 - t01 + t00) with the border rule applied, so a
-//      bilinear sample is one ds_read_b128 and three FMAs;
-//   C. every voxel samples from LDS.
-// A footprint larger than the box capacity (or a tile reaching behind the
-// source) takes the global-memory path for that projection.
+// LDS-staged kernel.  A workgroup owns a tile of TX x TY x TZ voxels (32 x 32
+// x 8 for parallel beam; 16 x 16 x 16 for cone beam, whose magnification
+// widens the footprint); thread (lx, ly) owns the columns (lx + 16 kx,
+// ly + 16 ky) of TZ voxels.
+//   * Per chunk of 32 projections, the workgroup projects the 8 tile corners
+//     of each (the image of a box under a projective map with positive depth
+//     is the hull of the corner images) and keeps the 32 detector footprints
+//     in LDS; the next chunk's footprints are computed a chunk ahead.
+//   * The footprint of projection p + 1 is loaded into registers (one texel
+//     column and 12 rows per thread) while the voxels sample projection p
+//     from LDS; the two boxes alternate, one barrier per projection.
+//   * A bilinear sample reads its 4 texels with two ds_read2_b32.
+// A footprint larger than the box (or a tile reaching behind the source)
+// takes the global-memory path for that projection.
 // ---------------------------------------------------------------------------
-constexpr int kTX = 32, kTY = 32, kTZ = 8;
-constexpr int kBoxPitch = 64, kBoxRows = 40;  // 2560 float4 = 40 KiB
+constexpr int kBoxW = 64, kBoxH = 48;  // texels per staged footprint
+constexpr int kChunk = 32;             // projections per footprint chunk
+constexpr int kRowsPerThread = kBoxH / 4;
 
 struct BoxGeom {
     int i_lo, j_lo, w, h;  // w = 0: global path
+};
+
+template <int MODE> struct TileShape {
+    static constexpr bool CONE = MODE == MODE_CONE_ZSEP || MODE == MODE_CONE;
+    static constexpr int TX = CONE ? 16 : 32, TY = TX, TZ = CONE ? 16 : 8;
+    static constexpr int KA = TX / 16, KB = TY / 16;
 };
 
 // Projective (or affine) map of one projection at X = (x, y, z).
@@ -242,112 +251,135 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     float y0, float ys, const float *__restrict__ zs, int nx, int ny, int nz, float norm, float gmin, float gscale,
     float gmax, ST *__restrict__ vol, int tiles_x, int tiles_y)
 {
-    constexpr bool CONE = MODE == MODE_CONE_ZSEP || MODE == MODE_CONE;
+    using TS = TileShape<MODE>;
+    constexpr bool CONE = TS::CONE;
+    constexpr int TX = TS::TX, TY = TS::TY, TZ = TS::TZ, KA = TS::KA, KB = TS::KB;
     // U (and for cone D, C) independent of z: checked on the host for the ZSEP modes
     constexpr bool ZSEP = MODE == MODE_CONE_ZSEP || MODE == MODE_PARALLEL_ZSEP;
-    __shared__ float4 box[kBoxRows * kBoxPitch];
-    __shared__ BoxGeom bg;
+    __shared__ float box[2][kBoxH * kBoxW];
+    __shared__ BoxGeom bgs[2][kChunk];
 
     const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
-    const int X0 = tx * kTX, Y0 = ty * kTY, Z0 = tz * kTZ;
+    const int X0 = tx * TX, Y0 = ty * TY, Z0 = tz * TZ;
     const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
 
-    float xv[2], yv[2], z[kTZ];
+    float xv[KA], yv[KB], z[TZ];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-        xv[k] = fmaf((float) (X0 + lx + 16 * k), xs, x0);
-        yv[k] = fmaf((float) (Y0 + ly + 16 * k), ys, y0);
-    }
+    for (int k = 0; k < KA; k++) xv[k] = fmaf((float) (X0 + lx + 16 * k), xs, x0);
 #pragma unroll
-    for (int k = 0; k < kTZ; k++)
-        z[k] = zs[min(Z0 + k, nz - 1)];
-    // tile corners (all 32 x 32 x 8 positions, also those beyond the volume)
-    const float cxa = fmaf((float) X0, xs, x0), cxb = fmaf((float) (X0 + kTX - 1), xs, x0);
-    const float cya = fmaf((float) Y0, ys, y0), cyb = fmaf((float) (Y0 + kTY - 1), ys, y0);
-    const float cza = z[0], czb = z[kTZ - 1];
+    for (int k = 0; k < KB; k++) yv[k] = fmaf((float) (Y0 + ly + 16 * k), ys, y0);
+#pragma unroll
+    for (int k = 0; k < TZ; k++) z[k] = zs[min(Z0 + k, nz - 1)];
+    // tile corners (all TX x TY x TZ positions, also those beyond the volume)
+    const float cxa = fmaf((float) X0, xs, x0), cxb = fmaf((float) (X0 + TX - 1), xs, x0);
+    const float cya = fmaf((float) Y0, ys, y0), cyb = fmaf((float) (Y0 + TY - 1), ys, y0);
+    const float cza = z[0], czb = z[TZ - 1];
 
-    float acc[2][2][kTZ];
+    // Footprints of projections [c0, c0 + 32) into bgs[buf]: thread t maps
+    // corner t % 8 of projection c0 + t / 8.
+    auto footprints = [&](int c0, int buf) {
+        const int pj = c0 + (tid >> 3), c = tid & 7;
+        const bool live = pj < P;
+        float u = 0.0f, v = 0.0f, d = 1.0f;
+        if (live) {
+            const float4 mu = mats[4 * pj + 0], mv = mats[4 * pj + 1], md = mats[4 * pj + 2];
+            map_point<CONE>(mu, mv, md, (c & 1) ? cxb : cxa, (c & 2) ? cyb : cya, (c & 4) ? czb : cza, u, v, d);
+        }
+        float umin = u, umax = u, vmin = v, vmax = v, dmin = d;
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+        for (int o = 1; o < 8; o <<= 1) {
+            umin = fminf(umin, __shfl_xor(umin, o, 8));
+            umax = fmaxf(umax, __shfl_xor(umax, o, 8));
+            vmin = fminf(vmin, __shfl_xor(vmin, o, 8));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, o, 8));
+            dmin = fminf(dmin, __shfl_xor(dmin, o, 8));
+        }
+        if (c == 0) {
+            BoxGeom g;
+            const bool finite = umin > -1e6f && umax < 1e6f && vmin > -1e6f && vmax < 1e6f;
+            g.i_lo = finite ? (int) floorf(umin - 0.5f) - 1 : 0;
+            g.j_lo = finite ? (int) floorf(vmin - 0.5f) - 1 : 0;
+            g.w = finite ? (int) floorf(umax - 0.5f) - g.i_lo + 2 : 0;
+            g.h = finite ? (int) floorf(vmax - 0.5f) - g.j_lo + 2 : 0;
+            if (!live || !(dmin > 0.0f) || g.w > kBoxW || g.h > kBoxH || g.w <= 0 || g.h <= 0)
+                g.w = 0;
+            bgs[buf][tid >> 3] = g;
+        }
+    };
+
+    // Staging registers: texel column tid % 64, rows tid / 64 + 4 k.
+    const int sc = tid & 63, sr0 = tid >> 6;
+    float st[kRowsPerThread];
+    const long plane = (long) W * H;
+    auto stage_load = [&](int p, const BoxGeom &g) {
+        const float *img = proj + (long) p * plane;
+        int i = g.i_lo + sc;
+        bool ci = sc < g.w;
+        if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
+            ci = ci && i >= 0 && i < W;
+        } else {
+            i = min(max(i, 0), W - 1);
+        }
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int k = 0; k < kRowsPerThread; k++) {
+            const int r = sr0 + 4 * k;
+            int j = g.j_lo + r;
+            bool ok = ci && r < g.h;
+            if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
+                ok = ok && j >= 0 && j < H;
+            } else {
+                j = min(max(j, 0), H - 1);
+            }
+            st[k] = ok ? img[(long) j * W + i] : 0.0f;
+        }
+    };
+    auto commit = [&](int buf, const BoxGeom &g) {
 #pragma unroll
-            for (int k = 0; k < kTZ; k++)
+        for (int k = 0; k < kRowsPerThread; k++) {
+            const int r = sr0 + 4 * k;
+            if (r < g.h) box[buf][r * kBoxW + sc] = st[k];
+        }
+    };
+
+    float acc[KA][KB][TZ];
+#pragma unroll
+    for (int a = 0; a < KA; a++)
+#pragma unroll
+        for (int b = 0; b < KB; b++)
+#pragma unroll
+            for (int k = 0; k < TZ; k++)
                 acc[a][b][k] = 0.0f;
 
-    const long plane = (long) W * H;
-    for (int p = 0; p < P; p++) {
-        const float *img = proj + (long) p * plane;
-        const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1], md = mats[4 * p + 2], mc = mats[4 * p + 3];
-        // A. footprint of the tile (wave 0)
-        if (tid < 64) {
-            const int c = tid & 7;
-            float u, v, d;
-            map_point<CONE>(mu, mv, md, (c & 1) ? cxb : cxa, (c & 2) ? cyb : cya, (c & 4) ? czb : cza, u, v, d);
-            float umin = u, umax = u, vmin = v, vmax = v, dmin = d;
-#pragma unroll
-            for (int o = 1; o < 8; o <<= 1) {
-                umin = fminf(umin, __shfl_xor(umin, o, 8));
-                umax = fmaxf(umax, __shfl_xor(umax, o, 8));
-                vmin = fminf(vmin, __shfl_xor(vmin, o, 8));
-                vmax = fmaxf(vmax, __shfl_xor(vmax, o, 8));
-                dmin = fminf(dmin, __shfl_xor(dmin, o, 8));
-            }
-            if (tid == 0) {
-                BoxGeom g;
-                const bool finite = umin > -1e6f && umax < 1e6f && vmin > -1e6f && vmax < 1e6f;
-                g.i_lo = finite ? (int) floorf(umin - 0.5f) - 1 : 0;
-                g.j_lo = finite ? (int) floorf(vmin - 0.5f) - 1 : 0;
-                g.w = finite ? (int) floorf(umax - 0.5f) - g.i_lo + 2 : 0;
-                g.h = finite ? (int) floorf(vmax - 0.5f) - g.j_lo + 2 : 0;
-                if (!(dmin > 0.0f) || g.w > kBoxPitch || g.h > kBoxRows || g.w <= 0 || g.h <= 0)
-                    g.w = 0;
-                bg = g;
-            }
+    footprints(0, 0);
+    __syncthreads();
+    {
+        const BoxGeom g0 = bgs[0][0];
+        if (g0.w > 0) {
+            stage_load(0, g0);
+            commit(0, g0);
         }
-        __syncthreads();  // box geometry ready; previous projection's samples done
-        const BoxGeom g = bg;
+    }
+    __syncthreads();
+    for (int p = 0; p < P; p++) {
+        if (p % kChunk == 0 && p + kChunk < P + kChunk - 1)
+            footprints(p + kChunk, ((p / kChunk) + 1) & 1);  // read a chunk later
+        const BoxGeom g = bgs[(p / kChunk) & 1][p % kChunk];
+        const bool next = p + 1 < P;
+        BoxGeom gn;
+        gn.w = 0;
+        if (next) {
+            gn = bgs[((p + 1) / kChunk) & 1][(p + 1) % kChunk];
+            if (gn.w > 0) stage_load(p + 1, gn);
+        }
+        const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1], md = mats[4 * p + 2], mc = mats[4 * p + 3];
         if (g.w > 0) {
-            // B. stage the footprint
-            const int c = tid & 63;
-            if (c < g.w) {
-                const int i = g.i_lo + c;
-                int i0 = i, i1 = i + 1;
-                bool ci0 = true, ci1 = true;
-                if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
-                    ci0 = i0 >= 0 && i0 < W;
-                    ci1 = i1 >= 0 && i1 < W;
-                } else {
-                    i0 = min(max(i0, 0), W - 1);
-                    i1 = min(max(i1, 0), W - 1);
-                }
-                for (int r = tid >> 6; r < g.h; r += 4) {
-                    const int j = g.j_lo + r;
-                    int j0 = j, j1 = j + 1;
-                    bool cj0 = true, cj1 = true;
-                    if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
-                        cj0 = j0 >= 0 && j0 < H;
-                        cj1 = j1 >= 0 && j1 < H;
-                    } else {
-                        j0 = min(max(j0, 0), H - 1);
-                        j1 = min(max(j1, 0), H - 1);
-                    }
-                    const float *p0 = img + (long) j0 * W, *p1 = img + (long) j1 * W;
-                    const float t00 = (cj0 && ci0) ? p0[i0] : 0.0f;
-                    const float t10 = (cj0 && ci1) ? p0[i1] : 0.0f;
-                    const float t01 = (cj1 && ci0) ? p1[i0] : 0.0f;
-                    const float t11 = (cj1 && ci1) ? p1[i1] : 0.0f;
-                    box[r * kBoxPitch + c] = make_float4(t00, t10 - t00, t01 - t00, (t11 - t10) - (t01 - t00));
-                }
-            }
-            __syncthreads();
-            // C. sample
+            const float *bx = box[p & 1];
             const float ilo = (float) g.i_lo + 0.5f, jlo = (float) g.j_lo + 0.5f;
 #pragma unroll
-            for (int a = 0; a < 2; a++) {
+            for (int a = 0; a < KA; a++) {
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
+                for (int b = 0; b < KB; b++) {
                     const float x = xv[a], y = yv[b];
                     const float ub = fmaf(mu.x, x, fmaf(mu.y, y, mu.w));
                     const float vb = fmaf(mv.x, x, fmaf(mv.y, y, mv.w));
@@ -363,11 +395,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
                         const float fa = __builtin_amdgcn_fractf(uu);
                         const float dv = mv.z * rd, v0 = fmaf(vb, rd, -jlo);
 #pragma unroll
-                        for (int k = 0; k < kTZ; k++) {
+                        for (int k = 0; k < TZ; k++) {
                             const float vv = fmaf(dv, z[k], v0);
-                            const float4 q = box[(int) vv * kBoxPitch + ci];
+                            const float *t = bx + (int) vv * kBoxW + ci;
                             const float fb = __builtin_amdgcn_fractf(vv);
-                            const float val = fmaf(fb, fmaf(fa, q.w, q.z), fmaf(fa, q.y, q.x));
+                            const float top = fmaf(fa, t[1] - t[0], t[0]);
+                            const float bot = fmaf(fa, t[kBoxW + 1] - t[kBoxW], t[kBoxW]);
+                            const float val = fmaf(fb, bot - top, top);
                             if constexpr (CONE)
                                 acc[a][b][k] = fmaf(val, w, acc[a][b][k]);
                             else
@@ -377,14 +411,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
                         const float db = fmaf(md.x, x, fmaf(md.y, y, md.w));
                         const float cb = fmaf(mc.x, x, fmaf(mc.y, y, mc.w));
 #pragma unroll
-                        for (int k = 0; k < kTZ; k++) {
+                        for (int k = 0; k < TZ; k++) {
                             const float rd = __builtin_amdgcn_rcpf(fmaf(md.z, z[k], db));
                             const float cf = __builtin_amdgcn_rcpf(fmaf(mc.z, z[k], cb));
                             const float uu = fmaf(fmaf(mu.z, z[k], ub), rd, -ilo);
                             const float vv = fmaf(fmaf(mv.z, z[k], vb), rd, -jlo);
-                            const float4 q = box[(int) vv * kBoxPitch + (int) uu];
+                            const float *t = bx + (int) vv * kBoxW + (int) uu;
                             const float fa = __builtin_amdgcn_fractf(uu), fb = __builtin_amdgcn_fractf(vv);
-                            const float val = fmaf(fb, fmaf(fa, q.w, q.z), fmaf(fa, q.y, q.x));
+                            const float top = fmaf(fa, t[1] - t[0], t[0]);
+                            const float bot = fmaf(fa, t[kBoxW + 1] - t[kBoxW], t[kBoxW]);
+                            const float val = fmaf(fb, bot - top, top);
                             acc[a][b][k] = fmaf(val, cf * cf, acc[a][b][k]);
                         }
                     }
@@ -395,10 +431,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
             // behind the source).  One voxel per iteration of a rolled loop and
             // a select chain into the accumulators, so this rare path does not
             // inflate the register budget of the LDS path.
+            const float *img = proj + (long) p * plane;
 #pragma unroll 1
-            for (int q = 0; q < 4 * kTZ; q++) {
-                const int a = q / (2 * kTZ), b = (q / kTZ) & 1, k = q % kTZ;
-                const float x = a ? xv[1] : xv[0], y = b ? yv[1] : yv[0];
+            for (int q = 0; q < KA * KB * TZ; q++) {
+                const int a = q / (KB * TZ), b = (q / TZ) % KB, k = q % TZ;
+                const float x = a ? xv[KA - 1] : xv[0], y = b ? yv[KB - 1] : yv[0];
                 const float zz = zs[min(Z0 + k, nz - 1)];
                 float u, v, d;
                 map_point<CONE>(mu, mv, md, x, y, zz, u, v, d);
@@ -410,26 +447,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
                     val *= cf * cf;
                 }
 #pragma unroll
-                for (int aa = 0; aa < 2; aa++)
+                for (int aa = 0; aa < KA; aa++)
 #pragma unroll
-                    for (int bb = 0; bb < 2; bb++)
+                    for (int bb = 0; bb < KB; bb++)
 #pragma unroll
-                        for (int kk = 0; kk < kTZ; kk++)
-                            acc[aa][bb][kk] += (q == (aa * 2 + bb) * kTZ + kk) ? val : 0.0f;
+                        for (int kk = 0; kk < TZ; kk++)
+                            acc[aa][bb][kk] += (q == (aa * KB + bb) * TZ + kk) ? val : 0.0f;
             }
         }
-        __syncthreads();  // all samples of this projection done before the next staging
+        if (next && gn.w > 0) commit((p + 1) & 1, gn);
+        __syncthreads();  // box p + 1 staged; box p free for p + 2
     }
 
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
+    for (int a = 0; a < KA; a++) {
         const int ix = X0 + lx + 16 * a;
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
+        for (int b = 0; b < KB; b++) {
             const int iy = Y0 + ly + 16 * b;
             if (ix < nx && iy < ny) {
 #pragma unroll
-                for (int k = 0; k < kTZ; k++) {
+                for (int k = 0; k < TZ; k++) {
                     const int iz = Z0 + k;
                     if (iz < nz)
                         store_voxel(vol, ((long) iz * ny + iy) * nx + ix, norm * acc[a][b][k], gmin, gscale, gmax);
@@ -499,7 +537,9 @@ int gbp_launch(ufomi_ctx *ctx, const float *d_proj, int W, int H, unsigned n, co
     for (size_t i = 0; i < n && zsep; i++)
         zsep = mats[16 * i + 2] == 0.0f && (f.parallel || (mats[16 * i + 10] == 0.0f && mats[16 * i + 14] == 0.0f));
     const int mode = f.parallel ? (zsep ? MODE_PARALLEL_ZSEP : MODE_PARALLEL) : (zsep ? MODE_CONE_ZSEP : MODE_CONE);
-    const int tiles_x = (nx + kTX - 1) / kTX, tiles_y = (ny + kTY - 1) / kTY, tiles_z = (nz + kTZ - 1) / kTZ;
+    const bool cone_tile = !f.parallel;
+    const int TXl = cone_tile ? 16 : 32, TZl = cone_tile ? 16 : 8;
+    const int tiles_x = (nx + TXl - 1) / TXl, tiles_y = (ny + TXl - 1) / TXl, tiles_z = (nz + TZl - 1) / TZl;
     const long n_tiles = (long) tiles_x * tiles_y * tiles_z;
     if (n_tiles > 0x7fffffffL)
         return ufomi_fail(UFOMI_ERR_ARG, "volume too large for one launch");

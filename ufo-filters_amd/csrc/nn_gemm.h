@@ -416,3 +416,113 @@ struct UpWgrad {
 };
 
 }  // namespace nn
+
+namespace nn {
+
+// 3x3 convolution (the Conv3 operation) with the input staged in LDS: a
+// workgroup owns TY = 4 rows x 64 columns of one image and BN output channels;
+// per 32-channel chunk the (TY + 2) x 66 pixel patch is staged once and all 9
+// taps read their A fragments from it (9x reuse), with the next chunk's patch
+// prefetched into registers.  Wave w computes row w of the tile (4 x NT MFMA
+// tiles).  Needs W >= 64 (a multiple of 64), H a multiple of 4, cin a
+// multiple of 32.
+template <int NT>
+__global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
+{
+    constexpr int TX = 64, TY = 4, PX = TX + 2, PY = TY + 2, MT = 4, BN = NT * 16;
+    constexpr int NVEC = PY * PX * 4;                   // 16-byte vectors per patch (32 channels)
+    constexpr int PER = (NVEC + 255) / 256;
+    __shared__ bf16x8 patch[PY * PX * 4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int W = 1 << op.lw, H = 1 << op.lh;
+    const int tiles_x = W / TX, tiles_y = H / TY;
+    int b = blockIdx.x;
+    const int tx = b % tiles_x;
+    b /= tiles_x;
+    const int ty = b % tiles_y;
+    const long img = b / tiles_y;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const long n0 = (long) blockIdx.y * BN;
+    const long pix0 = (img << (op.lh + op.lw)) + ((long) y0 << op.lw) + x0;  // pixel (y0, x0)
+    const int cin = 1 << op.lcin;
+
+    // staging: vector v of the patch = (row, col, channel group)
+    bf16x8 pre[PER];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int v = threadIdx.x + 256 * i;
+            bf16x8 val = zero8();
+            if (v < NVEC) {
+                const int g = v & 3, pc = (v >> 2) % PX, pr = (v >> 2) / PX;
+                const int yy = y0 + pr - 1, xx = x0 + pc - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    val = ld8(op.in.at(pix0 + (long) (pr - 1) * W + (pc - 1), c0 + 8 * g));
+            }
+            pre[i] = val;
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int v = threadIdx.x + 256 * i;
+            if (v < NVEC) patch[v] = pre[i];
+        }
+    };
+
+    const bf16 *wrow[NT];
+    bool wok[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const long co = n0 + j * 16 + r;
+        wok[j] = co < op.cout;
+        wrow[j] = op.w + (wok[j] ? co : 0) * (9L << op.lcin);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    const int nchunks = cin / 32;
+    fetch(0);
+    commit();
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int c0 = ch * 32;
+        if (ch + 1 < nchunks) fetch(c0 + 32);
+#pragma unroll 1
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;  // patch offsets (0..2)
+            bf16x8 bfr[NT];
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+                bfr[j] = wok[j] ? ld8(wrow[j] + ((long) tap << op.lcin) + c0 + 8 * q) : zero8();
+            bf16x8 afr[MT];
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+                afr[i] = patch[((wave + dy) * PX + (i * 16 + r + dx)) * 4 + q];
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (ch + 1 < nchunks) {
+            commit();
+            __syncthreads();
+        }
+    }
+    // D[4q + e][r] of tile (i, j): pixel (y0 + wave, x0 + 16 i + 4 q + e), channel n0 + 16 j + r
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                op.store(pix0 + ((long) wave << op.lw) + i * 16 + 4 * q + e, n0 + j * 16 + r, acc[i][j][e], 0);
+}
+
+}  // namespace nn

@@ -449,6 +449,21 @@ int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, i
     op.mask = mask;
     op.mask_from = mask_from;
     op.out = out;
+    const int lh = n->lh - l, lw = n->lw - l;
+    if (lw >= 6 && lh >= 2) {
+        // LDS-staged input patches (4 x 64 pixels per workgroup)
+        const long tiles = npix >> 8;
+        const int NT = cout <= 32 ? 2 : 4;
+        const long gy = (cout + 16 * NT - 1) / (16 * NT);
+        if (tiles > 0x7fffffffL || gy > 65535)
+            return ufomi_fail(UFOMI_ERR_SIZE, "conv grid too large");
+        if (NT == 2)
+            hipLaunchKernelGGL(nn::conv3_lds_kernel<2>, dim3((unsigned) tiles, (unsigned) gy), dim3(256), 0, s, op);
+        else
+            hipLaunchKernelGGL(nn::conv3_lds_kernel<4>, dim3((unsigned) tiles, (unsigned) gy), dim3(256), 0, s, op);
+        UFOMI_CHECK_LAUNCH();
+        return UFOMI_OK;
+    }
     return gemm_fwd(s, op, npix, cout);
 }
 
