@@ -58,13 +58,19 @@ typedef struct {
 
 /* UfoTaskMode bits */
 enum { UFOMI_TASK_MODE_PROCESSOR = 1 << 0, UFOMI_TASK_MODE_REDUCTOR = 1 << 1, UFOMI_TASK_MODE_GENERATOR = 1 << 2,
-       UFOMI_TASK_MODE_GPU = 1 << 4 };
+       UFOMI_TASK_MODE_SINK = 1 << 3, UFOMI_TASK_MODE_GPU = 1 << 4, UFOMI_TASK_MODE_CPU = 1 << 5 };
 
 typedef struct UfomiTask UfomiTask;
 
 /* Task names: "fft", "filter", "ifft", "backproject", "general-backproject",
- * "cone-beam-projection-weight" (libufofilter<name>.so in the reference).
- * Returns NULL for an unknown name. */
+ * "cone-beam-projection-weight" (libufofilter<name>.so in the reference), and
+ * the file tasks "read" (src/ufo-read-task.c: a generator, TIFF / BigTIFF,
+ * EDF and raw files) and "write" (src/ufo-write-task.c: a sink, TIFF / BigTIFF
+ * and raw).  Returns NULL for an unknown name.
+ *
+ * Generators take `inputs` NULL in get_requisition; a generator with nothing
+ * left leaves the requisition as it was and generate returns 0.  Sinks take
+ * `output` NULL in process. */
 UfomiTask *ufomi_task_new(const char *name);
 void ufomi_task_free(UfomiTask *task);
 const char *ufomi_task_name(const UfomiTask *task);
@@ -78,7 +84,8 @@ int ufomi_task_get_property(const UfomiTask *task, const char *name, char *buf, 
 /* Names of all properties, NULL terminated. */
 const char *const *ufomi_task_list_properties(const UfomiTask *task);
 
-/* UfoTaskIface */
+/* UfoTaskIface.  Tasks of UFOMI_TASK_MODE_CPU (read, write) may be set up
+ * with a NULL context when they only see host memory (the *_host calls). */
 int ufomi_task_setup(UfomiTask *task, ufomi_ctx *ctx, UfomiError *error);
 int ufomi_task_get_requisition(UfomiTask *task, UfomiBuffer **inputs, UfomiRequisition *requisition,
                                UfomiError *error);
@@ -88,6 +95,12 @@ unsigned ufomi_task_get_mode(const UfomiTask *task);
 /* Returns 1 (TRUE) like the reference's process/generate, 0 on failure or end. */
 int ufomi_task_process(UfomiTask *task, UfomiBuffer **inputs, UfomiBuffer *output, UfomiRequisition *requisition);
 int ufomi_task_generate(UfomiTask *task, UfomiBuffer *output, UfomiRequisition *requisition);
+/* CPU-mode tasks on host float32 memory, as ufo-core hands a CPU task its
+ * buffers' host arrays (ufo_buffer_get_host_array): read's generate
+ * (src/ufo-read-task.c:331-353) into `output` of `requisition`, write's
+ * process (src/ufo-write-task.c:310-392) of `input`.  Return 1 / 0 as above. */
+int ufomi_task_generate_host(UfomiTask *task, float *output, const UfomiRequisition *requisition);
+int ufomi_task_process_host(UfomiTask *task, const float *input, const UfomiRequisition *input_requisition);
 /* The output layout and storage a task writes (ufo_buffer_set_layout). */
 UfomiBufferLayout ufomi_task_output_layout(const UfomiTask *task);
 int ufomi_task_output_dtype(const UfomiTask *task);
@@ -109,6 +122,11 @@ size_t ufomi_requisition_elements(const UfomiRequisition *req);
  * backprojection consumes all items (projections) and emits its slices.
  * The output items are written contiguously to d_out (at most out_capacity
  * bytes); *n_out and *out_req describe them.
+ *
+ * A first stage `read` produces the input items from files (d_in, in_req and
+ * n_items are then ignored and may be NULL / 0); a last stage `write` writes
+ * the output items to files (d_out may then be NULL; *n_out and *out_req
+ * still describe what was written).
  */
 typedef struct UfomiPipeline UfomiPipeline;
 UfomiPipeline *ufomi_pipeline_new(const char *description, UfomiError *error);

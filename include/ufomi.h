@@ -34,7 +34,8 @@ typedef enum {
     UFOMI_ERR_MEMORY = -9,       /* volume does not fit (ufo-general-backproject-task.c:1542-1546) */
     UFOMI_ERR_NUM_PROJECTIONS = -10, /* general-backproject without num-projections (:1429-1433) */
     UFOMI_ERR_LAYOUT = -11,      /* ifft input is not complex (ufo-ifft-task.c:121-125) */
-    UFOMI_ERR_INCOMPLETE = -12   /* fewer projections than declared (ufo-general-backproject-task.c:1733-1738) */
+    UFOMI_ERR_INCOMPLETE = -12,  /* fewer projections than declared (ufo-general-backproject-task.c:1733-1738) */
+    UFOMI_ERR_IO = -13           /* a file of the read / write tasks could not be opened, parsed, read or written */
 } ufomi_status;
 
 /* Storage type of a filtered sinogram / projection handed to a backprojector. */

@@ -10,7 +10,10 @@
  *   - `backproject` runs all items as one multi-slice launch;
  *   - other processors run item by item;
  *   - `general-backproject` takes every item as a projection and emits its
- *     slices through generate.
+ *     slices through generate;
+ *   - a first stage `read` loads every image it generates into one host
+ *     buffer and uploads it in one copy; a last stage `write` downloads the
+ *     final batch in one copy and writes it item by item.
  */
 #include "tasks_internal.h"
 
@@ -135,12 +138,112 @@ static size_t rows_of(const UfomiRequisition *r)
     return (r->n_dims >= 2 ? r->dims[1] : 1) * (r->n_dims == 3 ? r->dims[2] : 1);
 }
 
+/* Every image of a `read' stage as one device batch (items of equal size). */
+static int read_batch(ufomi_ctx *ctx, UfomiTask *t, Batch *out, UfomiError *e)
+{
+    float *host = NULL;
+    size_t cap = 0, elems = 0;
+    UfomiRequisition item;
+    memset(&item, 0, sizeof(item));
+    memset(out, 0, sizeof(*out));
+    int n = 0, rc = UFOMI_OK;
+    for (;;) {
+        UfomiRequisition req;
+        memset(&req, 0, sizeof(req));
+        if ((rc = ufomi_task_get_requisition(t, NULL, &req, e)))
+            goto done;
+        if (!req.n_dims)
+            break;
+        if (!n) {
+            item = req;
+            elems = ufomi_requisition_elements(&req);
+        } else if (req.dims[0] != item.dims[0] || req.dims[1] != item.dims[1]) {
+            rc = task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG,
+                            "read: image %d is %zux%zu but the first was %zux%zu; a batch needs one size", n,
+                            req.dims[0], req.dims[1], item.dims[0], item.dims[1]);
+            goto done;
+        }
+        if ((size_t) (n + 1) * elems > cap) {
+            const size_t want = cap * 2 > (size_t) (n + 1) * elems ? cap * 2 : (size_t) (n + 1) * elems;
+            float *grown = realloc(host, want * sizeof(float));
+            if (!grown) {
+                rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_MEMORY, "read: out of host memory");
+                goto done;
+            }
+            host = grown;
+            cap = want;
+        }
+        t->err_status = 0;
+        if (!read_task_generate_host(t, host + (size_t) n * elems, &req)) {
+            if (t->err_status) {
+                rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, t->err_status, "%s", ufomi_task_last_error(t));
+                goto done;
+            }
+            break;
+        }
+        t->num_processed++;
+        n++;
+    }
+    if (n) {
+        if ((rc = batch_alloc(ctx, out, &item, n, sizeof(float), UFOMI_F32, UFOMI_LAYOUT_REAL))) {
+            rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_MEMORY, "out of device memory: %s",
+                            ufomi_last_error());
+            goto done;
+        }
+        if ((rc = ufomi_memcpy(ctx, out->buf.data, host, out->item_bytes * (size_t) n, 0))) {
+            ufomi_buffer_release(ctx, &out->buf);
+            task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "read: %s", ufomi_last_error());
+        }
+    }
+done:
+    free(host);
+    return rc;
+}
+
+/* Every item of the final batch through a `write' stage. */
+static int write_batch(ufomi_ctx *ctx, UfomiTask *t, const Batch *cur, UfomiError *e)
+{
+    if (cur->buf.dtype != UFOMI_F32)
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_UNSUPPORTED, "write: input items must be float32");
+    const size_t total = cur->item_bytes * (size_t) cur->n;
+    float *host = malloc(total ? total : 1);
+    if (!host)
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_MEMORY, "write: out of host memory");
+    int rc = total ? ufomi_memcpy(ctx, host, cur->buf.data, total, 1) : UFOMI_OK;
+    if (rc) {
+        task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "write: %s", ufomi_last_error());
+    } else {
+        for (int i = 0; i < cur->n; i++) {
+            if ((rc = write_task_write_host(t, (const float *) ((const char *) host + (size_t) i * cur->item_bytes),
+                                            &cur->buf.req))) {
+                task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "%s", ufomi_task_last_error(t));
+                break;
+            }
+            t->num_processed++;
+        }
+    }
+    write_task_flush(t);
+    free(host);
+    return rc;
+}
+
 int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const UfomiRequisition *in_req,
                        int n_items, void *d_out, size_t out_capacity, int *n_out, UfomiRequisition *out_req,
                        UfomiError *e)
 {
-    if (!pl || !ctx || !d_in || !in_req || n_items <= 0 || !d_out || !n_out || !out_req)
+    if (!pl || !ctx || !n_out || !out_req)
         return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "invalid pipeline arguments");
+    const int has_read = is(pl->tasks[0], "read"), has_write = is(pl->tasks[pl->n - 1], "write");
+    const int first = has_read, last = pl->n - has_write;
+    if ((!has_read && (!d_in || !in_req || n_items <= 0)) || (!has_write && !d_out))
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "invalid pipeline arguments");
+    for (int i = first; i < last; i++)
+        if (is(pl->tasks[i], "read") || is(pl->tasks[i], "write"))
+            return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG,
+                              "`%s' must be the %s stage of a pipeline", ufomi_task_name(pl->tasks[i]),
+                              is(pl->tasks[i], "read") ? "first" : "last");
+    if (has_read && pl->n == 1)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "a pipeline of `read' alone has no sink");
     int rc;
     for (int i = 0; i < pl->n; i++)
         if ((rc = ufomi_task_setup(pl->tasks[i], ctx, e)))
@@ -148,22 +251,35 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
     pl->fused = 0;
 
     Batch cur;
-    memset(&cur, 0, sizeof(cur));
-    cur.buf.data = (void *) d_in;
-    cur.buf.req = *in_req;
-    cur.buf.layout = UFOMI_LAYOUT_REAL;
-    cur.buf.dtype = UFOMI_F32;
-    cur.n = n_items;
-    cur.item_bytes = ufomi_requisition_elements(in_req) * sizeof(float);
     int cur_owned = 0;
+    if (has_read) {
+        if ((rc = read_batch(ctx, pl->tasks[0], &cur, e)))
+            return rc;
+        cur_owned = cur.n > 0;
+        if (!cur.n) {  /* nothing matched the image range: an empty stream */
+            if (has_write)
+                write_task_flush(pl->tasks[pl->n - 1]);
+            *n_out = 0;
+            memset(out_req, 0, sizeof(*out_req));
+            return UFOMI_OK;
+        }
+    } else {
+        memset(&cur, 0, sizeof(cur));
+        cur.buf.data = (void *) d_in;
+        cur.buf.req = *in_req;
+        cur.buf.layout = UFOMI_LAYOUT_REAL;
+        cur.buf.dtype = UFOMI_F32;
+        cur.n = n_items;
+        cur.item_bytes = ufomi_requisition_elements(in_req) * sizeof(float);
+    }
 
-    for (int s = 0; s < pl->n; s++) {
+    for (int s = first; s < last; s++) {
         UfomiTask *t = pl->tasks[s];
         Batch next;
         UfomiBuffer in0 = item_view(&cur, 0), *ins[1] = {&in0};
         UfomiRequisition req;
 
-        if (is(t, "fft") && s + 2 < pl->n && is(pl->tasks[s + 1], "filter") && is(pl->tasks[s + 2], "ifft")) {
+        if (is(t, "fft") && s + 2 < last && is(pl->tasks[s + 1], "filter") && is(pl->tasks[s + 2], "ifft")) {
             /* fused fft ! filter ! ifft: requisitions of the three tasks, one launch */
             UfomiRequisition r_fft, r_filt, r_ifft;
             UfomiBuffer spec = in0, filt;
@@ -278,7 +394,13 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
         goto out;
     }
 
-    {
+    if (has_write) {
+        if ((rc = write_batch(ctx, pl->tasks[pl->n - 1], &cur, e)))
+            goto out;
+        *n_out = cur.n;
+        *out_req = cur.buf.req;
+        rc = UFOMI_OK;
+    } else {
         const size_t total = cur.item_bytes * (size_t) cur.n;
         if (total > out_capacity) {
             rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_SIZE, "output needs %zu bytes, capacity %zu", total,

@@ -9,8 +9,16 @@
  *   backproject                  src/ufo-backproject-task.c
  *   cone-beam-projection-weight  src/ufo-cone-beam-projection-weight-task.c
  *   general-backproject          src/ufo-general-backproject-task.c
+ *   read                         src/ufo-read-task.c
+ *   write                        src/ufo-write-task.c
  */
 #include "tasks_internal.h"
+#include "io.h"
+
+#include <glob.h>
+#include <libgen.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <ctype.h>
 #include <errno.h>
@@ -164,7 +172,9 @@ static void prop_reset(UfomiTask *t)
         const PropSpec *s = &t->cls->props[i];
         PropValue *v = &t->v[i];
         free(v->arr);
+        free(v->str);
         v->arr = NULL;
+        v->str = s->type == PT_STRING ? strdup(s->sdflt ? s->sdflt : "") : NULL;
         v->n = 0;
         v->num = s->dflt;
         if (s->type == PT_DARRAY && s->n_default) {
@@ -215,6 +225,8 @@ static int parse_double(const char *s, double *out)
     return *end == '\0';
 }
 
+static int write_bits_property(UfomiTask *t, double bits);
+
 int ufomi_task_set_property(UfomiTask *t, const char *name, const char *value)
 {
     if (!t || !name || !value)
@@ -253,7 +265,13 @@ int ufomi_task_set_property(UfomiTask *t, const char *name, const char *value)
             d = (float) d;
         if (!(d >= s->min && d <= s->max))
             return ufomi_status_fail(UFOMI_ERR_ARG, "value %s of %s is outside [%g, %g]", value, name, s->min, s->max);
+        if (!strcmp(t->cls->name, "write") && !strcmp(s->name, "bits") && !write_bits_property(t, d))
+            return UFOMI_OK;
         v->num = d;
+        return UFOMI_OK;
+    case PT_STRING:
+        free(v->str);
+        v->str = strdup(value);
         return UFOMI_OK;
     case PT_DARRAY: {
         double vals[65536];
@@ -306,6 +324,9 @@ int ufomi_task_get_property(const UfomiTask *t, const char *name, char *buf, siz
     case PT_DOUBLE:
     case PT_FLOAT:
         snprintf(buf, len, "%.17g", v->num);
+        break;
+    case PT_STRING:
+        snprintf(buf, len, "%s", v->str ? v->str : "");
         break;
     case PT_DARRAY: {
         size_t off = 0;
@@ -765,6 +786,455 @@ static void gbp_finalize(UfomiTask *t)
     }
 }
 
+/* ------------------------------------------------------------------- read */
+
+/* read (ufo-read-task.c:551-663; init :668-690 sets path "."; type_values :53-63
+ * without hdf5, which needs libhdf5) */
+#define ULMAX 18446744073709551615.0
+static const char *const read_type_nicks[] = {"edf", "raw", "tiff", "unspecified", NULL};
+static const PropSpec read_props[] = {
+    {"path", PT_STRING, 0, 0, 0, NULL, 0, "."},
+    {"start", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"image-start", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"number", PT_UINT, 0, U32MAX, U32MAX, NULL, 0, NULL},
+    {"step", PT_UINT, 1, U32MAX, 1, NULL, 0, NULL},
+    {"y", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"height", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"y-step", PT_UINT, 1, U32MAX, 1, NULL, 0, NULL},
+    {"image-step", PT_UINT, 1, U32MAX, 1, NULL, 0, NULL},
+    {"convert", PT_BOOL, 0, 1, 1, NULL, 0, NULL},
+    {"raw-width", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"raw-height", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"raw-bitdepth", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"raw-pre-offset", PT_UINT, 0, ULMAX, 0, NULL, 0, NULL},
+    {"raw-post-offset", PT_UINT, 0, ULMAX, 0, NULL, 0, NULL},
+    {"type", PT_ENUM, 0, 3, 3, read_type_nicks, 0, NULL},
+};
+enum { RD_PATH, RD_START, RD_IMAGE_START, RD_NUMBER, RD_STEP, RD_Y, RD_HEIGHT, RD_YSTEP, RD_IMAGE_STEP, RD_CONVERT,
+       RD_RAW_W, RD_RAW_H, RD_RAW_BITS, RD_RAW_PRE, RD_RAW_POST, RD_TYPE };
+enum { RT_EDF, RT_RAW, RT_TIFF, RT_UNSPECIFIED };
+
+typedef struct {
+    char **files;
+    size_t n_files, cur;
+    int single, open, done;
+    ImgFile f;
+    unsigned image_start;   /* images still to skip when the next file opens */
+    unsigned roi_height;    /* `height', resolved against the first file */
+    unsigned index;         /* next image of the open file */
+    unsigned current;       /* images generated */
+    float *stage;
+    size_t stage_cap;
+} ReadPriv;
+
+static unsigned uprop(const UfomiTask *t, int i) { return (unsigned) t->v[i].num; }
+
+/* get_reader (ufo-read-task.c:213-233): the extension or `type', TIFF first, then EDF, then raw */
+static int read_format(const UfomiTask *t, const char *name)
+{
+    const int ext = img_format_of(name), type = (int) t->v[RD_TYPE].num;
+    if (ext == IMG_TIFF || type == RT_TIFF)
+        return IMG_TIFF;
+    if (ext == IMG_EDF || type == RT_EDF)
+        return IMG_EDF;
+    if (ext == IMG_RAW || type == RT_RAW)
+        return IMG_RAW;
+    return IMG_NONE;
+}
+
+static void read_reset(ReadPriv *p)
+{
+    if (p->open)
+        img_close(&p->f);
+    p->open = 0;
+    for (size_t i = 0; i < p->n_files; i++)
+        free(p->files[i]);
+    free(p->files);
+    p->files = NULL;
+    p->n_files = 0;
+}
+
+static int cmp_names(const void *a, const void *b) { return strcmp(*(char *const *) a, *(char *const *) b); }
+
+/* read_filenames + setup (ufo-read-task.c:139-211): a regular file, or a glob
+ * (a directory means every file in it), filtered by reader, sorted */
+static int read_setup(UfomiTask *t, UfomiError *e)
+{
+    ReadPriv *p = t->priv;
+    read_reset(p);
+    const char *path = t->v[RD_PATH].str;
+    struct stat st;
+    p->single = stat(path, &st) == 0 && S_ISREG(st.st_mode);
+    const size_t len = strlen(path);
+    char *pattern = malloc(len + 3);
+    if (p->single || strchr(path, '*'))
+        memcpy(pattern, path, len + 1);
+    else
+        snprintf(pattern, len + 3, "%s%s*", path, len && path[len - 1] == '/' ? "" : "/");
+    glob_t g;
+    memset(&g, 0, sizeof(g));
+    if (glob(pattern, GLOB_MARK | GLOB_TILDE, NULL, &g) == 0) {
+        p->files = calloc(g.gl_pathc + 1, sizeof(char *));
+        for (size_t i = 0; i < g.gl_pathc; i++)
+            if (read_format(t, g.gl_pathv[i]) != IMG_NONE)
+                p->files[p->n_files++] = strdup(g.gl_pathv[i]);
+    }
+    globfree(&g);
+    free(pattern);
+    if (!p->n_files)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "`%s' does not match any files", path);
+    qsort(p->files, p->n_files, sizeof(char *), cmp_names);
+    p->cur = p->single ? 0 : uprop(t, RD_START);
+    if (p->cur >= p->n_files)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "start=%u skips too many files",
+                          uprop(t, RD_START));
+    p->image_start = uprop(t, RD_IMAGE_START);
+    p->roi_height = uprop(t, RD_HEIGHT);
+    p->index = p->current = 0;
+    p->done = 0;
+    return UFOMI_OK;
+}
+
+/* get_requisition (ufo-read-task.c:235-309): the next file with an image at
+ * the running image-start, the vertical ROI checked against it */
+static int read_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    (void) in;
+    ReadPriv *p = t->priv;
+    const unsigned y = uprop(t, RD_Y);
+    if (!p->open || p->index >= p->f.n_images) {
+        for (;;) {
+            if (p->open) {
+                img_close(&p->f);
+                p->open = 0;
+                p->cur += uprop(t, RD_STEP);
+            }
+            if (p->cur >= p->n_files) {
+                p->done = 1;
+                return UFOMI_OK;
+            }
+            const char *name = p->files[p->cur];
+            const ImgRawSpec raw = {uprop(t, RD_RAW_W), uprop(t, RD_RAW_H), uprop(t, RD_RAW_BITS),
+                                    (unsigned long) t->v[RD_RAW_PRE].num, (unsigned long) t->v[RD_RAW_POST].num};
+            char msg[320];
+            if (img_open(name, read_format(t, name), &raw, &p->f, msg, sizeof msg))
+                return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_IO, "%s", msg);
+            p->open = 1;
+            if (y >= p->f.height)
+                return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG,
+                                  "read: vertical ROI start %u >= height %u", y, p->f.height);
+            if (!p->roi_height)
+                p->roi_height = p->f.height - y;
+            else if ((unsigned long long) y + p->roi_height > p->f.height)
+                return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG,
+                                  "read: vertical ROI start + height %llu >= height %u",
+                                  (unsigned long long) y + p->roi_height, p->f.height);
+            if (p->image_start >= p->f.n_images) {
+                p->image_start -= p->f.n_images;
+                continue;
+            }
+            p->index = p->image_start;
+            p->image_start = 0;
+            break;
+        }
+    }
+    req->n_dims = 2;
+    req->dims[0] = p->f.width;
+    req->dims[1] = (p->roi_height - 1) / uprop(t, RD_YSTEP) + 1;
+    req->dims[2] = 1;
+    return UFOMI_OK;
+}
+
+/* generate (ufo-read-task.c:331-353) with the readers' image stepping
+ * (ufo-tiff-reader.c:186-216, ufo-edf-reader.c:160-165, ufo-raw-reader.c:150-155):
+ * one image read, then image-step images consumed or the rest of the file */
+int read_task_generate_host(UfomiTask *t, float *dst, const UfomiRequisition *req)
+{
+    ReadPriv *p = t->priv;
+    if (p->current == uprop(t, RD_NUMBER) || p->done || !p->open || p->index >= p->f.n_images)
+        return 0;
+    const unsigned ystep = uprop(t, RD_YSTEP), rows = (p->roi_height - 1) / ystep + 1;
+    if (req->dims[0] != p->f.width || req->dims[1] != rows)
+        return process_fail(t, UFOMI_ERR_ARG, "read: requisition does not match the open file");
+    if (!t->v[RD_CONVERT].num && !(p->f.kind == 'f'))
+        return process_fail(t, UFOMI_ERR_ARG,
+                            "read: convert=false keeps %d-bit integer samples, but buffers here are float32",
+                            8 * p->f.bytes);
+    char msg[320];
+    if (img_read(&p->f, p->index, uprop(t, RD_Y), ystep, rows, dst, msg, sizeof msg))
+        return process_fail(t, UFOMI_ERR_IO, "read: %s", msg);
+    const unsigned image_step = uprop(t, RD_IMAGE_STEP), left = p->f.n_images - p->index;
+    const unsigned consumed = image_step < left ? image_step : left;
+    p->index += consumed;
+    p->image_start = image_step - consumed;
+    p->current++;
+    return 1;
+}
+
+static int read_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
+{
+    ReadPriv *p = t->priv;
+    const size_t n = req->dims[0] * req->dims[1];
+    if (n > p->stage_cap) {
+        free(p->stage);
+        p->stage = malloc(n * sizeof(float));
+        p->stage_cap = p->stage ? n : 0;
+        if (!p->stage)
+            return process_fail(t, UFOMI_ERR_MEMORY, "read: out of host memory");
+    }
+    if (!read_task_generate_host(t, p->stage, req))
+        return 0;
+    int rc = ufomi_memcpy(t->ctx, out->data, p->stage, n * sizeof(float), 0);
+    if (rc)
+        return process_fail(t, rc, "read: %s", ufomi_last_error());
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+static void read_finalize(UfomiTask *t)
+{
+    ReadPriv *p = t->priv;
+    if (p) {
+        read_reset(p);
+        free(p->stage);
+    }
+}
+
+/* ------------------------------------------------------------------ write */
+
+/* write (ufo-write-task.c:606-690, init :693-718); jpeg-quality exists only
+ * with libjpeg and HDF5 output needs libhdf5, neither of which is built here */
+static const PropSpec write_props[] = {
+    {"filename", PT_STRING, 0, 0, 0, NULL, 0, ""},
+    {"counter-start", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"counter-step", PT_UINT, 1, U32MAX, 1, NULL, 0, NULL},
+    {"bytes-per-file", PT_UINT, 0, ULMAX, 137438953472.0, NULL, 0, NULL},
+    {"append", PT_BOOL, 0, 1, 0, NULL, 0, NULL},
+    {"bits", PT_UINT, 8, 32, 32, NULL, 0, NULL},
+    {"minimum", PT_FLOAT, -FLT_MAX, FLT_MAX, FLT_MAX, NULL, 0, NULL},
+    {"maximum", PT_FLOAT, -FLT_MAX, FLT_MAX, -FLT_MAX, NULL, 0, NULL},
+    {"rescale", PT_BOOL, 0, 1, 1, NULL, 0, NULL},
+    {"tiff-bigtiff", PT_BOOL, 0, 1, 1, NULL, 0, NULL},
+};
+enum { WR_FILENAME, WR_CSTART, WR_CSTEP, WR_BPF, WR_APPEND, WR_BITS, WR_MIN, WR_MAX, WR_RESCALE, WR_BIGTIFF };
+
+typedef struct {
+    ImgWriter w;
+    int opened, n_fmt, format;
+    unsigned counter;
+    double written;
+    /* bits per sample of the bytes-per-file accounting: an explicit bits=32
+     * counts 16 there (ufo-write-task.c:438-441), the default counts 32 */
+    unsigned acct_bits;
+    float *stage;
+    size_t stage_cap;
+} WritePriv;
+
+/* set_property of bits (ufo-write-task.c:419-443): values other than 8, 16
+ * and 32 are ignored with a warning; returns 0 then, 1 to store the value. */
+static int write_bits_property(UfomiTask *t, double bits)
+{
+    WritePriv *p = t->priv;
+    if (bits != 8 && bits != 16 && bits != 32) {
+        fprintf(stderr, "warning: Write::bits can only 8, 16 or 32\n");
+        return 0;
+    }
+    p->acct_bits = bits == 32 ? 16 : (unsigned) bits;
+    return 1;
+}
+
+/* The one conversion allowed in `filename': %[flags][width][.precision] of d, i, o, u, x or X. */
+static int int_format_ok(const char *s)
+{
+    const char *q = strchr(s, '%');
+    if (!q)
+        return 1;
+    q++;
+    q += strspn(q, "-+ #0");
+    q += strspn(q, "0123456789");
+    if (*q == '.') {
+        q++;
+        q += strspn(q, "0123456789");
+    }
+    return *q && strchr("diouxX", *q);
+}
+
+static void write_name(const UfomiTask *t, const WritePriv *p, char *buf, size_t len)
+{
+    const char *fmt = t->v[WR_FILENAME].str;
+    if (p->n_fmt)
+        snprintf(buf, len, fmt, p->counter);
+    else
+        snprintf(buf, len, "%s", fmt);
+}
+
+static int exists(const char *path)
+{
+    struct stat st;
+    return stat(path, &st) == 0;
+}
+
+static int mkdir_parents(const char *dir)
+{
+    char tmp[4096];
+    snprintf(tmp, sizeof tmp, "%s", dir);
+    for (char *c = tmp + 1; *c; c++) {
+        if (*c != '/')
+            continue;
+        *c = '\0';
+        if (!exists(tmp) && mkdir(tmp, 0755) && errno != EEXIST)
+            return -1;
+        *c = '/';
+    }
+    return !exists(tmp) && mkdir(tmp, 0755) && errno != EEXIST ? -1 : 0;
+}
+
+/* setup (ufo-write-task.c:165-262) */
+static int write_setup(UfomiTask *t, UfomiError *e)
+{
+    WritePriv *p = t->priv;
+    if (p->opened)
+        img_writer_close(&p->w);
+    p->opened = 0;
+    p->written = 0;
+    p->counter = 0;
+    if (!p->acct_bits)
+        p->acct_bits = 32;
+    const char *fn = t->v[WR_FILENAME].str;
+    if (!fn || !*fn) {  /* no filename: raw to stdout */
+        p->format = IMG_RAW;
+        p->n_fmt = 0;
+        return UFOMI_OK;
+    }
+    p->n_fmt = 0;
+    for (const char *c = fn; *c; c++)
+        p->n_fmt += *c == '%';
+    if (p->n_fmt > 1)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "`%s' has too many format specifiers", fn);
+    if (!int_format_ok(fn))
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG,
+                          "`%s' needs an integer format specifier such as %%i or %%05d", fn);
+    if (!p->n_fmt && exists(fn) && access(fn, W_OK) < 0)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_IO, "Cannot access `%s': %s.", fn, strerror(errno));
+    p->format = img_format_of(fn);
+    if (p->format != IMG_TIFF && p->format != IMG_RAW)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG,
+                          "`%s' does not have a valid file extension or requires format specifiers", fn);
+    char *copy = strdup(fn), *dir = dirname(copy);
+    if (!exists(dir) && mkdir_parents(dir)) {
+        task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_IO, "Could not create `%s'.", dir);
+        free(copy);
+        return UFOMI_ERR_IO;
+    }
+    free(copy);
+    p->counter = uprop(t, WR_CSTART);
+    if (t->v[WR_APPEND].num && p->n_fmt) {
+        char name[4096];
+        for (;;) {
+            write_name(t, p, name, sizeof name);
+            if (!exists(name))
+                break;
+            p->counter++;
+        }
+    }
+    return UFOMI_OK;
+}
+
+static int write_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    (void) t;
+    (void) in;
+    (void) e;
+    req->n_dims = 0;
+    return UFOMI_OK;
+}
+
+/* process (ufo-write-task.c:310-392): each plane of the item to the open file,
+ * a new file (counter += counter-step) once bytes-per-file would be exceeded */
+int write_task_write_host(UfomiTask *t, const float *src, const UfomiRequisition *req)
+{
+    WritePriv *p = t->priv;
+    const unsigned w = (unsigned) req->dims[0], h = req->n_dims >= 2 ? (unsigned) req->dims[1] : 1;
+    const unsigned frames = req->n_dims == 3 ? (unsigned) req->dims[2] : 1;
+    const double out_size = (double) w * h * p->acct_bits / 8;
+    const char *fn = t->v[WR_FILENAME].str;
+    char msg[320];
+    for (unsigned i = 0; i < frames; i++) {
+        if (!p->opened) {
+            char name[4096];
+            const char *path = NULL;
+            if (fn && *fn) {
+                for (;;) {
+                    write_name(t, p, name, sizeof name);
+                    if (!(exists(name) && access(name, W_OK) < 0))
+                        break;
+                    fprintf(stderr, "warning: Cannot access `%s': %s.\n", name, strerror(errno));
+                    p->counter += uprop(t, WR_CSTEP);
+                }
+                path = name;
+            }
+            if (img_writer_open(&p->w, path, p->format, (int) t->v[WR_BIGTIFF].num, msg, sizeof msg)) {
+                process_fail(t, UFOMI_ERR_IO, "write: %s", msg);
+                return UFOMI_ERR_IO;
+            }
+            p->opened = 1;
+        }
+        if (img_writer_write(&p->w, src + (size_t) i * w * h, w, h, uprop(t, WR_BITS), (float) t->v[WR_MIN].num,
+                             (float) t->v[WR_MAX].num, (int) t->v[WR_RESCALE].num, msg, sizeof msg)) {
+            process_fail(t, UFOMI_ERR_IO, "write: %s", msg);
+            return UFOMI_ERR_IO;
+        }
+        fflush(p->w.fp);
+        p->written += out_size;
+        if (p->n_fmt && p->written + out_size > t->v[WR_BPF].num) {
+            img_writer_close(&p->w);
+            p->opened = 0;
+            p->written = 0;
+            p->counter += uprop(t, WR_CSTEP);
+        }
+    }
+    return UFOMI_OK;
+}
+
+static int write_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) out;
+    (void) req;
+    WritePriv *p = t->priv;
+    const UfomiBuffer *b = in[0];
+    if (b->dtype != UFOMI_F32)
+        return process_fail(t, UFOMI_ERR_UNSUPPORTED, "write: input items must be float32");
+    const size_t n = ufomi_requisition_elements(&b->req);
+    if (n > p->stage_cap) {
+        free(p->stage);
+        p->stage = malloc(n * sizeof(float));
+        p->stage_cap = p->stage ? n : 0;
+        if (!p->stage)
+            return process_fail(t, UFOMI_ERR_MEMORY, "write: out of host memory");
+    }
+    int rc = ufomi_memcpy(t->ctx, p->stage, b->data, n * sizeof(float), 1);
+    if (rc)
+        return process_fail(t, rc, "write: %s", ufomi_last_error());
+    return write_task_write_host(t, p->stage, &b->req) == UFOMI_OK;
+}
+
+void write_task_flush(UfomiTask *t)
+{
+    WritePriv *p = t->priv;
+    if (p->opened)
+        img_writer_close(&p->w);
+    p->opened = 0;
+}
+
+static void write_finalize(UfomiTask *t)
+{
+    WritePriv *p = t->priv;
+    if (p) {
+        write_task_flush(t);
+        free(p->stage);
+    }
+}
+
 /* ---------------------------------------------------------------- classes */
 
 #define NPROPS(a) ((int) (sizeof(a) / sizeof((a)[0])))
@@ -782,6 +1252,10 @@ static const TaskClass classes[] = {
      sizeof(CwPriv), cw_setup, cw_requisition, cw_process, NULL, NULL},
     {"general-backproject", gbp_props, NPROPS(gbp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
      sizeof(GbpPriv), gbp_setup, gbp_requisition, gbp_process, gbp_generate, gbp_finalize},
+    {"read", read_props, NPROPS(read_props), UFOMI_TASK_MODE_GENERATOR | UFOMI_TASK_MODE_CPU, sizeof(ReadPriv),
+     read_setup, read_requisition, NULL, read_generate, read_finalize},
+    {"write", write_props, NPROPS(write_props), UFOMI_TASK_MODE_SINK | UFOMI_TASK_MODE_CPU, sizeof(WritePriv),
+     write_setup, write_requisition, write_process, NULL, write_finalize},
 };
 
 UfomiTask *ufomi_task_new(const char *name)
@@ -810,8 +1284,10 @@ void ufomi_task_free(UfomiTask *t)
         return;
     if (t->cls->finalize)
         t->cls->finalize(t);
-    for (int i = 0; i < t->cls->n_props; i++)
+    for (int i = 0; i < t->cls->n_props; i++) {
         free(t->v[i].arr);
+        free(t->v[i].str);
+    }
     free(t->priv);
     free(t);
 }
@@ -820,27 +1296,33 @@ const char *ufomi_task_name(const UfomiTask *t) { return t ? t->cls->name : NULL
 
 int ufomi_task_setup(UfomiTask *t, ufomi_ctx *ctx, UfomiError *e)
 {
-    if (!t || !ctx)
+    if (!t || (!ctx && !(t->cls->mode & UFOMI_TASK_MODE_CPU)))
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "null task or context");
     t->ctx = ctx;
-    return t->cls->setup ? t->cls->setup(t, e) : UFOMI_OK;
+    t->ready = 0;
+    const int rc = t->cls->setup ? t->cls->setup(t, e) : UFOMI_OK;
+    t->ready = rc == UFOMI_OK;
+    return rc;
 }
 
 int ufomi_task_get_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
 {
-    if (!t || !in || !in[0] || !req)
+    if (!t || !req || (!(t->cls->mode & UFOMI_TASK_MODE_GENERATOR) && (!in || !in[0])))
         return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG, "null argument");
-    if (!t->ctx)
+    if (!t->ready)
         return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG, "task not set up");
     return t->cls->requisition(t, in, req, e);
 }
 
-unsigned ufomi_task_get_num_inputs(const UfomiTask *t) { return t ? 1 : 0; }
+unsigned ufomi_task_get_num_inputs(const UfomiTask *t)
+{
+    return t && !(t->cls->mode & UFOMI_TASK_MODE_GENERATOR) ? 1 : 0;
+}
 
 unsigned ufomi_task_get_num_dimensions(const UfomiTask *t, unsigned input)
 {
     (void) input;
-    if (!t)
+    if (!t || (t->cls->mode & UFOMI_TASK_MODE_GENERATOR))
         return 0;
     if (!strcmp(t->cls->name, "fft"))
         return (int) t->v[FFT_DIMS].num > 2 ? 3 : 2;
@@ -853,7 +1335,8 @@ unsigned ufomi_task_get_mode(const UfomiTask *t) { return t ? t->cls->mode : 0; 
 
 int ufomi_task_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
 {
-    if (!t || !in || !in[0] || !out || !req || !t->ctx)
+    if (!t || !t->cls->process || !in || !in[0] || (!out && !(t->cls->mode & UFOMI_TASK_MODE_SINK)) || !req ||
+        !t->ctx)
         return 0;
     t->err[0] = '\0';
     t->err_status = 0;
@@ -863,10 +1346,33 @@ int ufomi_task_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRe
     return ok;
 }
 
+int ufomi_task_generate_host(UfomiTask *t, float *out, const UfomiRequisition *req)
+{
+    if (!t || !out || !req || !t->ready || strcmp(t->cls->name, "read"))
+        return 0;
+    t->err[0] = '\0';
+    t->err_status = 0;
+    return read_task_generate_host(t, out, req);
+}
+
+int ufomi_task_process_host(UfomiTask *t, const float *in, const UfomiRequisition *req)
+{
+    if (!t || !in || !req || !t->ready || strcmp(t->cls->name, "write"))
+        return 0;
+    t->err[0] = '\0';
+    t->err_status = 0;
+    if (write_task_write_host(t, in, req) != UFOMI_OK)
+        return 0;
+    t->num_processed++;
+    return 1;
+}
+
 int ufomi_task_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
 {
-    if (!t || !out || !req || !t->cls->generate)
+    if (!t || !out || !req || !t->cls->generate || !t->ctx)
         return 0;
+    t->err[0] = '\0';
+    t->err_status = 0;
     return t->cls->generate(t, out, req);
 }
 

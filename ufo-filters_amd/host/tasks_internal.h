@@ -6,7 +6,7 @@
 
 #define UFOMI_MAX_PROPS 40
 
-typedef enum { PT_UINT, PT_DOUBLE, PT_FLOAT, PT_BOOL, PT_ENUM, PT_DARRAY } PropType;
+typedef enum { PT_UINT, PT_DOUBLE, PT_FLOAT, PT_BOOL, PT_ENUM, PT_DARRAY, PT_STRING } PropType;
 
 typedef struct {
     const char *name;
@@ -14,12 +14,14 @@ typedef struct {
     double min, max, dflt;      /* numeric range and default (enums: default index) */
     const char *const *nicks;   /* enum value nicks */
     int n_default;              /* arrays: number of default elements (all = dflt) */
+    const char *sdflt;          /* strings: default value */
 } PropSpec;
 
 typedef struct {
     double num;
     double *arr;
     int n;
+    char *str;
 } PropValue;
 
 typedef struct TaskClass {
@@ -44,6 +46,7 @@ struct UfomiTask {
     unsigned num_processed;
     char err[512];
     int err_status;
+    int ready;                   /* setup succeeded */
 };
 
 int task_error(UfomiError *e, int code, int status, const char *fmt, ...);
@@ -55,5 +58,12 @@ int filter_task_coefficients(UfomiTask *t, size_t width, UfomiError *e);
 const float *filter_task_device_coefficients(const UfomiTask *t);
 void backproject_task_params(const UfomiTask *t, ufomi_bp_params *bp);
 int gbp_task_store_type(const UfomiTask *t);
+/* read: the next image into host memory (req from ufomi_task_get_requisition);
+ * 1 = produced, 0 = no more images or an error (then t->err_status is set). */
+int read_task_generate_host(UfomiTask *t, float *dst, const UfomiRequisition *req);
+/* write: one item from host memory; UFOMI_OK or a status with t->err set. */
+int write_task_write_host(UfomiTask *t, const float *src, const UfomiRequisition *req);
+/* write: closes the open file (the end of a stream). */
+void write_task_flush(UfomiTask *t);
 
 #endif

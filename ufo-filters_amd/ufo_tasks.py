@@ -9,7 +9,9 @@ gi.repository.Ufo API the reference's tests drive (tests/test_fft.py:57-112):
 Scheduler runs a linear graph item by item through ufomi_task_get_requisition /
 process / generate, as ufo-core does; Pipeline runs the C batch runner
 (ufomi_pipeline_*) that fuses fft ! filter ! ifft.  Both go through libufomi.so
-on the GPU; there is no CPU path.
+on the GPU; there is no CPU path.  The file tasks `read` (a generator) and
+`write` (a sink) are CPU-mode tasks: they also run without a device through
+Task.generate_host / Task.process_host.
 """
 import ctypes
 
@@ -18,7 +20,7 @@ import numpy as np
 import ufomi
 
 LAYOUT_REAL, LAYOUT_COMPLEX = 0, 1
-MODE_PROCESSOR, MODE_REDUCTOR, MODE_GENERATOR, MODE_GPU = 1, 2, 4, 16
+MODE_PROCESSOR, MODE_REDUCTOR, MODE_GENERATOR, MODE_SINK, MODE_GPU, MODE_CPU = 1, 2, 4, 8, 16, 32
 
 
 class Requisition(ctypes.Structure):
@@ -63,6 +65,7 @@ def _lib():
             "ufomi_task_get_mode": ([vp], cu),
             "ufomi_task_process": ([vp, ctypes.POINTER(B), B, R], ci),
             "ufomi_task_generate": ([vp, B, R], ci),
+            "ufomi_task_generate_host": ([vp, vp, R], ci), "ufomi_task_process_host": ([vp, vp, R], ci),
             "ufomi_task_output_layout": ([vp], ci), "ufomi_task_last_error": ([vp], cs),
             "ufomi_pipeline_new": ([cs, E], vp), "ufomi_pipeline_free": ([vp], None),
             "ufomi_pipeline_num_tasks": ([vp], ci), "ufomi_pipeline_task": ([vp, ci], vp),
@@ -81,11 +84,16 @@ TASK_EXPORTS = [
     "ufomi_task_new", "ufomi_task_free", "ufomi_task_name", "ufomi_task_set_property", "ufomi_task_get_property",
     "ufomi_task_list_properties", "ufomi_task_setup", "ufomi_task_get_requisition", "ufomi_task_get_num_inputs",
     "ufomi_task_get_num_dimensions", "ufomi_task_get_mode", "ufomi_task_process", "ufomi_task_generate",
+    "ufomi_task_generate_host", "ufomi_task_process_host",
     "ufomi_task_output_layout", "ufomi_task_output_dtype", "ufomi_task_last_error", "ufomi_buffer_new",
     "ufomi_buffer_release", "ufomi_requisition_elements", "ufomi_pipeline_new", "ufomi_pipeline_free",
     "ufomi_pipeline_num_tasks", "ufomi_pipeline_task", "ufomi_pipeline_run", "ufomi_pipeline_used_fused_filter",
     "ufomi_device_alloc", "ufomi_device_free", "ufomi_memcpy", "ufomi_device_memory",
 ]
+
+
+# properties of GType gchararray: returned as text, whatever they look like
+_STRING_PROPS = {("read", "path"), ("write", "filename")}
 
 
 class _Props:
@@ -137,6 +145,8 @@ class Task:
         if rc:
             raise ufomi.UfomiError(rc, self.L.ufomi_last_error().decode())
         s = buf.value.decode()
+        if (self.L.ufomi_task_name(self.h).decode(), name.replace("_", "-")) in _STRING_PROPS:
+            return s
         if s in ("true", "false"):
             return s == "true"
         try:
@@ -163,17 +173,50 @@ class Task:
     def mode(self):
         return self.L.ufomi_task_get_mode(self.h)
 
-    def setup(self, ctx):
+    def setup(self, ctx=None):
+        """ctx None is allowed for CPU-mode tasks (read, write) on host memory."""
         err = Error()
-        if self.L.ufomi_task_setup(self.h, ctx.h, ctypes.byref(err)):
+        if self.L.ufomi_task_setup(self.h, ctx.h if ctx is not None else None, ctypes.byref(err)):
             raise TaskError(err)
 
-    def get_requisition(self, buf):
+    def get_requisition(self, buf=None):
+        """buf None for generators; an exhausted generator returns n_dims 0."""
         err, req = Error(), Requisition()
-        arr = (ctypes.POINTER(Buffer) * 1)(ctypes.pointer(buf))
+        arr = (ctypes.POINTER(Buffer) * 1)(ctypes.pointer(buf)) if buf is not None else None
         if self.L.ufomi_task_get_requisition(self.h, arr, ctypes.byref(req), ctypes.byref(err)):
             raise TaskError(err)
         return req
+
+    def generate_host(self, req):
+        """One generated item as a float32 numpy array, None at the end; raises on failure."""
+        out = np.empty(_req_shape(req), dtype=np.float32)
+        if self.L.ufomi_task_generate_host(self.h, out.ctypes.data, ctypes.byref(req)):
+            return out
+        msg = self.last_error()
+        if msg:
+            raise RuntimeError("%s.generate failed: %s" % (self.name, msg))
+        return None
+
+    def process_host(self, array):
+        """Hands one float32 item (numpy [h][w] or [d][h][w]) to a CPU-mode sink."""
+        a = np.ascontiguousarray(array, dtype=np.float32)
+        req = Requisition()
+        req.n_dims = a.ndim
+        for i, n in enumerate(reversed(a.shape)):
+            req.dims[i] = n
+        if not self.L.ufomi_task_process_host(self.h, a.ctypes.data, ctypes.byref(req)):
+            raise RuntimeError("%s.process failed: %s" % (self.name, self.last_error()))
+
+    def items(self):
+        """Every item of a set-up generator, as numpy arrays."""
+        while True:
+            req = self.get_requisition()
+            if req.n_dims == 0:
+                return
+            item = self.generate_host(req)
+            if item is None:
+                return
+            yield item
 
     def process(self, buf, out, req):
         arr = (ctypes.POINTER(Buffer) * 1)(ctypes.pointer(buf))
@@ -253,6 +296,10 @@ class Scheduler:
         src, sink, tasks = nodes[0], nodes[-1], nodes[1:-1]
         for t in tasks:
             t.setup(self.ctx)
+        if isinstance(src, Task):
+            src.setup(self.ctx)
+        if isinstance(sink, Task):
+            sink.setup(self.ctx)
         keep = []
 
         def make_buf(tensor, layout):
@@ -269,7 +316,10 @@ class Scheduler:
             """Send one item into tasks[stage:], returns nothing; outputs go to the sink."""
             if stage == len(tasks):
                 torch.cuda.synchronize()
-                sink.items.append(tensor.cpu().numpy())
+                if isinstance(sink, Task):
+                    sink.process_host(tensor.cpu().numpy())
+                else:
+                    sink.items.append(tensor.cpu().numpy())
                 return
             t = tasks[stage]
             buf = make_buf(tensor, layout)
@@ -296,7 +346,7 @@ class Scheduler:
 
         reducer_in = {}
         dev = torch.device("cuda:%d" % self.ctx.device)
-        for item in src.array:
+        for item in (src.items() if isinstance(src, Task) else src.array):
             x = torch.from_numpy(item).to(dev)
             buf = make_buf(x, LAYOUT_REAL)
             for k, t in enumerate(tasks):
@@ -335,17 +385,26 @@ class Pipeline:
     def used_fused_filter(self):
         return bool(self.L.ufomi_pipeline_used_fused_filter(self.h))
 
-    def run(self, ctx, items, out_capacity_items=None, out_bytes=None):
-        """items: torch tensor [n][h][w] float32 on the device; returns (tensor of bytes, n_out, shape)."""
+    def run(self, ctx, items=None, out_capacity_items=None, out_bytes=None):
+        """items: torch tensor [n][h][w] float32 on the device (None when the first
+        stage is `read`); returns (tensor of bytes or None when the last stage is
+        `write`, n_out, shape)."""
         import torch
-        n, h, w = items.shape
         req = Requisition()
-        req.n_dims = 2
-        req.dims[0], req.dims[1], req.dims[2] = w, h, 1
-        cap = out_bytes or (items.numel() * 4 * 8)
-        out = torch.empty(cap, dtype=torch.uint8, device=items.device)
+        n, ptr, reqp = 0, None, None
+        if items is not None:
+            n, h, w = items.shape
+            req.n_dims = 2
+            req.dims[0], req.dims[1], req.dims[2] = w, h, 1
+            ptr, reqp = items.data_ptr(), ctypes.byref(req)
+        n_tasks = self.L.ufomi_pipeline_num_tasks(self.h)
+        last = self.L.ufomi_task_name(self.L.ufomi_pipeline_task(self.h, n_tasks - 1)).decode()
+        out, cap = None, 0
+        if last != "write":
+            cap = out_bytes or (items.numel() * 4 * 8 if items is not None else 1 << 30)
+            out = torch.empty(cap, dtype=torch.uint8, device="cuda:%d" % ctx.device)
         n_out, oreq, err = ctypes.c_int(), Requisition(), Error()
-        rc = self.L.ufomi_pipeline_run(self.h, ctx.h, items.data_ptr(), ctypes.byref(req), n, out.data_ptr(), cap,
+        rc = self.L.ufomi_pipeline_run(self.h, ctx.h, ptr, reqp, n, out.data_ptr() if out is not None else None, cap,
                                        ctypes.byref(n_out), ctypes.byref(oreq), ctypes.byref(err))
         if rc:
             raise TaskError(err)

@@ -18,7 +18,7 @@ OK = 0
 STATUS = {
     -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOT_POW2", -4: "ERR_UNSUPPORTED", -5: "ERR_PROJECTIONS",
     -6: "ERR_SIZE", -7: "ERR_NO_DEVICE", -8: "ERR_GRAY_MAP", -9: "ERR_MEMORY", -10: "ERR_NUM_PROJECTIONS",
-    -11: "ERR_LAYOUT", -12: "ERR_INCOMPLETE",
+    -11: "ERR_LAYOUT", -12: "ERR_INCOMPLETE", -13: "ERR_IO",
 }
 F32, F16, BF16 = 0, 1, 2
 FILTERS = {"ramp": 0, "ramp-fromreal": 1, "butterworth": 2, "faris-byer": 3, "hamming": 4, "bh3": 5,
