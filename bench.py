@@ -19,6 +19,8 @@ Other workloads (--workload):
   learned 512^2 textured phantoms: train the residual U-Net on (120-view FBP,
           1800-view FBP) pairs, then slices/s and RMSE to the phantom of 120-view
           FBP + U-Net against plain 1800-view FBP (--views 60 for 60 views)
+  transpose  transpose-projections of 1800 projections of 2048^2 (the fbp2048
+          data as projections) into 2048 sinograms: GB/s against HBM
   batch   256 single-slice requests of uneven ROI (log-uniform in [256,2048]^2)
           and angle count ({360,720,1200,1800,3600}) as one HIP graph: one
           segmented ramp launch per FFT size + one persistent backprojection
@@ -337,6 +339,60 @@ def general(args, ctx, torch, np, ufomi, kind):
                 metric="GUPS (voxel*proj/s) for %s" % ("FDK cone-beam" if kind == "cone" else "laminography"))
 
 
+def transpose(args, ctx, torch, np, ufomi):
+    """transpose-projections of P projections [P][H][W] into H sinograms [H][P][W]; one step = one launch."""
+    dev = torch.device("cuda:0")
+    P, H, W = 1800, 2048, 2048
+    g = torch.Generator(device=dev).manual_seed(17)
+    projs = torch.randn((P, H, W), device=dev, generator=g)
+    sinos = torch.empty((H, P, W), device=dev)
+    torch.cuda.synchronize()
+    log("transpose: %d projections of %dx%d (%.1f GiB each way)" % (P, H, W, projs.numel() * 4 / 2 ** 30))
+
+    def step(e):
+        if e: e[0].record()
+        ctx.transpose_projections(projs, W, H, P, sinos)
+        if e: e[1].record()
+
+    wall, ev = run_timed(torch, args.steps, args.warmup, step, 2)
+    k_ms = mean_ms(ev, 0, 1)
+    moved = 2.0 * P * H * W * 4
+    # exactness on a probe: sinogram rows equal projection rows
+    probe = [(0, 0), (H - 1, P - 1), (H // 3, P // 2)]
+    exact = all(torch.equal(sinos[h, p], projs[p, h]) for h, p in probe)
+    cpu = None
+    if not args.no_cpu_baseline:
+        oracle = oracle_module(1)
+        sub = 48
+        host = projs[:, :sub].cpu().numpy()
+        t0 = time.perf_counter()
+        oracle.transpose_projections(host, P)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(2.0 * host.nbytes / dt / 1e9, 2), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": "rows 0..%d of every projection (%d x %d x %d float32) through the numpy oracle, %.2f s"
+                         % (sub - 1, P, sub, W, dt)}
+    hbm_copy = measure_hbm_copy(ctx, torch)
+    key = "transpose_rows_kernel transpose"
+    traffic = traffic_for(key)
+    result = {
+        "metric": "GB/s transpose-projections (projections -> sinograms, bytes read + written)",
+        "value": round(moved / (k_ms * 1e-3) / 1e9, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic: seeded normal projections",
+        "config": {"workload": "transpose-projections: %d projections of %dx%d -> %d sinograms of %dx%d"
+                               % (P, H, W, H, P, W)},
+        "roofline": {"kernel": "transpose_rows_kernel<float4>", "bound": "hbm",
+                     "achieved": round(moved / (k_ms * 1e-3) / 1e9, 1), "peak": MI355X_HBM_TBPS * 1e3,
+                     "unit": "GB/s", "frac": round(moved / (k_ms * 1e-3) / (MI355X_HBM_TBPS * 1e12), 4),
+                     "traffic": traffic, "algorithmic_bytes": moved, "kernel_ms": round(k_ms, 3),
+                     "hbm_copy_GBps": round(hbm_copy / 1e9, 1)},
+        "exact": exact,
+    }
+    if cpu is not None:
+        result["cpu_baseline"] = cpu
+    return result
+
+
 def batch(args, ctx, torch, np, ufomi):
     """256 uneven single-slice FBP requests as one HIP graph: segmented ramp filters + one persistent
     backprojection launch, tile / angle chunk chosen by the autotuner."""
@@ -591,7 +647,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["fbp2048", "fbp512", "cone", "lamino", "batch", "learned"], default="fbp2048")
+    ap.add_argument("--workload", choices=["fbp2048", "fbp512", "cone", "lamino", "batch", "learned", "transpose"],
+                    default="fbp2048")
     ap.add_argument("--slices", type=int, default=2048)
     ap.add_argument("--storage", choices=["f32", "f16", "bf16", "all"], default=None,
                     help="filtered-sinogram storage; 'all' (the fbp2048 default) reports f32 as the value "
@@ -628,6 +685,10 @@ def main():
         r = fbp_parallel(args, ctx, torch, np, ufomi, 1, 512, 512, "ramp-fromreal", 0.01, storages)
     elif args.workload == "batch":
         r = batch(args, ctx, torch, np, ufomi)
+    elif args.workload == "transpose":
+        print(json.dumps(transpose(args, ctx, torch, np, ufomi)), flush=True)
+        ctx.close()
+        return
     elif args.workload == "learned":
         r = learned_fbp(args, ctx, torch, np, ufomi)
         result = {

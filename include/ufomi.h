@@ -289,6 +289,17 @@ int ufomi_cone_weight(ufomi_ctx *ctx, const float *d_in, float *d_out, int W, in
                       const double *center_x, const double *center_z, const double *source_distance,
                       const double *detector_distance, const double *axis_angle_x, int n_param);
 
+/*
+ * `transpose-projections` (src/ufo-transpose-projections-task.c:64-94,
+ * 120-145): n projections [n][H][W] become rows first .. first + n - 1 of
+ * the H sinograms [H][number][W] at d_sino (row h of projection p is row
+ * first + p of sinogram h).  d_proj NULL writes zero rows there instead: the
+ * reference's store is zero-initialised, so rows of projections that never
+ * arrive read as zeros.  Requires first + n <= number.
+ */
+int ufomi_transpose_projections(ufomi_ctx *ctx, const float *d_proj, int W, int H, int n, int first, int number,
+                                float *d_sino);
+
 /* ------------------------------------------------- general backprojection */
 
 /* Per-projection geometry, the arrays of the general-backproject task

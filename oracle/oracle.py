@@ -208,6 +208,22 @@ def cone_weight_ref32(proj, center_x, center_z, source_distance, detector_distan
     return out
 
 
+# ------------------------------------------------------- transpose-projections
+
+def transpose_projections(projs, number):
+    """Sinograms [H][number][W] of projections [n][H][W]
+    (src/ufo-transpose-projections-task.c:64-94, 120-145): row h of projection
+    p is row p of sinogram h; rows of projections that never arrive stay zero
+    (the store is zero-initialised) and projections past `number` are not
+    taken (process returns FALSE)."""
+    projs = np.asarray(projs, dtype=np.float32)
+    n, h, w = projs.shape
+    out = np.zeros((h, number, w), dtype=np.float32)
+    m = min(n, number)
+    out[:, :m] = projs[:m].transpose(1, 0, 2)
+    return out
+
+
 # ---------------------------------------------------------- general backproject
 
 _GEOM_KEYS = {

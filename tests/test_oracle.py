@@ -341,3 +341,28 @@ def test_planted_mistakes_are_caught():
     assert np.abs(good.T - ref).max() > 1e-3
     assert np.abs(oracle.bp_texture_f64(sino, axis_pos=axis + 0.5) - ref).max() > 1e-3
     assert np.abs(oracle.bp_texture_f64(sino[::-1], axis_pos=axis) - ref).max() > 1e-3
+
+
+def _transpose_flat_brute(projs, number):
+    """The reference's own loop over flat memory (ufo-transpose-projections-task.c:73-90):
+    projection p (1-based) lands at (p - 1) * width + i * width * number for row i."""
+    n, h, w = projs.shape
+    flat = np.zeros(number * w * h, dtype=np.float32)
+    for proj in range(1, n + 1):
+        if proj > number:
+            break
+        sino_index = (proj - 1) * w
+        src = projs[proj - 1].ravel()
+        for i in range(h):
+            flat[sino_index + i * w * number: sino_index + i * w * number + w] = src[i * w:(i + 1) * w]
+    return flat.reshape(h, number, w)
+
+
+@pytest.mark.parametrize("n,h,w,number", [(5, 3, 7, 5), (4, 6, 2, 9), (9, 2, 5, 6), (1, 1, 1, 1)])
+def test_transpose_projections_vs_reference_loop(n, h, w, number):
+    projs = np.random.default_rng(n * 100 + number).standard_normal((n, h, w)).astype(np.float32)
+    got = oracle.transpose_projections(projs, number)
+    np.testing.assert_array_equal(got, _transpose_flat_brute(projs, number))
+    # involution when every row is filled: transposing the sinograms back gives the projections
+    if n == number:
+        np.testing.assert_array_equal(oracle.transpose_projections(got, h), projs)

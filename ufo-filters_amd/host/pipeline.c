@@ -8,6 +8,8 @@
  *   - `fft ! filter ! ifft` (1-D) becomes one launch of the fused LDS
  *     ramp-filter kernel over every row of every item;
  *   - `backproject` runs all items as one multi-slice launch;
+ *   - `transpose-projections` turns all projections into sinograms in one
+ *     launch;
  *   - other processors run item by item;
  *   - `general-backproject` takes every item as a projection and emits its
  *     slices through generate;
@@ -327,6 +329,31 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
                 goto out;
             }
             t->num_processed += (unsigned) cur.n;
+        } else if (is(t, "transpose-projections")) {
+            /* all projections of the batch in one launch, zero rows for missing ones; the
+             * requisition of ufo-transpose-projections-task.c:113-146 without the task's own store */
+            req.n_dims = 2;
+            req.dims[0] = cur.buf.req.dims[0];
+            req.dims[1] = (size_t) t->v[0].num;
+            req.dims[2] = 1;
+            if (cur.buf.dtype != UFOMI_F32) {
+                rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_UNSUPPORTED,
+                                "transpose-projections: input must be float32");
+                goto out;
+            }
+            const int W = (int) req.dims[0], number = (int) req.dims[1], H = (int) rows_of(&cur.buf.req);
+            const int used = cur.n < number ? cur.n : number;
+            if ((rc = batch_alloc(ctx, &next, &req, H, sizeof(float), UFOMI_F32, UFOMI_LAYOUT_REAL)))
+                goto fail_alloc;
+            rc = ufomi_transpose_projections(ctx, cur.buf.data, W, H, used, 0, number, next.buf.data);
+            if (!rc && used < number)
+                rc = ufomi_transpose_projections(ctx, NULL, W, H, number - used, used, number, next.buf.data);
+            if (rc) {
+                ufomi_buffer_release(ctx, &next.buf);
+                task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "transpose-projections: %s", ufomi_last_error());
+                goto out;
+            }
+            t->num_processed += (unsigned) used;
         } else if (ufomi_task_get_mode(t) & UFOMI_TASK_MODE_REDUCTOR) {
             if ((rc = ufomi_task_get_requisition(t, ins, &req, e)))
                 goto out;

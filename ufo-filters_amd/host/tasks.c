@@ -9,6 +9,7 @@
  *   backproject                  src/ufo-backproject-task.c
  *   cone-beam-projection-weight  src/ufo-cone-beam-projection-weight-task.c
  *   general-backproject          src/ufo-general-backproject-task.c
+ *   transpose-projections        src/ufo-transpose-projections-task.c
  *   read                         src/ufo-read-task.c
  *   write                        src/ufo-write-task.c
  */
@@ -786,6 +787,113 @@ static void gbp_finalize(UfomiTask *t)
     }
 }
 
+/* --------------------------------------------------- transpose-projections */
+
+/* transpose-projections (ufo-transpose-projections-task.c:245-251) */
+static const PropSpec tp_props[] = {
+    {"number", PT_UINT, 1, U32MAX, 1, NULL, 0, NULL},
+};
+
+typedef struct {
+    float *d_sino;         /* [height][number][width] */
+    unsigned width, height, number, count, generated;
+    int zeroed;
+} TpPriv;
+
+static void tp_free(UfomiTask *t)
+{
+    TpPriv *p = t->priv;
+    if (p->d_sino && t->ctx)
+        ufomi_device_free(t->ctx, p->d_sino);
+    p->d_sino = NULL;
+}
+
+static int tp_setup(UfomiTask *t, UfomiError *e)
+{
+    (void) e;
+    tp_free(t);
+    memset(t->priv, 0, sizeof(TpPriv));
+    return UFOMI_OK;
+}
+
+/* get_requisition (:113-146): the first projection fixes width and height;
+ * the output is one sinogram of `number` rows */
+static int tp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    TpPriv *p = t->priv;
+    const UfomiRequisition *ir = &in[0]->req;
+    const unsigned W = (unsigned) ir->dims[0], H = ir->n_dims >= 2 ? (unsigned) ir->dims[1] : 1;
+    if (!p->d_sino) {
+        p->number = (unsigned) t->v[0].num;
+        const size_t bytes = (size_t) H * p->number * W * sizeof(float);
+        int rc = ufomi_device_alloc(t->ctx, bytes, (void **) &p->d_sino);
+        if (rc)
+            return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_MEMORY,
+                              "transpose-projections: %zu bytes of sinograms do not fit: %s", bytes,
+                              ufomi_last_error());
+        p->width = W;
+        p->height = H;
+        p->count = p->generated = 0;
+        p->zeroed = 0;
+    } else if (W != p->width || H != p->height) {
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG,
+                          "transpose-projections: projection size changed");
+    }
+    req->n_dims = 2;
+    req->dims[0] = W;
+    req->dims[1] = p->number;
+    req->dims[2] = 1;
+    return UFOMI_OK;
+}
+
+/* process (:64-94): the rows of one projection into the sinograms; a
+ * projection beyond `number` ends the input (process returns FALSE) */
+static int tp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) out;
+    (void) req;
+    TpPriv *p = t->priv;
+    if (p->count >= p->number)
+        return 0;
+    if (in[0]->dtype != UFOMI_F32)
+        return process_fail(t, UFOMI_ERR_UNSUPPORTED, "transpose-projections: input must be float32");
+    int rc = ufomi_transpose_projections(t->ctx, in[0]->data, (int) p->width, (int) p->height, 1, (int) p->count,
+                                         (int) p->number, p->d_sino);
+    if (rc)
+        return process_fail(t, rc, "transpose-projections: %s", ufomi_last_error());
+    p->count++;
+    return 1;
+}
+
+/* generate (:96-111): sinogram after sinogram; rows of projections that
+ * never arrived are zero */
+static int tp_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    TpPriv *p = t->priv;
+    if (!p->d_sino || p->generated >= p->height)
+        return 0;
+    int rc;
+    if (!p->zeroed && p->count < p->number &&
+        (rc = ufomi_transpose_projections(t->ctx, NULL, (int) p->width, (int) p->height,
+                                          (int) (p->number - p->count), (int) p->count, (int) p->number,
+                                          p->d_sino)))
+        return process_fail(t, rc, "transpose-projections: %s", ufomi_last_error());
+    p->zeroed = 1;
+    const size_t sino = (size_t) p->number * p->width;
+    if ((rc = ufomi_memcpy(t->ctx, out->data, p->d_sino + p->generated * sino, sino * sizeof(float), 2)))
+        return process_fail(t, rc, "transpose-projections: %s", ufomi_last_error());
+    p->generated++;
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+static void tp_finalize(UfomiTask *t)
+{
+    if (t->priv)
+        tp_free(t);
+}
+
 /* ------------------------------------------------------------------- read */
 
 /* read (ufo-read-task.c:551-663; init :668-690 sets path "."; type_values :53-63
@@ -1252,6 +1360,8 @@ static const TaskClass classes[] = {
      sizeof(CwPriv), cw_setup, cw_requisition, cw_process, NULL, NULL},
     {"general-backproject", gbp_props, NPROPS(gbp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
      sizeof(GbpPriv), gbp_setup, gbp_requisition, gbp_process, gbp_generate, gbp_finalize},
+    {"transpose-projections", tp_props, NPROPS(tp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
+     sizeof(TpPriv), tp_setup, tp_requisition, tp_process, tp_generate, tp_finalize},
     {"read", read_props, NPROPS(read_props), UFOMI_TASK_MODE_GENERATOR | UFOMI_TASK_MODE_CPU, sizeof(ReadPriv),
      read_setup, read_requisition, NULL, read_generate, read_finalize},
     {"write", write_props, NPROPS(write_props), UFOMI_TASK_MODE_SINK | UFOMI_TASK_MODE_CPU, sizeof(WritePriv),

@@ -32,7 +32,7 @@ EXPORTS = [
     "ufomi_create", "ufomi_destroy", "ufomi_set_stream", "ufomi_synchronize", "ufomi_last_error",
     "ufomi_version", "ufomi_filter_coefficients", "ufomi_fft_rows", "ufomi_filter_apply", "ufomi_ifft_rows",
     "ufomi_ramp_filter", "ufomi_bp_params_init", "ufomi_bp_parallel",
-    "ufomi_cone_weight", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
+    "ufomi_cone_weight", "ufomi_transpose_projections", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
     "ufomi_gbp", "ufomi_phantom_ellipses", "ufomi_phantom_ellipsoids", "ufomi_slice_ellipsoids",
     "ufomi_sinogram_ellipses", "ufomi_add_noise", "ufomi_bench_lds_read", "ufomi_bench_copy",
     "ufomi_gbp_projection_rows", "ufomi_project_ellipsoids", "ufomi_device_alloc", "ufomi_device_free",
@@ -109,6 +109,7 @@ def load():
         "ufomi_bp_params_init": [ctypes.POINTER(BpParams)],
         "ufomi_bp_parallel": [vp, vp, ci, ci, ci, ci, csz, ctypes.POINTER(BpParams), vp, csz],
         "ufomi_cone_weight": [vp, vp, vp, ci, ci, ci, dp, dp, dp, dp, dp, ci],
+        "ufomi_transpose_projections": [vp, vp, ci, ci, ci, ci, ci, vp],
         "ufomi_gbp_params_init": [ctypes.POINTER(GbpParams)], "ufomi_gbp_geometry_init": [ctypes.POINTER(GbpGeometry)],
         "ufomi_gbp_volume_shape": [ctypes.POINTER(GbpParams), ci, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                    ctypes.POINTER(ci)],
@@ -322,6 +323,11 @@ class Context:
         arrs = [np.full(n, a[0]) if len(a) == 1 else a for a in arrs]
         check(self.L.ufomi_cone_weight(self.h, _ptr(d_in), _ptr(d_out), W, H, n_proj,
                                        *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrs], n))
+
+    def transpose_projections(self, d_proj, W, H, n, d_sino, first=0, number=None):
+        """Projections [n][H][W] (None: zero rows) -> rows first.. of sinograms [H][number][W]."""
+        check(self.L.ufomi_transpose_projections(self.h, _ptr(d_proj) if d_proj is not None else None, W, H, n,
+                                                 first, number if number is not None else n, _ptr(d_sino)))
 
     def gbp(self, d_proj, W, H, P, geometry, params, d_vol):
         check(self.L.ufomi_gbp(self.h, _ptr(d_proj), W, H, P, ctypes.byref(geometry), ctypes.byref(params),
