@@ -97,8 +97,8 @@ int ufomi_task_process(UfomiTask *task, UfomiBuffer **inputs, UfomiBuffer *outpu
 int ufomi_task_generate(UfomiTask *task, UfomiBuffer *output, UfomiRequisition *requisition);
 /* CPU-mode tasks on host float32 memory, as ufo-core hands a CPU task its
  * buffers' host arrays (ufo_buffer_get_host_array): read's generate
- * (src/ufo-read-task.c:331-353) into `output` of `requisition`, write's
- * process (src/ufo-write-task.c:310-392) of `input`.  Return 1 / 0 as above. */
+ * (src/ufo-read-task.c:341-363) into `output` of `requisition`, write's
+ * process (src/ufo-write-task.c:301-392) of `input`.  Return 1 / 0 as above. */
 int ufomi_task_generate_host(UfomiTask *task, float *output, const UfomiRequisition *requisition);
 int ufomi_task_process_host(UfomiTask *task, const float *input, const UfomiRequisition *input_requisition);
 /* The output layout and storage a task writes (ufo_buffer_set_layout). */

@@ -115,6 +115,24 @@ int ufomi_filter_apply(ufomi_ctx *ctx, const float *d_in, float *d_out, const fl
 int ufomi_ifft_rows(ufomi_ctx *ctx, const float *d_in, int N, int P, int crop, float *d_out);
 
 /*
+ * The fft and ifft tasks in full (src/ufo-fft-task.c:122-346,
+ * src/ufo-ifft-task.c:111-306, src/common/ufo-fft.c:161-354): 1-D, 2-D or 3-D
+ * transforms over the first `dimensions` axes of an item, any size.
+ * ufomi_fft_nd: the input [n2][n1][n0] (real, or interleaved complex when
+ * in_complex) zero-padded to m[0..2] on every axis, forward DFT over the
+ * transformed axes, interleaved complex [m2][m1][m0] out (m[k] >= n[k]).
+ * ufomi_ifft_nd: interleaved complex [m2][m1][m0] in, inverse DFT over the
+ * transformed axes normalised by their sizes, real part cropped to
+ * crop_width x crop_height (0 = no crop) out, [m2][crop_height][crop_width].
+ * Powers of two up to 8192 and other sizes up to 4096 (chirp-z) are built;
+ * either runs one LDS pass per transformed axis.
+ */
+int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, const int n[3], int dimensions, const int m[3],
+                 float *d_out);
+int ufomi_ifft_nd(ufomi_ctx *ctx, const float *d_in, const int m[3], int dimensions, int crop_width, int crop_height,
+                  float *d_out);
+
+/*
  * The fused chain `fft size-x=N ! filter ! ifft crop-width=crop` in one
  * launch: rows [P][W] (row pitch W) -> [P][crop] in out_dtype (f32, f16 or
  * bf16 storage).  d_coefficients: device float[2N] as produced by
@@ -290,8 +308,7 @@ int ufomi_cone_weight(ufomi_ctx *ctx, const float *d_in, float *d_out, int W, in
                       const double *detector_distance, const double *axis_angle_x, int n_param);
 
 /*
- * `transpose-projections` (src/ufo-transpose-projections-task.c:64-94,
- * 120-145): n projections [n][H][W] become rows first .. first + n - 1 of
+ * `transpose-projections` (src/ufo-transpose-projections-task.c:64-150): n projections [n][H][W] become rows first .. first + n - 1 of
  * the H sinograms [H][number][W] at d_sino (row h of projection p is row
  * first + p of sinogram h).  d_proj NULL writes zero rows there instead: the
  * reference's store is zero-initialised, so rows of projections that never

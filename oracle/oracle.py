@@ -93,6 +93,43 @@ def dft_naive(x, direction=-1):
     return o_re + 1j * o_im
 
 
+def _dft_matrix(m, direction):
+    k = np.arange(m)
+    return np.exp(direction * 2j * np.pi * (np.outer(k, k) % m) / m)
+
+
+def fft_nd(x, dimensions, m, in_complex=False):
+    """The fft task (src/ufo-fft-task.c:122-346, src/common/ufo-fft.c:161-354) by
+    definition: x [n2][n1][n0] (real, or complex when in_complex) zero-padded to
+    m = (m0, m1, m2) (x first), DFT exp(-2 pi i k n / M) over the first
+    `dimensions` axes (x first), complex128 [m2][m1][m0]."""
+    x = np.asarray(x, dtype=np.complex128 if in_complex else np.float64)
+    while x.ndim < 3:
+        x = x[None]
+    out = np.zeros((m[2], m[1], m[0]), dtype=np.complex128)
+    out[:x.shape[0], :x.shape[1], :x.shape[2]] = x
+    for a in range(dimensions):
+        ax = 2 - a
+        out = np.moveaxis(np.tensordot(out, _dft_matrix(m[a], -1), axes=([ax], [0])), -1, ax)
+    return out
+
+
+def ifft_nd(spec, dimensions, crop_width=0, crop_height=0):
+    """The ifft task (src/ufo-ifft-task.c:111-306): inverse DFT of complex
+    [m2][m1][m0] over the first `dimensions` axes, divided by their sizes,
+    real part cropped to [m2][crop_height][crop_width] (0 = no crop)."""
+    out = np.asarray(spec, dtype=np.complex128)
+    while out.ndim < 3:
+        out = out[None]
+    for a in range(dimensions):
+        ax = 2 - a
+        n = out.shape[ax]
+        out = np.moveaxis(np.tensordot(out, _dft_matrix(n, +1), axes=([ax], [0])), -1, ax) / n
+    ch = crop_height or out.shape[1]
+    cw = crop_width or out.shape[2]
+    return out.real[:, :ch, :cw]
+
+
 # ------------------------------------------------------------------- filter chain
 
 def filter_coefficients(width, filter="ramp-fromreal", scale=1.0, cutoff=0.5, order=4.0, tau=0.1, theta=1.0):
@@ -212,7 +249,7 @@ def cone_weight_ref32(proj, center_x, center_z, source_distance, detector_distan
 
 def transpose_projections(projs, number):
     """Sinograms [H][number][W] of projections [n][H][W]
-    (src/ufo-transpose-projections-task.c:64-94, 120-145): row h of projection
+    (src/ufo-transpose-projections-task.c:64-150): row h of projection
     p is row p of sinogram h; rows of projections that never arrive stay zero
     (the store is zero-initialised) and projections past `number` are not
     taken (process returns FALSE)."""

@@ -366,3 +366,22 @@ def test_transpose_projections_vs_reference_loop(n, h, w, number):
     # involution when every row is filled: transposing the sinograms back gives the projections
     if n == number:
         np.testing.assert_array_equal(oracle.transpose_projections(got, h), projs)
+
+
+@pytest.mark.parametrize("shape,dims,m", [((3, 5, 7), 1, (7, 5, 3)), ((3, 5, 7), 2, (9, 6, 3)),
+                                          ((2, 3, 4), 3, (5, 3, 4)), ((1, 4, 6), 1, (16, 4, 1))])
+def test_fft_nd_vs_numpy_and_brute_force(shape, dims, m):
+    """fft_nd is the DFT of the zero-padded item: numpy's FFT (an independent routine) and the C
+    brute-force DFT agree with it; ifft_nd inverts it."""
+    x = np.random.default_rng(sum(shape) + dims).standard_normal(shape)
+    got = oracle.fft_nd(x, dims, m)
+    pad = np.zeros((m[2], m[1], m[0]))
+    pad[:shape[0], :shape[1], :shape[2]] = x
+    ref = np.fft.fftn(pad, axes=tuple(range(2, 2 - dims, -1)))
+    np.testing.assert_allclose(got, ref, atol=1e-9)
+    for line in pad.reshape(-1, m[0])[:3]:
+        if dims >= 1:
+            np.testing.assert_allclose(np.fft.fft(line), oracle.dft_naive(line), atol=1e-9)
+    np.testing.assert_allclose(oracle.ifft_nd(got, dims), pad, atol=1e-9)
+    # a planted mistake (conjugated kernel) is caught
+    assert np.abs(np.conj(got) - ref).max() > 1e-3

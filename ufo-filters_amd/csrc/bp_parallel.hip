@@ -32,7 +32,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTile = 32;       // voxels per tile side
-constexpr int kA = 16;          // angles per chunk
 constexpr float kPiF = 3.14159274101257324219f;  // M_PI_F
 
 // Layout of one staged detector entry for S slices in LDS.

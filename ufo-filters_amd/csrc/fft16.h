@@ -26,6 +26,71 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 // multiply by -i (DIR < 0) or +i (DIR > 0)
 template <int DIR> __device__ __forceinline__ float2 rot(float2 a) { return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x); }
 
+// N-point FFT in LDS (x[0..N)), exp(DIR * 2 pi i k n / N), cooperative over the
+// TF threads with local index t.  Caller synchronises before (data ready); the
+// function ends with a barrier (result ready).
+template <int B, int DIR>
+__device__ void lds_fft(float2 *x, int N, int log2N, int t, int TF, const float2 *__restrict__ tw)
+{
+    const int M = UFOMI_FFT_MAX;
+    int Ns = 1;
+    if (log2N & 1) {
+        // radix-2 pass at Ns = 1: no twiddles; y[2j] = a + b, y[2j+1] = a - b.
+        float2 a[2 * B], b[2 * B];
+#pragma unroll
+        for (int q = 0; q < 2 * B; q++) {
+            const int j = t + q * TF;
+            a[q] = x[j];
+            b[q] = x[j + N / 2];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2 * B; q++) {
+            const int j = t + q * TF;
+            x[2 * j] = cadd(a[q], b[q]);
+            x[2 * j + 1] = csub(a[q], b[q]);
+        }
+        __syncthreads();
+        Ns = 2;
+    }
+    for (; Ns < N; Ns *= 4) {
+        float2 v[B][4];
+        const int quarter = N / 4;
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int j = t + q * TF;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                v[q][r] = x[j + r * quarter];
+        }
+        __syncthreads();
+        const int tstep = M / (Ns * 4);  // table stride of one unit of angle 2 pi / (4 Ns)
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int j = t + q * TF;
+            const int k = j & (Ns - 1);
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < 4; r++) {
+                    float2 w = tw[k * r * tstep];
+                    if (DIR > 0)
+                        w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            // 4-point DFT with W4 = exp(DIR * 2 pi i / 4)
+            const float2 s02 = cadd(v[q][0], v[q][2]), d02 = csub(v[q][0], v[q][2]);
+            const float2 s13 = cadd(v[q][1], v[q][3]), d13 = rot<DIR>(csub(v[q][1], v[q][3]));
+            const int idx = (j - k) * 4 + k;
+            x[idx] = cadd(s02, s13);
+            x[idx + Ns] = cadd(d02, d13);
+            x[idx + 2 * Ns] = csub(s02, s13);
+            x[idx + 3 * Ns] = csub(d02, d13);
+        }
+        __syncthreads();
+    }
+}
+
 constexpr float kS2 = 0.70710678118654752440f;   // cos(pi/4)
 constexpr float kC8 = 0.92387953251128675613f;   // cos(pi/8)
 constexpr float kS8 = 0.38268343236508977173f;   // sin(pi/8)

@@ -1,0 +1,399 @@
+// `fft` / `ifft` with the reference's full semantics: 1-D, 2-D and 3-D
+// transforms over the leading axes of an item, any transform size, zero
+// padding to a user size and cropping (src/ufo-fft-task.c:122-346,
+// src/ufo-ifft-task.c:111-306, src/common/ufo-fft.c:161-354).
+//
+// A transform is a sequence of axis passes.  One pass is one kernel launch
+// that moves every line of the array along one axis through LDS:
+//   load the line (zero beyond the input extent) -> FFT in LDS -> store
+// the first n_out outputs.  Sizes that are not a power of two (and powers of
+// two below 16) use Bluestein's algorithm (L. Bluestein, "A linear filtering
+// approach to the computation of the discrete Fourier transform", 1970)
+// fused into the same pass:
+//   X_k = c_k * sum_n (x_n c_n) conj(c_{k-n}),   c_n = exp(dir i pi n^2 / M),
+// a circular convolution of length L >= 2M - 1 done as FFT_L, a product with
+// the precomputed spectrum of conj(c) (1/L folded in), and an inverse FFT_L,
+// all in LDS.  The reference runs the same mathematics as a chain of
+// separate kernels over global memory (spread, modulate, FFT, multiply,
+// IFFT, multiply, pack); here each axis costs one HBM round trip.
+//
+// Lines along axis 0 are contiguous: a block takes G whole lines.  Lines
+// along axes 1 and 2 are strided: a block takes G neighbouring lines, so
+// one row of the block's tile is G consecutive complex values in memory
+// (coalesced), and LDS lines are padded by one element against bank
+// conflicts on the transposed write.
+#include "ufomi_internal.h"
+#include "fft16.h"
+
+#include <cmath>
+#include <complex>
+#include <vector>
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxL = 8192;  // longest line transform (64 KiB of LDS)
+
+using fft16::cmul;
+using fft16::lds_fft;
+
+struct Pass {
+    long in_a, in_b, in_e;     // input offsets of line (a, b) and element stride
+    long out_a, out_b, out_e;  // output offsets and element stride
+    int A, Bn;                 // lines: a in [0, A) (neighbours in memory on strided axes), b in [0, Bn)
+    int n_in, n_out;           // input elements per line (zero beyond), outputs per line
+    int L, log2L;              // transform length in LDS
+    int G, Ls;                 // lines per block, LDS line stride
+    float scale;
+};
+
+__device__ __forceinline__ float2 load_as_complex(const float *p, long o) { return make_float2(p[o], 0.0f); }
+__device__ __forceinline__ float2 load_as_complex(const float2 *p, long o) { return p[o]; }
+
+template <int BB, int DIR, bool CHIRP, typename InT, bool OUT_REAL>
+__global__ __launch_bounds__(kThreads) void axis_dft_kernel(const InT *__restrict__ in, void *__restrict__ out_, Pass g,
+                                                            const float2 *__restrict__ chirp,
+                                                            const float2 *__restrict__ kern,
+                                                            const float2 *__restrict__ tw)
+{
+    extern __shared__ float2 lds[];
+    const int TF = g.L / (4 * BB);
+    const int f = threadIdx.x / TF, t = threadIdx.x - f * TF;
+    const int ablocks = (g.A + g.G - 1) / g.G;
+    const int b = blockIdx.x / ablocks;
+    const int a0 = (blockIdx.x - b * ablocks) * g.G;
+    const bool strided_in = g.in_e != 1, strided_out = g.out_e != 1;
+
+    const int total = g.G * g.L;
+    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+        int gl, i;
+        if (strided_in) {
+            gl = idx % g.G;
+            i = idx / g.G;
+        } else {
+            gl = idx / g.L;
+            i = idx - gl * g.L;
+        }
+        const int a = a0 + gl;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (a < g.A && i < g.n_in) {
+            v = load_as_complex(in, (long) a * g.in_a + (long) b * g.in_b + (long) i * g.in_e);
+            if (CHIRP)
+                v = cmul(v, chirp[i]);
+        }
+        lds[gl * g.Ls + i] = v;
+    }
+    __syncthreads();
+    float2 *x = lds + f * g.Ls;
+    if (CHIRP) {
+        lds_fft<BB, -1>(x, g.L, g.log2L, t, TF, tw);
+        for (int i = t; i < g.L; i += TF)
+            x[i] = cmul(x[i], kern[i]);
+        __syncthreads();
+        lds_fft<BB, +1>(x, g.L, g.log2L, t, TF, tw);
+    } else {
+        lds_fft<BB, DIR>(x, g.L, g.log2L, t, TF, tw);
+    }
+
+    const int total_out = g.G * g.n_out;
+    for (int idx = threadIdx.x; idx < total_out; idx += kThreads) {
+        int gl, i;
+        if (strided_out) {
+            gl = idx % g.G;
+            i = idx / g.G;
+        } else {
+            gl = idx / g.n_out;
+            i = idx - gl * g.n_out;
+        }
+        const int a = a0 + gl;
+        if (a >= g.A)
+            continue;
+        float2 y = lds[gl * g.Ls + i];
+        if (CHIRP)
+            y = cmul(y, chirp[i]);
+        const long o = (long) a * g.out_a + (long) b * g.out_b + (long) i * g.out_e;
+        if (OUT_REAL)
+            ((float *) out_)[o] = y.x * g.scale;
+        else
+            ((float2 *) out_)[o] = make_float2(y.x * g.scale, y.y * g.scale);
+    }
+}
+
+bool pow2(long x) { return x > 0 && (x & (x - 1)) == 0; }
+
+long next_pow2(long x)
+{
+    long p = 1;
+    while (p < x)
+        p <<= 1;
+    return p;
+}
+
+// Transform length for a DFT of size M: M itself for powers of two >= 16,
+// else the Bluestein length.
+int line_length(int M, bool *chirp)
+{
+    *chirp = !(pow2(M) && M >= 16);
+    return *chirp ? (int) std::max(16L, next_pow2(2L * M - 1)) : M;
+}
+
+// Radix-2 FFT in double on the host (for the Bluestein kernel spectrum).
+void host_fft(std::vector<std::complex<double>> &a, int dir)
+{
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; i++) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1)
+            j ^= bit;
+        j ^= bit;
+        if (i < j)
+            std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; k++) {
+                const std::complex<double> w = std::polar(1.0, dir * 2.0 * M_PI * (double) k / (double) len);
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+// chirp[n] = exp(dir i pi n^2 / M) (n^2 reduced mod 2M before the angle),
+// kern = FFT_L(conj(chirp) placed at n and L - n) / L.
+void bluestein_tables(int M, int L, int dir, float2 *chirp, float2 *kern)
+{
+    std::vector<std::complex<double>> c(M), b(L, 0.0);
+    for (int n = 0; n < M; n++) {
+        const long long q = ((long long) n * n) % (2LL * M);
+        c[n] = std::polar(1.0, dir * M_PI * (double) q / (double) M);
+        chirp[n] = make_float2((float) c[n].real(), (float) c[n].imag());
+    }
+    b[0] = std::conj(c[0]);
+    for (int n = 1; n < M; n++)
+        b[n] = b[L - n] = std::conj(c[n]);
+    host_fft(b, -1);
+    for (int k = 0; k < L; k++)
+        kern[k] = make_float2((float) (b[k].real() / L), (float) (b[k].imag() / L));
+}
+
+struct PassSpec {
+    Pass g;
+    int BB;
+    bool chirp;
+    int M;
+    size_t table_off;  // float2 offset of this pass's chirp table in the table block (kern follows)
+};
+
+// Lines along `axis` of a source array with extents S (x fastest) into a
+// destination with extents D; E[] gives how many lines exist along each other
+// axis (coordinates beyond E are not computed).
+int make_pass(int axis, const int S[3], const int D[3], const int E[3], int n_in, int n_out, int M, PassSpec *ps)
+{
+    Pass &g = ps->g;
+    bool chirp;
+    const int L = line_length(M, &chirp);
+    if (L > kMaxL)
+        return ufomi_fail(UFOMI_ERR_UNSUPPORTED,
+                          "transform size %d is not built (powers of two up to %d, other sizes up to %d)", M, kMaxL,
+                          kMaxL / 2);
+    ps->chirp = chirp;
+    ps->M = M;
+    g.L = L;
+    g.log2L = ufomi_log2(L);
+    ps->BB = L >= 8192 ? 8 : std::min(4, L / 16);
+    g.G = 1024 * ps->BB / L;
+    g.Ls = g.G > 1 ? L + 1 : L;
+    g.n_in = n_in;
+    g.n_out = n_out;
+    g.scale = 1.0f;
+    const long s1 = S[0], s2 = (long) S[0] * S[1], d1 = D[0], d2 = (long) D[0] * D[1];
+    if (axis == 0) {
+        g.A = E[1];
+        g.Bn = E[2];
+        g.in_e = 1, g.in_a = s1, g.in_b = s2;
+        g.out_e = 1, g.out_a = d1, g.out_b = d2;
+    } else if (axis == 1) {
+        g.A = E[0];
+        g.Bn = E[2];
+        g.in_e = s1, g.in_a = 1, g.in_b = s2;
+        g.out_e = d1, g.out_a = 1, g.out_b = d2;
+    } else {
+        if (S[0] != D[0] || S[1] != D[1] || E[0] != S[0] || E[1] != S[1])
+            return ufomi_fail(UFOMI_ERR_ARG, "internal: axis-2 pass needs whole planes");
+        g.A = E[0] * E[1];
+        g.Bn = 1;
+        g.in_e = s2, g.in_a = 1, g.in_b = 0;
+        g.out_e = d2, g.out_a = 1, g.out_b = 0;
+    }
+    const long blocks = (long) g.Bn * ((g.A + g.G - 1) / g.G);
+    if (blocks <= 0 || blocks >= (1L << 31))
+        return ufomi_fail(UFOMI_ERR_ARG, "transform of %d x %d lines does not fit one launch", g.A, g.Bn);
+    return UFOMI_OK;
+}
+
+template <int DIR, bool CHIRP, typename InT, bool OUT_REAL>
+int launch_bb(hipStream_t s, const PassSpec &ps, const void *in, void *out, const float2 *chirp, const float2 *kern,
+              const float2 *tw)
+{
+    const Pass &g = ps.g;
+    const unsigned blocks = (unsigned) ((long) g.Bn * ((g.A + g.G - 1) / g.G));
+    const size_t lds = (size_t) g.G * g.Ls * sizeof(float2);
+    switch (ps.BB) {
+#define UFOMI_AXIS_CASE(BBV)                                                                                       \
+    case BBV:                                                                                                      \
+        hipLaunchKernelGGL((axis_dft_kernel<BBV, DIR, CHIRP, InT, OUT_REAL>), dim3(blocks), dim3(kThreads), lds, s, \
+                           (const InT *) in, out, g, chirp, kern, tw);                                             \
+        break;
+        UFOMI_AXIS_CASE(1)
+        UFOMI_AXIS_CASE(2)
+        UFOMI_AXIS_CASE(4)
+        UFOMI_AXIS_CASE(8)
+#undef UFOMI_AXIS_CASE
+    default: return ufomi_fail(UFOMI_ERR_ARG, "internal: no axis kernel for %d butterflies", ps.BB);
+    }
+    UFOMI_CHECK_LAUNCH();
+    return UFOMI_OK;
+}
+
+template <int DIR, typename InT, bool OUT_REAL>
+int launch_pass(hipStream_t s, const PassSpec &ps, const void *in, void *out, const float2 *tables, const float2 *tw)
+{
+    const float2 *chirp = tables + ps.table_off, *kern = chirp + ps.M;
+    return ps.chirp ? launch_bb<DIR, true, InT, OUT_REAL>(s, ps, in, out, chirp, kern, tw)
+                    : launch_bb<DIR, false, InT, OUT_REAL>(s, ps, in, out, nullptr, nullptr, tw);
+}
+
+// Scratch: two complex intermediates of `tmp_elems` and the Bluestein tables of every pass.
+int setup_scratch(ufomi_ctx *ctx, PassSpec *ps, int np, int dir, size_t tmp_elems, float2 **tmp0, float2 **tmp1,
+                  float2 **tables)
+{
+    size_t n_tab = 0;
+    for (int p = 0; p < np; p++) {
+        ps[p].table_off = n_tab;
+        if (ps[p].chirp)
+            n_tab += (size_t) ps[p].M + ps[p].g.L;
+    }
+    const size_t tmp_bytes = ((tmp_elems * sizeof(float2) + 255) / 256) * 256;
+    void *base = nullptr;
+    int rc = ufomi_scratch(ctx, 2 * tmp_bytes + n_tab * sizeof(float2) + 256, &base);
+    if (rc) return rc;
+    *tmp0 = (float2 *) base;
+    *tmp1 = (float2 *) ((char *) base + tmp_bytes);
+    *tables = (float2 *) ((char *) base + 2 * tmp_bytes);
+    if (n_tab) {
+        std::vector<float2> h(n_tab);
+        for (int p = 0; p < np; p++)
+            if (ps[p].chirp)
+                bluestein_tables(ps[p].M, ps[p].g.L, dir, h.data() + ps[p].table_off,
+                                 h.data() + ps[p].table_off + ps[p].M);
+        if ((rc = ufomi_upload(ctx, *tables, h.data(), n_tab * sizeof(float2)))) return rc;
+    }
+    return UFOMI_OK;
+}
+
+}  // namespace
+
+extern "C" int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, const int *n, int dimensions,
+                            const int *m, float *d_out)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!d_in || !d_out || !n || !m || dimensions < 1 || dimensions > 3)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid fft arguments");
+    for (int a = 0; a < 3; a++) {
+        if (n[a] < 1 || m[a] < n[a])
+            return ufomi_fail(n[a] < 1 ? UFOMI_ERR_ARG : UFOMI_ERR_SIZE,
+                              "Specified size must be greater than or equal to input size");
+    }
+    // the radix-16 row kernels for plain 1-D power-of-two transforms of real rows
+    if (dimensions == 1 && !in_complex && pow2(m[0]) && m[0] >= 16 && m[0] <= kMaxL && m[1] == n[1] && m[2] == n[2])
+        return ufomi_fft_rows(ctx, d_in, n[0], n[1] * n[2], m[0], d_out);
+
+    // passes from axis 0 up; the last one writes the output extents on every axis
+    PassSpec ps[3];
+    int S[3] = {n[0], n[1], n[2]};
+    size_t tmp_elems = 1;
+    for (int a = 0; a < dimensions; a++) {
+        int D[3] = {S[0], S[1], S[2]};
+        D[a] = m[a];
+        if (a == dimensions - 1)
+            for (int k = 0; k < 3; k++)
+                D[k] = m[k];  // batch axes padded with zeros in the output
+        if ((rc = make_pass(a, S, D, S, n[a], m[a], m[a], &ps[a]))) return rc;
+        if (a < dimensions - 1)
+            tmp_elems = std::max(tmp_elems, (size_t) D[0] * D[1] * D[2]);
+        S[a] = m[a];
+    }
+    float2 *tmp[2], *tables;
+    if ((rc = setup_scratch(ctx, ps, dimensions, -1, tmp_elems, &tmp[0], &tmp[1], &tables))) return rc;
+    bool padded_batch = false;
+    for (int k = dimensions; k < 3; k++)
+        padded_batch |= m[k] > n[k];
+    if (padded_batch)
+        UFOMI_HIP(hipMemsetAsync(d_out, 0, (size_t) m[0] * m[1] * m[2] * sizeof(float2), ctx->stream));
+    const void *src = d_in;
+    for (int a = 0; a < dimensions; a++) {
+        void *dst = a == dimensions - 1 ? (void *) d_out : (void *) tmp[a & 1];
+        if (a == 0 && !in_complex)
+            rc = launch_pass<-1, float, false>(ctx->stream, ps[a], src, dst, tables, ctx->twiddles);
+        else
+            rc = launch_pass<-1, float2, false>(ctx->stream, ps[a], src, dst, tables, ctx->twiddles);
+        if (rc) return rc;
+        src = dst;
+    }
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_ifft_nd(ufomi_ctx *ctx, const float *d_in, const int *m, int dimensions, int crop_width,
+                             int crop_height, float *d_out)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!d_in || !d_out || !m || dimensions < 1 || dimensions > 3 || m[0] < 1 || m[1] < 1 || m[2] < 1 ||
+        crop_width < 0 || crop_height < 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid ifft arguments");
+    if (crop_width > m[0] || crop_height > m[1])
+        return ufomi_fail(UFOMI_ERR_SIZE, "Cropped size must be less than or equal to input size");
+    const int cw = crop_width ? crop_width : m[0], ch = crop_height ? crop_height : m[1];
+    if (dimensions == 1 && pow2(m[0]) && m[0] >= 16 && m[0] <= kMaxL && (ch == m[1] || m[2] == 1))
+        return ufomi_ifft_rows(ctx, d_in, m[0], ch * m[2], cw, d_out);
+
+    // passes from the highest transformed axis down to axis 0, which writes the real, cropped rows
+    PassSpec ps[3];
+    int S[3] = {m[0], m[1], m[2]};
+    size_t tmp_elems = 1;
+    double norm = 1.0;
+    int np = 0;
+    for (int a = dimensions - 1; a >= 0; a--, np++) {
+        const int n_out = a == 0 ? cw : (a == 1 ? ch : m[2]);
+        int D[3] = {S[0], S[1], S[2]};
+        D[a] = n_out;
+        int E[3] = {S[0], S[1], S[2]};
+        if (a == 0)
+            E[1] = ch;  // rows cropped away are never transformed
+        if (a == 0)
+            D[1] = ch;
+        if ((rc = make_pass(a, S, D, E, m[a], n_out, m[a], &ps[np]))) return rc;
+        norm *= m[a];
+        if (a > 0)
+            tmp_elems = std::max(tmp_elems, (size_t) D[0] * D[1] * D[2]);
+        for (int k = 0; k < 3; k++)
+            S[k] = D[k];
+    }
+    ps[np - 1].g.scale = (float) (1.0 / norm);
+    float2 *tmp[2], *tables;
+    if ((rc = setup_scratch(ctx, ps, np, +1, tmp_elems, &tmp[0], &tmp[1], &tables))) return rc;
+    const void *src = d_in;
+    for (int p = 0; p < np; p++) {
+        if (p == np - 1) {
+            rc = launch_pass<+1, float2, true>(ctx->stream, ps[p], src, d_out, tables, ctx->twiddles);
+        } else {
+            rc = launch_pass<+1, float2, false>(ctx->stream, ps[p], src, tmp[p & 1], tables, ctx->twiddles);
+            src = tmp[p & 1];
+        }
+        if (rc) return rc;
+    }
+    return UFOMI_OK;
+}

@@ -1,6 +1,6 @@
 // `transpose-projections`: projections [n][H][W] -> sinograms [H][number][W].
 //
-// src/ufo-transpose-projections-task.c:64-94 copies row h of projection p to
+// src/ufo-transpose-projections-task.c:64-99 copies row h of projection p to
 // row p of sinogram h; rows of projections that never arrive stay zero (the
 // store is g_malloc0'd at :143).  Rows stay contiguous under this
 // permutation, so the kernel moves whole rows: a block owns a run of output

@@ -177,8 +177,14 @@ static int tiff_open(ImgFile *f, char *err, size_t errlen)
     const uint64_t fsz = (uint64_t) file_size(f->fp);
     const int es = f->big ? 20 : 12, cn = f->big ? 8 : 2, on = f->big ? 8 : 4;
     while (ofs > 0 && ofs < fsz) {
-        if (f->n_images && (long) ofs <= f->ifd[f->n_images - 1])
-            break;  /* offsets must grow: no cycles */
+        if (f->n_images && (long) ofs <= f->ifd[f->n_images - 1]) {
+            /* pages out of file order are valid; a page seen before is a cycle */
+            int seen = 0;
+            for (unsigned k = 0; k < f->n_images && !seen; k++)
+                seen = f->ifd[k] == (long) ofs;
+            if (seen)
+                break;
+        }
         if (f->n_images == cap) {
             cap *= 2;
             f->ifd = realloc(f->ifd, cap * sizeof(long));
@@ -277,7 +283,7 @@ static int edf_open(ImgFile *f, char *err, size_t errlen)
     }
     const long header = close - h + 2;
     free(h);
-    /* the reference's DataType map (ufo-edf-reader.c:182-210) plus the other ESRF names */
+    /* the reference's DataType map (ufo-edf-reader.c:178-199) plus the other ESRF names */
     static const struct { const char *name; int kind, bytes; } map[] = {
         {"UnsignedShort", 'u', 2}, {"SignedInteger", 's', 4}, {"UnsignedLong", 'u', 4}, {"Float", 'f', 4},
         {"FloatValue", 'f', 4}, {"UnsignedByte", 'u', 1}, {"SignedByte", 's', 1}, {"SignedShort", 's', 2},

@@ -281,7 +281,8 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
         UfomiBuffer in0 = item_view(&cur, 0), *ins[1] = {&in0};
         UfomiRequisition req;
 
-        if (is(t, "fft") && s + 2 < last && is(pl->tasks[s + 1], "filter") && is(pl->tasks[s + 2], "ifft")) {
+        if (is(t, "fft") && s + 2 < last && is(pl->tasks[s + 1], "filter") && is(pl->tasks[s + 2], "ifft") &&
+            fft_chain_fusable(t, pl->tasks[s + 2], &in0)) {
             /* fused fft ! filter ! ifft: requisitions of the three tasks, one launch */
             UfomiRequisition r_fft, r_filt, r_ifft;
             UfomiBuffer spec = in0, filt;
@@ -331,7 +332,7 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
             t->num_processed += (unsigned) cur.n;
         } else if (is(t, "transpose-projections")) {
             /* all projections of the batch in one launch, zero rows for missing ones; the
-             * requisition of ufo-transpose-projections-task.c:113-146 without the task's own store */
+             * requisition of ufo-transpose-projections-task.c:128-150 without the task's own store */
             req.n_dims = 2;
             req.dims[0] = cur.buf.req.dims[0];
             req.dims[1] = (size_t) t->v[0].num;

@@ -366,38 +366,46 @@ static size_t closest_smaller_pow2(size_t x)
 static int is_pow2(size_t x) { return x && !(x & (x - 1)); }
 
 typedef struct {
-    size_t n;     /* transform size */
-    size_t rows;  /* batch */
+    int n[3];      /* input extents (complex elements along x for complex input) */
+    int m[3];      /* output extents */
+    int in_complex;
 } FftPriv;
 
+/* get_requisition (ufo-fft-task.c:122-199): per axis the user size (not
+ * smaller than the input), else the input size, rounded up to a power of two
+ * on transformed axes when auto-zeropadding; the output is that size,
+ * interleaved complex along x. */
 static int fft_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
 {
     FftPriv *p = t->priv;
     const UfomiRequisition *ir = &in[0]->req;
-    if ((int) t->v[FFT_DIMS].num != 1)
-        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
-                          "fft: only dimensions=1 is built (2-D and 3-D transforms are not)");
-    if (in[0]->layout == UFOMI_LAYOUT_COMPLEX_INTERLEAVED)
-        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
-                          "fft: complex input is not built");
-    const size_t user = (size_t) t->v[FFT_SX].num;
-    const size_t w = ir->dims[0];
-    /* ufo-fft-task.c:139-156 */
-    if (user != 0 && user < w)
-        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
-                          "Specified size must be greater than or equal to input size");
-    size_t n = user ? user : w;
-    if (!user && t->v[FFT_ZEROPAD].num != 0)
-        n = 2 * closest_smaller_pow2(n - 1);
-    if (n != 2 * closest_smaller_pow2(n - 1) || !is_pow2(n))
-        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
-                          "fft: size %zu is not a power of two; the chirp-z path is not built", n);
-    p->n = n;
-    p->rows = (ir->n_dims >= 2 ? ir->dims[1] : 1) * (ir->n_dims == 3 ? ir->dims[2] : 1);
+    const int dims = (int) t->v[FFT_DIMS].num;
+    const int cplx = in[0]->layout == UFOMI_LAYOUT_COMPLEX_INTERLEAVED;
+    const double user[3] = {t->v[FFT_SX].num, t->v[FFT_SY].num, t->v[FFT_SZ].num};
+    for (int i = 0; i < 3; i++) {
+        const size_t in_dim = i < (int) ir->n_dims ? ir->dims[i] : 1;
+        size_t size = in_dim;
+        if (i < (int) ir->n_dims) {
+            if (user[i] != 0 && user[i] < (double) in_dim)
+                return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
+                                  "Specified size must be greater than or equal to input size");
+            if (user[i] == 0) {
+                if (i == 0 && cplx)
+                    size >>= 1;
+                if (t->v[FFT_ZEROPAD].num != 0 && i <= dims - 1)
+                    size = 2 * closest_smaller_pow2(size - 1);
+            } else {
+                size = (size_t) user[i];
+            }
+        }
+        p->m[i] = (int) size;
+        p->n[i] = (int) (i == 0 && cplx ? in_dim / 2 : in_dim);
+    }
+    p->in_complex = cplx;
     req->n_dims = ir->n_dims;
-    req->dims[0] = 2 * n;
-    req->dims[1] = ir->n_dims >= 2 ? ir->dims[1] : 1;
-    req->dims[2] = ir->n_dims == 3 ? ir->dims[2] : 1;
+    req->dims[0] = 2 * (size_t) p->m[0];
+    req->dims[1] = p->m[1];
+    req->dims[2] = p->m[2];
     return UFOMI_OK;
 }
 
@@ -405,7 +413,7 @@ static int fft_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRe
 {
     (void) req;
     FftPriv *p = t->priv;
-    int rc = ufomi_fft_rows(t->ctx, in[0]->data, (int) in[0]->req.dims[0], (int) p->rows, (int) p->n, out->data);
+    int rc = ufomi_fft_nd(t->ctx, in[0]->data, p->in_complex, p->n, (int) t->v[FFT_DIMS].num, p->m, out->data);
     if (rc)
         return process_fail(t, rc, "fft: %s", ufomi_last_error());
     out->layout = UFOMI_LAYOUT_COMPLEX_INTERLEAVED;
@@ -413,41 +421,36 @@ static int fft_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRe
 }
 
 typedef struct {
-    size_t n, rows, crop;
+    int m[3];      /* complex input extents */
+    int cw, ch;    /* crops (0: none) */
 } IfftPriv;
 
+/* get_requisition (ufo-ifft-task.c:111-185): the input must be complex; the
+ * output is the real part, crop-width x crop-height when set. */
 static int ifft_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
 {
     IfftPriv *p = t->priv;
     const UfomiRequisition *ir = &in[0]->req;
-    /* ufo-ifft-task.c:121-125 */
+    /* :121-125 */
     if (in[0]->layout != UFOMI_LAYOUT_COMPLEX_INTERLEAVED)
         return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_LAYOUT, "ifft input must be complex");
-    if ((int) t->v[IFFT_DIMS].num != 1)
-        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
-                          "ifft: only dimensions=1 is built");
     const size_t cw = (size_t) t->v[IFFT_CW].num, ch = (size_t) t->v[IFFT_CH].num;
-    /* :130-134 compares the user size with the input requisition (complex width) */
+    /* :133-138 compares the user size with the input requisition (the complex width along x) */
     if ((cw && cw > ir->dims[0]) || (ch && ir->n_dims >= 2 && ch > ir->dims[1]))
         return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
                           "Cropped size must be less than or equal to input size");
-    const size_t n = ir->dims[0] / 2;
-    if (!is_pow2(n))
-        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
-                          "ifft: size %zu is not a power of two; the chirp-z path is not built", n);
-    if (ch && ch != (ir->n_dims >= 2 ? ir->dims[1] : 1))
-        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
-                          "ifft: crop-height needs dimensions=2, which is not built");
-    if (cw > n)
+    p->m[0] = (int) (ir->dims[0] / 2);
+    p->m[1] = ir->n_dims >= 2 ? (int) ir->dims[1] : 1;
+    p->m[2] = ir->n_dims == 3 ? (int) ir->dims[2] : 1;
+    if (cw > (size_t) p->m[0] || ch > (size_t) p->m[1])
         return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
                           "Cropped size must be less than or equal to input size");
-    p->n = n;
-    p->rows = (ir->n_dims >= 2 ? ir->dims[1] : 1) * (ir->n_dims == 3 ? ir->dims[2] : 1);
-    p->crop = cw ? cw : n;
+    p->cw = (int) cw;
+    p->ch = (int) ch;
     req->n_dims = ir->n_dims;
-    req->dims[0] = p->crop;
-    req->dims[1] = ir->n_dims >= 2 ? ir->dims[1] : 1;
-    req->dims[2] = ir->n_dims == 3 ? ir->dims[2] : 1;
+    req->dims[0] = cw ? cw : (size_t) p->m[0];
+    req->dims[1] = ch ? ch : (size_t) p->m[1];
+    req->dims[2] = p->m[2];
     return UFOMI_OK;
 }
 
@@ -455,11 +458,23 @@ static int ifft_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiR
 {
     (void) req;
     IfftPriv *p = t->priv;
-    int rc = ufomi_ifft_rows(t->ctx, in[0]->data, (int) p->n, (int) p->rows, (int) p->crop, out->data);
+    int rc = ufomi_ifft_nd(t->ctx, in[0]->data, p->m, (int) t->v[IFFT_DIMS].num, p->cw, p->ch, out->data);
     if (rc)
         return process_fail(t, rc, "ifft: %s", ufomi_last_error());
     out->layout = UFOMI_LAYOUT_REAL;
     return 1;
+}
+
+/* The chain fft ! filter ! ifft runs as the fused ramp kernel when it is a
+ * 1-D transform of real rows with no padding or crop across rows. */
+int fft_chain_fusable(const UfomiTask *fft, const UfomiTask *ifft, const UfomiBuffer *in)
+{
+    const UfomiRequisition *r = &in->req;
+    const double sy = fft->v[FFT_SY].num, sz = fft->v[FFT_SZ].num;
+    return in->layout == UFOMI_LAYOUT_REAL && in->dtype == UFOMI_F32 && (int) fft->v[FFT_DIMS].num == 1 &&
+           (int) ifft->v[IFFT_DIMS].num == 1 && (sy == 0 || r->n_dims < 2 || sy == (double) r->dims[1]) &&
+           (sz == 0 || r->n_dims < 3 || sz == (double) r->dims[2]) &&
+           (ifft->v[IFFT_CH].num == 0 || r->n_dims < 2 || ifft->v[IFFT_CH].num == (double) r->dims[1]);
 }
 
 /* ---------------------------------------------------------------- filter */
@@ -789,7 +804,7 @@ static void gbp_finalize(UfomiTask *t)
 
 /* --------------------------------------------------- transpose-projections */
 
-/* transpose-projections (ufo-transpose-projections-task.c:245-251) */
+/* transpose-projections (ufo-transpose-projections-task.c:256-261) */
 static const PropSpec tp_props[] = {
     {"number", PT_UINT, 1, U32MAX, 1, NULL, 0, NULL},
 };
@@ -816,7 +831,7 @@ static int tp_setup(UfomiTask *t, UfomiError *e)
     return UFOMI_OK;
 }
 
-/* get_requisition (:113-146): the first projection fixes width and height;
+/* get_requisition (:128-150): the first projection fixes width and height;
  * the output is one sinogram of `number` rows */
 static int tp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
 {
@@ -846,7 +861,7 @@ static int tp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req,
     return UFOMI_OK;
 }
 
-/* process (:64-94): the rows of one projection into the sinograms; a
+/* process (:64-99): the rows of one projection into the sinograms; a
  * projection beyond `number` ends the input (process returns FALSE) */
 static int tp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
 {
@@ -865,7 +880,7 @@ static int tp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiReq
     return 1;
 }
 
-/* generate (:96-111): sinogram after sinogram; rows of projections that
+/* generate (:101-126): sinogram after sinogram; rows of projections that
  * never arrived are zero */
 static int tp_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
 {
@@ -896,7 +911,7 @@ static void tp_finalize(UfomiTask *t)
 
 /* ------------------------------------------------------------------- read */
 
-/* read (ufo-read-task.c:551-663; init :668-690 sets path "."; type_values :53-63
+/* read (ufo-read-task.c:551-663; init :671-700 sets path "."; type_values :53-63
  * without hdf5, which needs libhdf5) */
 #define ULMAX 18446744073709551615.0
 static const char *const read_type_nicks[] = {"edf", "raw", "tiff", "unspecified", NULL};
@@ -937,7 +952,7 @@ typedef struct {
 
 static unsigned uprop(const UfomiTask *t, int i) { return (unsigned) t->v[i].num; }
 
-/* get_reader (ufo-read-task.c:213-233): the extension or `type', TIFF first, then EDF, then raw */
+/* get_reader (ufo-read-task.c:225-245): the extension or `type', TIFF first, then EDF, then raw */
 static int read_format(const UfomiTask *t, const char *name)
 {
     const int ext = img_format_of(name), type = (int) t->v[RD_TYPE].num;
@@ -964,7 +979,7 @@ static void read_reset(ReadPriv *p)
 
 static int cmp_names(const void *a, const void *b) { return strcmp(*(char *const *) a, *(char *const *) b); }
 
-/* read_filenames + setup (ufo-read-task.c:139-211): a regular file, or a glob
+/* read_filenames + setup (ufo-read-task.c:139-223): a regular file, or a glob
  * (a directory means every file in it), filtered by reader, sorted */
 static int read_setup(UfomiTask *t, UfomiError *e)
 {
@@ -1003,7 +1018,7 @@ static int read_setup(UfomiTask *t, UfomiError *e)
     return UFOMI_OK;
 }
 
-/* get_requisition (ufo-read-task.c:235-309): the next file with an image at
+/* get_requisition (ufo-read-task.c:247-325): the next file with an image at
  * the running image-start, the vertical ROI checked against it */
 static int read_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
 {
@@ -1053,8 +1068,8 @@ static int read_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *re
     return UFOMI_OK;
 }
 
-/* generate (ufo-read-task.c:331-353) with the readers' image stepping
- * (ufo-tiff-reader.c:186-216, ufo-edf-reader.c:160-165, ufo-raw-reader.c:150-155):
+/* generate (ufo-read-task.c:341-363) with the readers' image stepping
+ * (ufo-tiff-reader.c:188-216, ufo-edf-reader.c:160-166, ufo-raw-reader.c:150-156):
  * one image read, then image-step images consumed or the rest of the file */
 int read_task_generate_host(UfomiTask *t, float *dst, const UfomiRequisition *req)
 {
@@ -1110,7 +1125,7 @@ static void read_finalize(UfomiTask *t)
 
 /* ------------------------------------------------------------------ write */
 
-/* write (ufo-write-task.c:606-690, init :693-718); jpeg-quality exists only
+/* write (ufo-write-task.c:606-690, init :694-720); jpeg-quality exists only
  * with libjpeg and HDF5 output needs libhdf5, neither of which is built here */
 static const PropSpec write_props[] = {
     {"filename", PT_STRING, 0, 0, 0, NULL, 0, ""},
@@ -1197,7 +1212,7 @@ static int mkdir_parents(const char *dir)
     return !exists(tmp) && mkdir(tmp, 0755) && errno != EEXIST ? -1 : 0;
 }
 
-/* setup (ufo-write-task.c:165-262) */
+/* setup (ufo-write-task.c:167-278) */
 static int write_setup(UfomiTask *t, UfomiError *e)
 {
     WritePriv *p = t->priv;
@@ -1257,7 +1272,7 @@ static int write_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *r
     return UFOMI_OK;
 }
 
-/* process (ufo-write-task.c:310-392): each plane of the item to the open file,
+/* process (ufo-write-task.c:301-392): each plane of the item to the open file,
  * a new file (counter += counter-step) once bytes-per-file would be exceeded */
 int write_task_write_host(UfomiTask *t, const float *src, const UfomiRequisition *req)
 {

@@ -54,6 +54,7 @@ int task_error(UfomiError *e, int code, int status, const char *fmt, ...);
 int ufomi_status_fail(int status, const char *fmt, ...);
 
 /* Helpers the pipeline uses to fuse fft ! filter ! ifft and batch backproject. */
+int fft_chain_fusable(const UfomiTask *fft, const UfomiTask *ifft, const UfomiBuffer *in);
 int filter_task_coefficients(UfomiTask *t, size_t width, UfomiError *e);
 const float *filter_task_device_coefficients(const UfomiTask *t);
 void backproject_task_params(const UfomiTask *t, ufomi_bp_params *bp);

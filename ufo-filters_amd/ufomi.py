@@ -31,6 +31,7 @@ ADDRESSING = {"none": 0, "clamp_to_edge": 1, "clamp": 2, "repeat": 3, "mirrored_
 EXPORTS = [
     "ufomi_create", "ufomi_destroy", "ufomi_set_stream", "ufomi_synchronize", "ufomi_last_error",
     "ufomi_version", "ufomi_filter_coefficients", "ufomi_fft_rows", "ufomi_filter_apply", "ufomi_ifft_rows",
+    "ufomi_fft_nd", "ufomi_ifft_nd",
     "ufomi_ramp_filter", "ufomi_bp_params_init", "ufomi_bp_parallel",
     "ufomi_cone_weight", "ufomi_transpose_projections", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
     "ufomi_gbp", "ufomi_phantom_ellipses", "ufomi_phantom_ellipsoids", "ufomi_slice_ellipsoids",
@@ -105,6 +106,7 @@ def load():
         "ufomi_synchronize": [vp], "ufomi_last_error": [], "ufomi_version": [],
         "ufomi_filter_coefficients": [ci, cu, cf, cf, cf, cf, cf, vp],
         "ufomi_fft_rows": [vp, vp, ci, ci, ci, vp], "ufomi_filter_apply": [vp, vp, vp, vp, ci, ci],
+        "ufomi_fft_nd": [vp, vp, ci, vp, ci, vp, vp], "ufomi_ifft_nd": [vp, vp, vp, ci, ci, ci, vp],
         "ufomi_ifft_rows": [vp, vp, ci, ci, ci, vp], "ufomi_ramp_filter": [vp, vp, ci, ci, ci, vp, ci, vp, ci],
         "ufomi_bp_params_init": [ctypes.POINTER(BpParams)],
         "ufomi_bp_parallel": [vp, vp, ci, ci, ci, ci, csz, ctypes.POINTER(BpParams), vp, csz],
@@ -295,6 +297,15 @@ class Context:
     # -- ramp filter chain
     def fft_rows(self, d_in, W, P, N, d_out):
         check(self.L.ufomi_fft_rows(self.h, _ptr(d_in), W, P, N, _ptr(d_out)))
+
+    def fft_nd(self, d_in, n, dimensions, m, d_out, in_complex=False):
+        """Forward DFT over the first `dimensions` axes of [n2][n1][n0] zero-padded to m (x first)."""
+        ni, mi = (ctypes.c_int * 3)(*n), (ctypes.c_int * 3)(*m)
+        check(self.L.ufomi_fft_nd(self.h, _ptr(d_in), int(in_complex), ni, dimensions, mi, _ptr(d_out)))
+
+    def ifft_nd(self, d_in, m, dimensions, d_out, crop_width=0, crop_height=0):
+        mi = (ctypes.c_int * 3)(*m)
+        check(self.L.ufomi_ifft_nd(self.h, _ptr(d_in), mi, dimensions, crop_width, crop_height, _ptr(d_out)))
 
     def filter_apply(self, d_in, d_out, d_coef, width, P):
         check(self.L.ufomi_filter_apply(self.h, _ptr(d_in), _ptr(d_out), _ptr(d_coef), width, P))
