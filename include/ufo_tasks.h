@@ -64,6 +64,8 @@ typedef struct UfomiTask UfomiTask;
 
 /* Task names: "fft", "filter", "ifft", "backproject", "general-backproject",
  * "cone-beam-projection-weight" (libufofilter<name>.so in the reference), and
+ * "transpose-projections", "stacked-backproject", "flat-field-correct" (three
+ * inputs), "average", "center-of-rotation", and
  * the file tasks "read" (src/ufo-read-task.c: a generator, TIFF / BigTIFF,
  * EDF and raw files) and "write" (src/ufo-write-task.c: a sink, TIFF / BigTIFF
  * and raw).  Returns NULL for an unknown name.
@@ -97,8 +99,10 @@ int ufomi_task_process(UfomiTask *task, UfomiBuffer **inputs, UfomiBuffer *outpu
 int ufomi_task_generate(UfomiTask *task, UfomiBuffer *output, UfomiRequisition *requisition);
 /* CPU-mode tasks on host float32 memory, as ufo-core hands a CPU task its
  * buffers' host arrays (ufo_buffer_get_host_array): read's generate
- * (src/ufo-read-task.c:341-363) into `output` of `requisition`, write's
- * process (src/ufo-write-task.c:301-392) of `input`.  Return 1 / 0 as above. */
+ * (src/ufo-read-task.c:341-363) into `output` of `requisition`; the process
+ * of write (src/ufo-write-task.c:301-392) and of center-of-rotation
+ * (src/ufo-center-of-rotation-task.c:111-166) on `input`.  Return 1 / 0 as
+ * above. */
 int ufomi_task_generate_host(UfomiTask *task, float *output, const UfomiRequisition *requisition);
 int ufomi_task_process_host(UfomiTask *task, const float *input, const UfomiRequisition *input_requisition);
 /* The output layout and storage a task writes (ufo_buffer_set_layout). */

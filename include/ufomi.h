@@ -184,6 +184,24 @@ int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int 
                       int n_slices, size_t sino_slice_pitch, const ufomi_bp_params *params,
                       float *d_out, size_t out_slice_pitch);
 
+/*
+ * `stacked-backproject` (src/ufo-stacked-backproject-task.c:300-430,
+ * src/kernels/stacked-backproject.cl): n_slices float32 sinograms [P_in][W]
+ * (consecutive) -> slices [roi_h][roi_w] with the `backproject` LUT, axis and
+ * ROI rules but h = axis + bx cos - by sin, in one of the task's precision
+ * modes (enum order and nicks of precision_values, :37-42):
+ *   UFOMI_STACKED_SINGLE  float32 texels, linear interpolation, zero border;
+ *   UFOMI_STACKED_HALF    texels rounded to half, interpolated in float;
+ *   UFOMI_STACKED_INT8    texels (v - min) * 255 / (max - min) truncated to
+ *                         8 bits over the stack, nearest texel clamped to the
+ *                         edge, integer sums, (sum (max - min) / 255 + min) pi / P.
+ * params->mode must be texture.  Every slice is computed (the reference fills
+ * slices only in groups of 2 or 4).
+ */
+enum { UFOMI_STACKED_INT8 = 0, UFOMI_STACKED_HALF = 1, UFOMI_STACKED_SINGLE = 2 };
+int ufomi_bp_stacked(ufomi_ctx *ctx, const float *d_sino, int W, int P_in, int n_slices,
+                     const ufomi_bp_params *params, int precision, float *d_out);
+
 /* --------------------------------------------------- batched FBP requests */
 
 /*
@@ -308,6 +326,26 @@ int ufomi_cone_weight(ufomi_ctx *ctx, const float *d_in, float *d_out, int W, in
                       const double *detector_distance, const double *axis_angle_x, int n_param);
 
 /*
+ * `flat-field-correct` (src/kernels/ffc.cl:20-48,
+ * src/ufo-flat-field-correct-task.c:107-160) of n items [H][W] at d_data:
+ * with d = dark * dark_scale and f = flat * flat_scale per pixel,
+ * (data - d) / (f - d), or log((f - d) / (data - d)) when absorption_correct;
+ * NaN and Inf become 0 when fix_nan_and_inf.  With sinogram_input the dark and
+ * flat items are one row of W.  Item i uses dark item min(i, n_dark - 1) and
+ * flat item min(i, n_flat - 1): a finished input stream keeps feeding its last
+ * buffer.  d_out may equal d_data.
+ */
+int ufomi_flat_field_correct(ufomi_ctx *ctx, const float *d_data, int W, int H, int n, const float *d_dark,
+                             int n_dark, const float *d_flat, int n_flat, int sinogram_input, int absorption_correct,
+                             int fix_nan_and_inf, float dark_scale, float flat_scale, float *d_out);
+
+/*
+ * `average` (src/ufo-average-task.c:102-151): the float32 sum of n items of
+ * `elems` floats in order, divided by n, written `copies` times.
+ */
+int ufomi_average(ufomi_ctx *ctx, const float *d_items, size_t elems, int n, int copies, float *d_out);
+
+/*
  * `transpose-projections` (src/ufo-transpose-projections-task.c:64-150): n projections [n][H][W] become rows first .. first + n - 1 of
  * the H sinograms [H][number][W] at d_sino (row h of projection p is row
  * first + p of sinogram h).  d_proj NULL writes zero rows there instead: the
@@ -368,7 +406,8 @@ int ufomi_gbp_projection_rows(const ufomi_gbp_geometry *geometry, const ufomi_gb
 int ufomi_gbp_volume_shape(const ufomi_gbp_params *p, int W, int *nx, int *ny, int *nz);
 
 /*
- * `general-backproject` (parameter z): P projections [H][W] float32 (stacked,
+ * `general-backproject` (parameter z; other parameters: ufomi_gbp_sweep):
+ * P projections [H][W] float32 (stacked,
  * projection pitch H*W) -> volume [nz][ny][nx] in store_type.  Each
  * projection's transformation chain (ufo-general-backproject-task.c:807-1048)
  * is composed in FP64 into a projective 4x4 map on the host; the kernel
@@ -377,6 +416,19 @@ int ufomi_gbp_volume_shape(const ufomi_gbp_params *p, int W, int *nx, int *ny, i
  */
 int ufomi_gbp(ufomi_ctx *ctx, const float *d_projections, int W, int H, int P,
               const ufomi_gbp_geometry *geometry, const ufomi_gbp_params *params, void *d_volume);
+
+/*
+ * `general-backproject` with any `parameter` (the enum of
+ * ufo-general-backproject-task.c:345-364, same order): the volume's third axis
+ * sweeps that parameter over region (from, to, step) while voxels lie at
+ * z = params->z; slice k replaces the parameter by region[0] + k region[2]
+ * for every projection (make_parameter_assignment, :755-775).  parameter z
+ * is ufomi_gbp.  axis-angle-z holds the tomographic angles and is rejected.
+ */
+enum { UFOMI_GBP_PARAMETER_Z = 17 };
+int ufomi_gbp_sweep(ufomi_ctx *ctx, const float *d_projections, int W, int H, int P,
+                    const ufomi_gbp_geometry *geometry, const ufomi_gbp_params *params, int parameter,
+                    void *d_volume);
 
 /* ------------------------------------------------- synthetic phantom data */
 

@@ -235,6 +235,63 @@ def bp_nearest_ref32(sino, axis_pos=-1.0, angle_step=-1.0, angle_offset=0.0, num
     return out
 
 
+def stacked_bp_ref32(sinos, precision="single", axis_pos=-1.0, angle_step=-1.0, angle_offset=0.0,
+                     num_projections=0, offset=0, roi_x=0, roi_y=0, roi_width=0, roi_height=0):
+    """stacked-backproject (src/ufo-stacked-backproject-task.c:300-430, src/kernels/stacked-backproject.cl)
+    with float32 semantics: sinograms [S][P][W] -> slices [S][roi_h][roi_w].  h = bx cos - by sin + axis
+    (bx = ((x - axis) + roi_x) + 0.5; a work-item's rows y0 + 4q use h(y0) - 4q sin) sampled
+      single/half: linearly at h - 0.5 with a zero border (half: texels rounded to float16 first),
+                   sums over the 4 projection classes p mod 4 added as (c0 + c2) + (c1 + c3), times pi / P;
+      int8: texel (v - min) * (255 / (max - min)) truncated and saturated to 8 bits (min, max of the
+            whole stack), nearest texel floor(h) clamped to the edge, integer sum s, then
+            ((s * ((max - min) / 255)) + min) * pi / P."""
+    f32 = np.float32
+    sinos = np.ascontiguousarray(sinos, dtype=np.float32)
+    n_sl, p_in, w = sinos.shape
+    n_proj, axis, rw, rh = _bp_defaults(w, p_in, num_projections, axis_pos, roi_width, roi_height)
+    s, c, _ = bp_lut(n_proj, angle_step, angle_offset)
+    s, c = s[offset:offset + p_in].astype(f32), c[offset:offset + p_in].astype(f32)
+    axis = f32(axis)
+    x = np.arange(rw, dtype=np.int64)
+    y = np.arange(rh, dtype=np.int64)
+    y0 = (y // 16) * 16 + (y % 16) % 4
+    q4 = ((y % 16) // 4).astype(f32) * f32(4)
+    bx = (x.astype(f32) - axis + f32(roi_x)) + f32(0.5)
+    by = (y0.astype(f32) - axis + f32(roi_y)) + f32(0.5)
+    out = np.empty((n_sl, rh, rw), dtype=np.float32)
+    if precision == "int8":
+        lo, hi = sinos.min(), sinos.max()
+        scale = f32(255.0) / (hi - lo)
+        with np.errstate(invalid="ignore"):
+            qv = (sinos - lo) * scale
+        qv = np.where(qv > 0, qv, f32(0))
+        qv = np.minimum(np.floor(qv), f32(255)).astype(np.uint32)
+        acc = np.zeros((n_sl, rh, rw), dtype=np.uint64)
+        for p in range(p_in):
+            h = ((bx[None, :] * c[p] - by[:, None] * s[p]) + axis) - q4[:, None] * s[p]
+            i = np.clip(np.floor(h).astype(np.int64), 0, w - 1)
+            acc += qv[:, p, :][:, i]
+        rs = (hi - lo) / f32(255.0)
+        r = ((acc.astype(np.float32) * rs) + lo) * f32(np.pi)
+        out[:] = r / f32(p_in)
+        return out
+    data = sinos.astype(np.float16).astype(np.float32) if precision == "half" else sinos
+    classes = np.zeros((4, n_sl, rh, rw), dtype=np.float32)
+    for p in range(p_in):
+        h = ((bx[None, :] * c[p] - by[:, None] * s[p]) + axis) - q4[:, None] * s[p]
+        u = h - f32(0.5)
+        i0 = np.floor(u)
+        fr = (u - i0).astype(np.float32)
+        i0 = i0.astype(np.int64)
+        row = data[:, p, :]
+        a = np.where((i0 >= 0) & (i0 < w), row[:, np.clip(i0, 0, w - 1)], f32(0))
+        b = np.where((i0 + 1 >= 0) & (i0 + 1 < w), row[:, np.clip(i0 + 1, 0, w - 1)], f32(0))
+        classes[p % 4] += a * (f32(1) - fr) + b * fr
+    total = (classes[0] + classes[2]) + (classes[1] + classes[3])
+    out[:] = total * f32(np.pi) / f32(p_in)
+    return out
+
+
 def cone_weight_ref32(proj, center_x, center_z, source_distance, detector_distance, axis_angle_x=0.0):
     """weight_projection (conebeam.cl:20-36) with the task's argument setup."""
     proj = np.ascontiguousarray(proj, dtype=np.float32)
@@ -243,6 +300,39 @@ def cone_weight_ref32(proj, center_x, center_z, source_distance, detector_distan
     lib().orc_cone_weight_ref32(proj, out, w, h, center_x, center_z, source_distance,
                                 detector_distance, axis_angle_x)
     return out
+
+
+# ------------------------------------------------------- flat field, average
+
+def flat_field_correct_ref32(data, dark, flat, absorption_correct=False, fix_nan_and_inf=False,
+                             sinogram_input=False, dark_scale=1.0, flat_scale=1.0):
+    """flat_correct of src/kernels/ffc.cl:20-48 in float32 for items [n][H][W]: d = dark * dark_scale,
+    f = flat * flat_scale, (data - d) / (f - d) or log((f - d) / (data - d)); NaN / Inf -> 0 on request.
+    dark / flat: one item [H][W] (or a row [W] with sinogram_input) or one per data item."""
+    f32 = np.float32
+    data = np.asarray(data, dtype=f32)
+    dark = np.asarray(dark, dtype=f32) * f32(dark_scale)
+    flat = np.asarray(flat, dtype=f32) * f32(flat_scale)
+    if sinogram_input:
+        dark = dark.reshape(dark.shape[:-2] + (1, dark.shape[-1])) if dark.ndim >= 2 else dark[None, :]
+        flat = flat.reshape(flat.shape[:-2] + (1, flat.shape[-1])) if flat.ndim >= 2 else flat[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if absorption_correct:
+            r = np.log((flat - dark) / (data - dark)).astype(f32)
+        else:
+            r = ((data - dark) / (flat - dark)).astype(f32)
+    if fix_nan_and_inf:
+        r = np.where(np.isfinite(r), r, f32(0))
+    return r
+
+
+def average_ref32(items):
+    """average (src/ufo-average-task.c:102-151): float32 sum in arrival order / count."""
+    items = np.asarray(items, dtype=np.float32)
+    s = np.zeros(items.shape[1:], dtype=np.float32)
+    for it in items:
+        s = s + it
+    return s / np.float32(len(items))
 
 
 # ------------------------------------------------------- transpose-projections

@@ -261,3 +261,65 @@ def test_noise_is_seeded(ctx, torch_dev):
     assert torch.equal(a, b)
     d = (a - 1).cpu().numpy()
     assert abs(d.std() - 0.01) < 5e-4 and abs(d.mean()) < 2e-4
+
+
+SWEEP_BASE = dict(source_position_y=[-50.0], detector_position_y=[40.0], center_position_x=[10.0],
+                  center_position_z=[8.0])
+
+
+@pytest.mark.parametrize("parameter,region,extra", [
+    ("center-position-x", (8.5, 11.5, 0.5), {}),
+    ("axis-angle-x", (0.0, 0.6, 0.2), {}),
+    ("detector-angle-z", (-0.1, 0.1, 0.05), {}),
+    ("source-position-y", (-70.0, -40.0, 10.0), {}),
+    ("center-position-z", (6.0, 10.0, 1.0), {"z": 1.5}),
+    ("volume-angle-z", (0.0, 1.0, 0.25), {}),
+])
+def test_parameter_sweep(ctx, torch_dev, parameter, region, extra):
+    """Slice k of a sweep equals the plain reconstruction at z with the parameter set to
+    region[0] + k region[2] for every projection (make_parameter_assignment)."""
+    import torch
+    projs = _smooth_projs(21, 24, 16, 20)
+    p, h, w = projs.shape
+    z = extra.get("z", 0.0)
+    params = ufomi.gbp_params(p, region=region)
+    params.z = z
+    geom = ufomi.gbp_geometry(**SWEEP_BASE)
+    shape = ufomi.gbp_volume_shape(params, w)
+    vol = torch.zeros(shape, device=torch_dev)
+    ctx.gbp_sweep(_dev(projs, torch_dev), w, h, p, geom, params, ufomi.GBP_PARAMETERS.index(parameter), vol)
+    got = vol.cpu().numpy()
+    key = parameter.replace("-", "_")
+    n = shape[0]
+    assert n == int(math.ceil((region[1] - region[0]) / region[2]))
+    for k in range(n):
+        geo = dict(SWEEP_BASE)
+        geo[key] = [region[0] + k * region[2]]
+        ref = oracle.gbp(projs, region=(z, z + 1.0, 1.0), **geo)
+        assert rel_l2(got[k], ref[0]) < 1e-5, (k, rel_l2(got[k], ref[0]))
+
+
+def test_sweep_task_and_tomo_angle_rejected(ctx, torch_dev):
+    import ufo_tasks
+    projs = _smooth_projs(22, 16, 12, 16)
+    t = ufo_tasks.Task("general-backproject")
+    for k, v in dict(num_projections=16, parameter="center-position-x", region=[7.0, 9.0, 0.5], z=0.5,
+                     source_position_y=[-40.0], detector_position_y=[30.0], center_position_z=[6.0]).items():
+        setattr(t.props, k, v)
+    src, sink = ufo_tasks.MemoryIn(projs), ufo_tasks.MemoryOut()
+    g = ufo_tasks.TaskGraph()
+    g.connect_nodes(src, t)
+    g.connect_nodes(t, sink)
+    ufo_tasks.Scheduler(ctx).run(g)
+    got = sink.array
+    assert got.shape == (4, 16, 16)
+    for k in range(4):
+        ref = oracle.gbp(projs, region=(0.5, 1.5, 1.0), source_position_y=[-40.0], detector_position_y=[30.0],
+                         center_position_x=[7.0 + 0.5 * k], center_position_z=[6.0])
+        assert rel_l2(got[k], ref[0]) < 1e-5
+    bad = ufo_tasks.Task("general-backproject")
+    bad.props.num_projections = 16
+    bad.props.parameter = "axis-angle-z"
+    with pytest.raises(ufo_tasks.TaskError) as e:
+        bad.setup(ctx)
+    assert e.value.status == -4

@@ -385,3 +385,57 @@ def test_fft_nd_vs_numpy_and_brute_force(shape, dims, m):
     np.testing.assert_allclose(oracle.ifft_nd(got, dims), pad, atol=1e-9)
     # a planted mistake (conjugated kernel) is caught
     assert np.abs(np.conj(got) - ref).max() > 1e-3
+
+
+def _stacked_int8_brute(sinos, axis, roi=(0, 0, 0, 0)):
+    """stacked-backproject.cl texture_uint / uninterleave_uint, one pixel at a time in float32."""
+    f32 = np.float32
+    n_sl, p_in, w = sinos.shape
+    rx, ry, rw, rh = roi
+    rw, rh = rw or w, rh or w
+    s, c, _ = oracle.bp_lut(p_in)
+    lo, hi = sinos.min(), sinos.max()
+    scale = f32(255.0) / (hi - lo)
+    out = np.zeros((n_sl, rh, rw), np.float32)
+    for sl in range(n_sl):
+        for y in range(rh):
+            q = (y % 16) // 4
+            y0 = y - 4 * q
+            for x in range(rw):
+                bx = f32(f32(f32(x) - f32(axis)) + f32(rx)) + f32(0.5)
+                by = f32(f32(f32(y0) - f32(axis)) + f32(ry)) + f32(0.5)
+                acc = 0
+                for p in range(p_in):
+                    h = f32(f32(f32(bx * c[p]) - f32(by * s[p])) + f32(axis))
+                    h = f32(h - f32(f32(4 * q) * s[p]))
+                    i = min(max(int(math.floor(h)), 0), w - 1)
+                    v = f32(f32(sinos[sl, p, i] - lo) * scale)
+                    acc += 0 if not v > 0 else min(int(math.floor(v)), 255)
+                r = f32(f32(f32(f32(acc) * f32((hi - lo) / f32(255.0))) + lo) * f32(math.pi))
+                out[sl, y, x] = f32(r / f32(p_in))
+    return out
+
+
+def test_stacked_oracle_int8_vs_brute_force():
+    sinos = np.random.default_rng(4).standard_normal((3, 7, 11)).astype(np.float32)
+    got = oracle.stacked_bp_ref32(sinos, "int8", roi_x=1, roi_y=2, roi_width=9, roi_height=18)
+    ref = _stacked_int8_brute(sinos, np.float32(5.5), roi=(1, 2, 9, 18))
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_stacked_oracle_single_is_backproject_of_negated_angles():
+    """h = axis + bx cos - by sin is backproject's h with every angle negated: the C oracle
+    with a (-sin, cos) table gives the same slices up to float rounding; a plain table does not."""
+    sinos = np.random.default_rng(5).standard_normal((2, 40, 32)).astype(np.float32)
+    got = oracle.stacked_bp_ref32(sinos, "single", axis_pos=15.25, roi_x=3, roi_width=24, roi_height=20)
+    s, c, _ = oracle.bp_lut(40)
+    for k in range(2):
+        out = np.empty((20, 24), np.float32)
+        oracle.lib().orc_bp_texture_ref32(np.ascontiguousarray(sinos[k]), 32, 40, -s, c, 0, 15.25, 3, 0, 24, 20, out)
+        assert oracle.rel_l2(got[k], out) < 5e-6
+        plain = oracle.bp_texture_ref32(sinos[k], axis_pos=15.25, roi_x=3, roi_width=24, roi_height=20)
+        assert oracle.rel_l2(got[k], plain) > 1e-2
+    half = oracle.stacked_bp_ref32(sinos, "half", axis_pos=15.25, roi_x=3, roi_width=24, roi_height=20)
+    rounded = oracle.stacked_bp_ref32(sinos.astype(np.float16).astype(np.float32), "single", axis_pos=15.25,
+                                      roi_x=3, roi_width=24, roi_height=20)
+    np.testing.assert_array_equal(half, rounded)

@@ -449,17 +449,22 @@ __global__ __launch_bounds__(kThreads) void bp_nearest_kernel(
     out[(long) sl * out_pitch + (long) idy * roi_w + idx] = r / (float) P;
 }
 
+}  // namespace
+
 // LUT of ufo-backproject-task.c:174-196: FP64 sin/cos of angle_offset + i*step,
 // rounded to float; entries offset..offset+P-1 are uploaded, as (sin, cos).
-int get_trig(ufomi_ctx *ctx, double angle_offset, double step, unsigned offset, int P, float2 **out)
+// sin_sign -1 stores -sin (exact in float): the stacked kernels' h = bx cos - by sin.
+int ufomi_bp_trig(ufomi_ctx *ctx, double angle_offset, double step, unsigned offset, int P, int sin_sign,
+                  float2 **out)
 {
     const bool cached = ctx->lut && ctx->lut_n == P && ctx->lut_offset == angle_offset &&
-                        ctx->lut_step == step && ctx->lut_first == offset;
+                        ctx->lut_step == step && ctx->lut_first == offset && ctx->lut_sign == sin_sign;
     if (!cached) {
         std::vector<float2> h(P);
         for (int i = 0; i < P; i++) {
             const double a = angle_offset + (double) (offset + i) * step;
-            h[i] = make_float2((float) sin(a), (float) cos(a));
+            const float sv = (float) sin(a);
+            h[i] = make_float2(sin_sign < 0 ? -sv : sv, (float) cos(a));
         }
         if (ctx->lut) {
             UFOMI_HIP(hipStreamSynchronize(ctx->stream));
@@ -472,12 +477,11 @@ int get_trig(ufomi_ctx *ctx, double angle_offset, double step, unsigned offset, 
         ctx->lut_offset = angle_offset;
         ctx->lut_step = step;
         ctx->lut_first = offset;
+        ctx->lut_sign = sin_sign;
     }
     *out = ctx->lut;
     return UFOMI_OK;
 }
-
-}  // namespace
 
 extern "C" int ufomi_bp_configure(ufomi_ctx *ctx, int slices_per_tile, int angle_chunk, int half_staging)
 {
@@ -505,6 +509,13 @@ extern "C" void ufomi_bp_params_init(ufomi_bp_params *p)
 extern "C" int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in,
                                  int n_slices, size_t sino_pitch, const ufomi_bp_params *prm,
                                  float *d_out, size_t out_pitch)
+{
+    return ufomi_bp_parallel_ex(ctx, d_sino, dtype, W, P_in, n_slices, sino_pitch, prm, d_out, out_pitch, +1, true);
+}
+
+int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in, int n_slices,
+                         size_t sino_pitch, const ufomi_bp_params *prm, float *d_out, size_t out_pitch, int sin_sign,
+                         bool half_staging)
 {
     int rc = ufomi_enter(ctx);
     if (rc) return rc;
@@ -539,7 +550,7 @@ extern "C" int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, 
     if (!out_pitch) out_pitch = (size_t) roi_w * roi_h;
 
     float2 *trig = nullptr;
-    if ((rc = get_trig(ctx, prm->angle_offset, step, prm->offset, P_in, &trig))) return rc;
+    if ((rc = ufomi_bp_trig(ctx, prm->angle_offset, step, prm->offset, P_in, sin_sign, &trig))) return rc;
 
     if (prm->mode == UFOMI_BP_NEAREST) {
         if (dtype != UFOMI_F32)
@@ -560,7 +571,7 @@ extern "C" int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, 
     // fastest f32 configuration; half staging speeds f16 storage up further.
     int S = n_slices >= 8 ? 8 : (n_slices >= 4 ? 4 : (n_slices >= 2 ? 2 : 1));
     if (ctx->bp_slices && n_slices >= ctx->bp_slices) S = ctx->bp_slices;
-    const bool h16 = dtype == UFOMI_F16 && S >= 4 && ctx->bp_half >= 0;
+    const bool h16 = dtype == UFOMI_F16 && S >= 4 && ctx->bp_half >= 0 && half_staging;
     // S = 8 with half staging keeps its registers only with 8-angle chunks
     const int A = (S == 8 && h16) ? 8 : (ctx->bp_chunk ? ctx->bp_chunk : (S >= 4 ? 8 : 16));
     const int tiles_x = (roi_w + kTile - 1) / kTile, tiles_y = (roi_h + kTile - 1) / kTile;

@@ -650,6 +650,67 @@ extern "C" int ufomi_gbp(ufomi_ctx *ctx, const float *d_proj, int W, int H, int 
     return gbp_launch(ctx, d_proj, W, H, n, mats, f, kaddr, prm, nx, ny, nz, d_vol);
 }
 
+// The geometry array a swept parameter replaces (the reference's parameter
+// enum order, ufo-general-backproject-task.c:345-364).
+static bool sweep_field(ufomi_gbp_geometry *g, int parameter, const double ***ptr, int **n)
+{
+    switch (parameter) {
+    case 0: *ptr = &g->axis_angle_x; *n = &g->n_axis_angle_x; return true;
+    case 1: *ptr = &g->axis_angle_y; *n = &g->n_axis_angle_y; return true;
+    case 3: *ptr = &g->volume_angle_x; *n = &g->n_volume_angle_x; return true;
+    case 4: *ptr = &g->volume_angle_y; *n = &g->n_volume_angle_y; return true;
+    case 5: *ptr = &g->volume_angle_z; *n = &g->n_volume_angle_z; return true;
+    case 6: *ptr = &g->detector_angle_x; *n = &g->n_detector_angle_x; return true;
+    case 7: *ptr = &g->detector_angle_y; *n = &g->n_detector_angle_y; return true;
+    case 8: *ptr = &g->detector_angle_z; *n = &g->n_detector_angle_z; return true;
+    case 9: *ptr = &g->detector_position_x; *n = &g->n_detector_position_x; return true;
+    case 10: *ptr = &g->detector_position_y; *n = &g->n_detector_position_y; return true;
+    case 11: *ptr = &g->detector_position_z; *n = &g->n_detector_position_z; return true;
+    case 12: *ptr = &g->source_position_x; *n = &g->n_source_position_x; return true;
+    case 13: *ptr = &g->source_position_y; *n = &g->n_source_position_y; return true;
+    case 14: *ptr = &g->source_position_z; *n = &g->n_source_position_z; return true;
+    case 15: *ptr = &g->center_position_x; *n = &g->n_center_position_x; return true;
+    case 16: *ptr = &g->center_position_z; *n = &g->n_center_position_z; return true;
+    default: return false;
+    }
+}
+
+extern "C" int ufomi_gbp_sweep(ufomi_ctx *ctx, const float *d_proj, int W, int H, int P, const ufomi_gbp_geometry *g,
+                               const ufomi_gbp_params *prm, int parameter, void *d_vol)
+{
+    if (parameter == UFOMI_GBP_PARAMETER_Z)
+        return ufomi_gbp(ctx, d_proj, W, H, P, g, prm, d_vol);
+    if (!g || !prm || !d_vol)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid general backprojection arguments");
+    if (parameter == 2)
+        return ufomi_fail(UFOMI_ERR_UNSUPPORTED,
+                          "axis-angle-z holds the tomographic angles; it is not a sweep parameter");
+    ufomi_gbp_geometry gk = *g;
+    const double **field;
+    int *count;
+    if (!sweep_field(&gk, parameter, &field, &count))
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown parameter %d", parameter);
+    int nx, ny, nz, rc;
+    if ((rc = ufomi_gbp_volume_shape(prm, W, &nx, &ny, &nz))) return rc;
+    static const size_t st_size[] = {2, 4, 8, 1, 2, 4};
+    if (prm->store_type < UFOMI_ST_HALF || prm->store_type > UFOMI_ST_UINT)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown store type %d", prm->store_type);
+    const size_t slice = (size_t) nx * ny * st_size[prm->store_type];
+    // slice k: the parameter takes region[0] + k region[2] for every
+    // projection (make_parameter_assignment, :755-775); voxels lie at z
+    ufomi_gbp_params pk = *prm;
+    pk.region[0] = prm->z;
+    pk.region[1] = prm->z + 1.0;
+    pk.region[2] = 1.0;
+    for (int k = 0; k < nz; k++) {
+        const double value = prm->region[0] + (double) k * prm->region[2];
+        *field = &value;
+        *count = 1;
+        if ((rc = ufomi_gbp(ctx, d_proj, W, H, P, &gk, &pk, (char *) d_vol + (size_t) k * slice))) return rc;
+    }
+    return UFOMI_OK;
+}
+
 extern "C" int ufomi_gbp_projection_rows(const ufomi_gbp_geometry *g, const ufomi_gbp_params *prm, double *rows,
                                          int *parallel_beam)
 {

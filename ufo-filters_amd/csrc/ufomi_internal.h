@@ -25,6 +25,7 @@ struct ufomi_ctx {
     int lut_n;
     double lut_offset, lut_step;
     unsigned lut_first;
+    int lut_sign;            // +1: (sin, cos); -1: (-sin, cos)
     // parallel backprojection tile configuration (0 = automatic), ufomi_bp_configure
     int bp_slices, bp_chunk, bp_half;
 };
@@ -65,6 +66,16 @@ static inline int ufomi_log2(long x)
         l++;
     return l;
 }
+
+// ---- parallel backprojection (bp_parallel.hip), shared with stacked.hip
+
+// The (sin, cos) LUT of the context (sin_sign -1 stores -sin).
+int ufomi_bp_trig(ufomi_ctx *ctx, double angle_offset, double step, unsigned offset, int P, int sin_sign,
+                  float2 **out);
+// ufomi_bp_parallel with h = axis + bx cos + sin_sign * by sin and half staging allowed or not.
+int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in, int n_slices,
+                         size_t sino_pitch, const ufomi_bp_params *prm, float *d_out, size_t out_pitch, int sin_sign,
+                         bool half_staging);
 
 // ---- batched FBP (batch.hip) over the kernels of fft_filter.hip and bp_parallel.hip
 

@@ -13,6 +13,10 @@
  *   - other processors run item by item;
  *   - `general-backproject` takes every item as a projection and emits its
  *     slices through generate;
+ *   - `average` and `flat-field-correct` run over the whole batch in one
+ *     launch; a leading input group `[read ..., read ... ! average, ...] !
+ *     flat-field-correct' runs each branch first (a finished input keeps
+ *     feeding its last item);
  *   - a first stage `read` loads every image it generates into one host
  *     buffer and uploads it in one copy; a last stage `write` downloads the
  *     final batch in one copy and writes it item by item.
@@ -25,12 +29,17 @@
 #include <string.h>
 
 #define MAX_STAGES 16
+#define MAX_BRANCHES 4
 
 struct UfomiPipeline {
     UfomiTask *tasks[MAX_STAGES];
     int n;
     int fused;
+    struct UfomiPipeline *branches[MAX_BRANCHES];  /* [b0, b1, ...] ! tasks[0]: its inputs */
+    int n_branches;
 };
+
+static int is_name(const UfomiTask *t, const char *name) { return !strcmp(ufomi_task_name(t), name); }
 
 static char *trim(char *s)
 {
@@ -42,6 +51,34 @@ static char *trim(char *s)
     return s;
 }
 
+/* Stages `name key=value ... ! name ...` appended to pl. */
+static int parse_stages(UfomiPipeline *pl, char *text, UfomiError *e)
+{
+    char *save = NULL;
+    for (char *stage = strtok_r(text, "!", &save); stage; stage = strtok_r(NULL, "!", &save)) {
+        char *s = trim(stage), *save2 = NULL;
+        char *name = strtok_r(s, " \t\n", &save2);
+        if (!name || pl->n == MAX_STAGES)
+            return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "invalid pipeline stage `%s'", s);
+        UfomiTask *t = ufomi_task_new(name);
+        if (!t)
+            return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "no task named `%s'", name);
+        pl->tasks[pl->n++] = t;
+        for (char *kv = strtok_r(NULL, " \t\n", &save2); kv; kv = strtok_r(NULL, " \t\n", &save2)) {
+            char *eq = strchr(kv, '=');
+            if (!eq)
+                return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "expected key=value, got `%s'", kv);
+            *eq = '\0';
+            if (ufomi_task_set_property(t, kv, eq + 1))
+                return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "%s", ufomi_last_error());
+        }
+    }
+    return UFOMI_OK;
+}
+
+/* ufo-launch syntax: stages joined by `!', optionally led by one group of
+ * input branches `[a ! ..., b ! ..., c ! ...] ! task ! ...' feeding the
+ * inputs of a multi-input task (each branch starts with `read'). */
 UfomiPipeline *ufomi_pipeline_new(const char *description, UfomiError *e)
 {
     if (!description) {
@@ -49,39 +86,58 @@ UfomiPipeline *ufomi_pipeline_new(const char *description, UfomiError *e)
         return NULL;
     }
     UfomiPipeline *pl = calloc(1, sizeof(UfomiPipeline));
-    char *copy = strdup(description), *save = NULL;
-    for (char *stage = strtok_r(copy, "!", &save); stage; stage = strtok_r(NULL, "!", &save)) {
-        char *s = trim(stage), *save2 = NULL;
-        char *name = strtok_r(s, " \t", &save2);
-        if (!name || pl->n == MAX_STAGES) {
-            task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "invalid pipeline stage `%s'", s);
+    char *copy = strdup(description);
+    char *text = trim(copy);
+    if (*text == '[') {
+        char *close = strchr(text, ']');
+        char *nested = strchr(text + 1, '[');
+        if (!close || (nested && nested < close)) {
+            task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG,
+                       close ? "nested input groups are not supported" : "unterminated `['");
             goto fail;
         }
-        UfomiTask *t = ufomi_task_new(name);
-        if (!t) {
-            task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "no task named `%s'", name);
+        *close = '\0';
+        char *rest = trim(close + 1);
+        if (*rest != '!') {
+            task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "an input group must be followed by `! task'");
             goto fail;
         }
-        pl->tasks[pl->n++] = t;
-        for (char *kv = strtok_r(NULL, " \t", &save2); kv; kv = strtok_r(NULL, " \t", &save2)) {
-            char *eq = strchr(kv, '=');
-            if (!eq) {
-                task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "expected key=value, got `%s'", kv);
+        char *save = NULL;
+        for (char *br = strtok_r(text + 1, ",", &save); br; br = strtok_r(NULL, ",", &save)) {
+            if (pl->n_branches == MAX_BRANCHES) {
+                task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "more than %d inputs", MAX_BRANCHES);
                 goto fail;
             }
-            *eq = '\0';
-            if (ufomi_task_set_property(t, kv, eq + 1)) {
-                task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "%s", ufomi_last_error());
+            UfomiPipeline *b = ufomi_pipeline_new(br, e);
+            if (!b)
+                goto fail;
+            pl->branches[pl->n_branches++] = b;
+            if (!is_name(b->tasks[0], "read") || is_name(b->tasks[b->n - 1], "write")) {
+                task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG,
+                           "input %d must start with `read' and must not end with `write'", pl->n_branches - 1);
                 goto fail;
             }
         }
+        text = rest + 1;
     }
-    free(copy);
+    if (parse_stages(pl, text, e))
+        goto fail;
     if (!pl->n) {
         task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "empty pipeline");
-        ufomi_pipeline_free(pl);
-        return NULL;
+        goto fail;
     }
+    if (pl->n_branches && (int) ufomi_task_get_num_inputs(pl->tasks[0]) != pl->n_branches) {
+        task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "`%s' takes %u inputs, the group gives %d",
+                   ufomi_task_name(pl->tasks[0]), ufomi_task_get_num_inputs(pl->tasks[0]), pl->n_branches);
+        goto fail;
+    }
+    if (!pl->n_branches && ufomi_task_get_num_inputs(pl->tasks[0]) > 1) {
+        task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "`%s' needs a group of %u inputs `[a, b, ...] ! %s'",
+                   ufomi_task_name(pl->tasks[0]), ufomi_task_get_num_inputs(pl->tasks[0]),
+                   ufomi_task_name(pl->tasks[0]));
+        goto fail;
+    }
+    free(copy);
     return pl;
 fail:
     free(copy);
@@ -95,6 +151,8 @@ void ufomi_pipeline_free(UfomiPipeline *pl)
         return;
     for (int i = 0; i < pl->n; i++)
         ufomi_task_free(pl->tasks[i]);
+    for (int b = 0; b < pl->n_branches; b++)
+        ufomi_pipeline_free(pl->branches[b]);
     free(pl);
 }
 
@@ -229,24 +287,47 @@ static int write_batch(ufomi_ctx *ctx, UfomiTask *t, const Batch *cur, UfomiErro
     return rc;
 }
 
-int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const UfomiRequisition *in_req,
-                       int n_items, void *d_out, size_t out_capacity, int *n_out, UfomiRequisition *out_req,
-                       UfomiError *e)
+/* The multi-input first stage over its input batches: flat-field-correct
+ * pairs projection i with dark / flat item min(i, n - 1). */
+static int run_multi_input(ufomi_ctx *ctx, UfomiTask *t, const Batch *in, Batch *out, UfomiError *e)
 {
-    if (!pl || !ctx || !n_out || !out_req)
-        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "invalid pipeline arguments");
+    for (int k = 0; k < 3; k++)
+        if (in[k].buf.dtype != UFOMI_F32)
+            return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_UNSUPPORTED,
+                              "flat-field-correct: inputs must be float32");
+    int rc = ffc_check_inputs(t, &in[0].buf.req, &in[1].buf.req, &in[2].buf.req, e);
+    if (rc)
+        return rc;
+    if ((rc = batch_alloc(ctx, out, &in[0].buf.req, in[0].n, sizeof(float), UFOMI_F32, UFOMI_LAYOUT_REAL)))
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_MEMORY, "out of device memory: %s",
+                          ufomi_last_error());
+    rc = ffc_task_run(t, ctx, in[0].buf.data, &in[0].buf.req, in[0].n, in[1].buf.data, in[1].n, in[2].buf.data,
+                      in[2].n, out->buf.data);
+    if (rc) {
+        ufomi_buffer_release(ctx, &out->buf);
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "flat-field-correct: %s", ufomi_last_error());
+    }
+    t->num_processed += (unsigned) in[0].n;
+    return UFOMI_OK;
+}
+
+/* Runs every stage but a final `write' and leaves the last batch in *res
+ * (*res_owned: res holds device memory the caller releases). */
+static int run_batch(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const UfomiRequisition *in_req,
+                     int n_items, Batch *res, int *res_owned, UfomiError *e)
+{
     const int has_read = is(pl->tasks[0], "read"), has_write = is(pl->tasks[pl->n - 1], "write");
-    const int first = has_read, last = pl->n - has_write;
-    if ((!has_read && (!d_in || !in_req || n_items <= 0)) || (!has_write && !d_out))
+    const int first = has_read || pl->n_branches, last = pl->n - has_write;
+    memset(res, 0, sizeof(*res));
+    *res_owned = 0;
+    if (!has_read && !pl->n_branches && (!d_in || !in_req || n_items <= 0))
         return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "invalid pipeline arguments");
     for (int i = first; i < last; i++)
         if (is(pl->tasks[i], "read") || is(pl->tasks[i], "write"))
             return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG,
                               "`%s' must be the %s stage of a pipeline", ufomi_task_name(pl->tasks[i]),
                               is(pl->tasks[i], "read") ? "first" : "last");
-    if (has_read && pl->n == 1)
-        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "a pipeline of `read' alone has no sink");
-    int rc;
+    int rc = UFOMI_OK;
     for (int i = 0; i < pl->n; i++)
         if ((rc = ufomi_task_setup(pl->tasks[i], ctx, e)))
             return rc;
@@ -254,15 +335,30 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
 
     Batch cur;
     int cur_owned = 0;
-    if (has_read) {
+    if (pl->n_branches) {
+        /* input branches (ufo-launch's [a, b, c] ! task): each one a batch, then the multi-input stage */
+        Batch br[MAX_BRANCHES];
+        int owned[MAX_BRANCHES] = {0};
+        for (int b = 0; b < pl->n_branches && !rc; b++)
+            rc = run_batch(pl->branches[b], ctx, NULL, NULL, 0, &br[b], &owned[b], e);
+        for (int b = 0; b < pl->n_branches && !rc; b++)
+            if (!br[b].n)
+                rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "input %d of `%s' produced no items", b,
+                                ufomi_task_name(pl->tasks[0]));
+        if (!rc)
+            rc = run_multi_input(ctx, pl->tasks[0], br, &cur, e);
+        for (int b = 0; b < pl->n_branches; b++)
+            if (owned[b])
+                ufomi_buffer_release(ctx, &br[b].buf);
+        if (rc)
+            return rc;
+        cur_owned = 1;
+    } else if (has_read) {
         if ((rc = read_batch(ctx, pl->tasks[0], &cur, e)))
             return rc;
         cur_owned = cur.n > 0;
         if (!cur.n) {  /* nothing matched the image range: an empty stream */
-            if (has_write)
-                write_task_flush(pl->tasks[pl->n - 1]);
-            *n_out = 0;
-            memset(out_req, 0, sizeof(*out_req));
+            *res = cur;
             return UFOMI_OK;
         }
     } else {
@@ -330,6 +426,23 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
                 goto out;
             }
             t->num_processed += (unsigned) cur.n;
+        } else if (is(t, "average")) {
+            /* the float32 sum of all items in order / count, `number' copies, one launch */
+            const int copies = (int) t->v[0].num;
+            req = cur.buf.req;
+            if (cur.buf.dtype != UFOMI_F32) {
+                rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_UNSUPPORTED, "average: input must be float32");
+                goto out;
+            }
+            if ((rc = batch_alloc(ctx, &next, &req, copies, sizeof(float), UFOMI_F32, UFOMI_LAYOUT_REAL)))
+                goto fail_alloc;
+            if ((rc = ufomi_average(ctx, cur.buf.data, ufomi_requisition_elements(&req), cur.n, copies,
+                                    next.buf.data))) {
+                ufomi_buffer_release(ctx, &next.buf);
+                task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "average: %s", ufomi_last_error());
+                goto out;
+            }
+            t->num_processed += (unsigned) cur.n;
         } else if (is(t, "transpose-projections")) {
             /* all projections of the batch in one launch, zero rows for missing ones; the
              * requisition of ufo-transpose-projections-task.c:128-150 without the task's own store */
@@ -362,8 +475,9 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
                 UfomiBuffer v = item_view(&cur, i), *vi[1] = {&v};
                 UfomiBuffer dummy = v;
                 if (!ufomi_task_process(t, vi, &dummy, &req)) {
-                    rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, t->err_status ? t->err_status : UFOMI_ERR_ARG, "%s",
-                                    ufomi_task_last_error(t));
+                    if (!t->err_status)
+                        break;  /* the reductor takes no more input */
+                    rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, t->err_status, "%s", ufomi_task_last_error(t));
                     goto out;
                 }
             }
@@ -422,32 +536,53 @@ int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, con
         goto out;
     }
 
+    *res = cur;
+    *res_owned = cur_owned;
+    return UFOMI_OK;
+out:
+    if (cur_owned)
+        ufomi_buffer_release(ctx, &cur.buf);
+    return rc;
+}
+
+int ufomi_pipeline_run(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const UfomiRequisition *in_req,
+                       int n_items, void *d_out, size_t out_capacity, int *n_out, UfomiRequisition *out_req,
+                       UfomiError *e)
+{
+    if (!pl || !ctx || !n_out || !out_req)
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "invalid pipeline arguments");
+    const int has_read = is(pl->tasks[0], "read"), has_write = is(pl->tasks[pl->n - 1], "write");
+    if (!has_write && !d_out)
+        return task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_ARG, "invalid pipeline arguments");
+    if (has_read && pl->n == 1)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "a pipeline of `read' alone has no sink");
+    Batch cur;
+    int cur_owned = 0;
+    int rc = run_batch(pl, ctx, d_in, in_req, n_items, &cur, &cur_owned, e);
+    if (rc)
+        return rc;
     if (has_write) {
-        if ((rc = write_batch(ctx, pl->tasks[pl->n - 1], &cur, e)))
-            goto out;
-        *n_out = cur.n;
-        *out_req = cur.buf.req;
-        rc = UFOMI_OK;
+        rc = cur.n ? write_batch(ctx, pl->tasks[pl->n - 1], &cur, e) : UFOMI_OK;
+        if (!cur.n)
+            write_task_flush(pl->tasks[pl->n - 1]);
+        if (!rc) {
+            *n_out = cur.n;
+            *out_req = cur.buf.req;
+        }
     } else {
         const size_t total = cur.item_bytes * (size_t) cur.n;
         if (total > out_capacity) {
             rc = task_error(e, UFOMI_TASK_ERROR_PROCESS, UFOMI_ERR_SIZE, "output needs %zu bytes, capacity %zu", total,
                             out_capacity);
-            goto out;
-        }
-        if ((rc = ufomi_memcpy(ctx, d_out, cur.buf.data, total, 2))) {
+        } else if (total && (rc = ufomi_memcpy(ctx, d_out, cur.buf.data, total, 2))) {
             task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "%s", ufomi_last_error());
-            goto out;
-        }
-        if ((rc = ufomi_synchronize(ctx))) {
+        } else if ((rc = ufomi_synchronize(ctx))) {
             task_error(e, UFOMI_TASK_ERROR_PROCESS, rc, "%s", ufomi_last_error());
-            goto out;
+        } else {
+            *n_out = cur.n;
+            *out_req = cur.buf.req;
         }
-        *n_out = cur.n;
-        *out_req = cur.buf.req;
-        rc = UFOMI_OK;
     }
-out:
     if (cur_owned)
         ufomi_buffer_release(ctx, &cur.buf);
     return rc;

@@ -9,6 +9,10 @@
  *   backproject                  src/ufo-backproject-task.c
  *   cone-beam-projection-weight  src/ufo-cone-beam-projection-weight-task.c
  *   general-backproject          src/ufo-general-backproject-task.c
+ *   center-of-rotation           src/ufo-center-of-rotation-task.c
+ *   flat-field-correct           src/ufo-flat-field-correct-task.c
+ *   average                      src/ufo-average-task.c
+ *   stacked-backproject          src/ufo-stacked-backproject-task.c
  *   transpose-projections        src/ufo-transpose-projections-task.c
  *   read                         src/ufo-read-task.c
  *   write                        src/ufo-write-task.c
@@ -238,6 +242,8 @@ int ufomi_task_set_property(UfomiTask *t, const char *name, const char *value)
     const PropSpec *s = &t->cls->props[i];
     PropValue *v = &t->v[i];
     double d;
+    if (!strcmp(t->cls->name, "center-of-rotation") && i == 1)
+        return ufomi_status_fail(UFOMI_ERR_ARG, "property `center' of center-of-rotation is not writable");
     switch (s->type) {
     case PT_BOOL:
         if (!strcasecmp(value, "true") || !strcmp(value, "1"))
@@ -593,6 +599,340 @@ static int bp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiReq
     return 1;
 }
 
+/* ----------------------------------------------------- flat-field-correct */
+
+/* flat-field-correct (ufo-flat-field-correct-task.c:262-297, init :300-309) */
+static const PropSpec ffc_props[] = {
+    {"fix-nan-and-inf", PT_BOOL, 0, 1, 0, NULL, 0, NULL},
+    {"absorption-correct", PT_BOOL, 0, 1, 0, NULL, 0, NULL},
+    {"sinogram-input", PT_BOOL, 0, 1, 0, NULL, 0, NULL},
+    {"dark-scale", PT_FLOAT, -FLT_MAX, FLT_MAX, 1.0, NULL, 0, NULL},
+    {"flat-scale", PT_FLOAT, -FLT_MAX, FLT_MAX, 1.0, NULL, 0, NULL},
+};
+enum { FFC_FIX, FFC_ABSORB, FFC_SINO, FFC_DSCALE, FFC_FSCALE };
+
+typedef struct {
+    int unused;
+} FfcPriv;
+
+static size_t req_rows(const UfomiRequisition *r)
+{
+    return (r->n_dims >= 2 ? r->dims[1] : 1) * (r->n_dims == 3 ? r->dims[2] : 1);
+}
+
+/* get_requisition (:76-88): the projection's size; darks and flats must match
+ * it, or, with sinogram input, be one row of its width */
+int ffc_check_inputs(const UfomiTask *t, const UfomiRequisition *data, const UfomiRequisition *dark,
+                     const UfomiRequisition *flat, UfomiError *e)
+{
+    const int sino = (int) t->v[FFC_SINO].num;
+    const UfomiRequisition *o[2] = {dark, flat};
+    for (int k = 0; k < 2; k++) {
+        const int ok = sino ? o[k]->dims[0] == data->dims[0] && req_rows(o[k]) == 1
+                            : o[k]->dims[0] == data->dims[0] && req_rows(o[k]) == req_rows(data);
+        if (!ok)
+            return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE,
+                              "flat-field-correct inputs must have the same size");
+    }
+    return UFOMI_OK;
+}
+
+static int ffc_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    if (!in[1] || !in[2])
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG,
+                          "flat-field-correct needs projections, darks and flats on inputs 0, 1 and 2");
+    int rc = ffc_check_inputs(t, &in[0]->req, &in[1]->req, &in[2]->req, e);
+    if (rc)
+        return rc;
+    *req = in[0]->req;
+    return UFOMI_OK;
+}
+
+int ffc_task_run(UfomiTask *t, ufomi_ctx *ctx, const float *data, const UfomiRequisition *r, int n,
+                 const float *dark, int n_dark, const float *flat, int n_flat, float *out)
+{
+    return ufomi_flat_field_correct(ctx, data, (int) r->dims[0], (int) req_rows(r), n, dark, n_dark, flat, n_flat,
+                                    (int) t->v[FFC_SINO].num, (int) t->v[FFC_ABSORB].num, (int) t->v[FFC_FIX].num,
+                                    (float) t->v[FFC_DSCALE].num, (float) t->v[FFC_FSCALE].num, out);
+}
+
+/* process (:107-160) */
+static int ffc_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    for (int k = 0; k < 3; k++)
+        if (in[k]->dtype != UFOMI_F32)
+            return process_fail(t, UFOMI_ERR_UNSUPPORTED, "flat-field-correct: inputs must be float32");
+    int rc = ffc_task_run(t, t->ctx, in[0]->data, &in[0]->req, 1, in[1]->data, 1, in[2]->data, 1, out->data);
+    if (rc)
+        return process_fail(t, rc, "flat-field-correct: %s", ufomi_last_error());
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+/* ---------------------------------------------------------------- average */
+
+/* average (ufo-average-task.c:212-216, init :222-229) */
+static const PropSpec avg_props[] = {
+    {"number", PT_UINT, 1, U32MAX, 1, NULL, 0, NULL},
+};
+
+typedef struct {
+    float *d_items;        /* the items in arrival order */
+    float *d_avg;
+    size_t elems, cap;     /* floats per item, items the store holds */
+    unsigned count, generated;
+    int averaged;
+} AvgPriv;
+
+static void avg_free(UfomiTask *t)
+{
+    AvgPriv *p = t->priv;
+    if (t->ctx) {
+        if (p->d_items)
+            ufomi_device_free(t->ctx, p->d_items);
+        if (p->d_avg)
+            ufomi_device_free(t->ctx, p->d_avg);
+    }
+    memset(p, 0, sizeof(*p));
+}
+
+static int avg_setup(UfomiTask *t, UfomiError *e)
+{
+    (void) e;
+    avg_free(t);
+    return UFOMI_OK;
+}
+
+static int avg_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    AvgPriv *p = t->priv;
+    const size_t elems = ufomi_requisition_elements(&in[0]->req);
+    if (p->elems && elems != p->elems)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_SIZE, "average: input size changed");
+    p->elems = elems;
+    *req = in[0]->req;
+    return UFOMI_OK;
+}
+
+/* process (:102-122): the sum runs in arrival order when generate averages */
+static int avg_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) out;
+    (void) req;
+    AvgPriv *p = t->priv;
+    if (in[0]->dtype != UFOMI_F32)
+        return process_fail(t, UFOMI_ERR_UNSUPPORTED, "average: input must be float32");
+    int rc;
+    if (p->count == p->cap) {
+        const size_t cap = p->cap ? 2 * p->cap : 4;
+        float *grown = NULL;
+        if ((rc = ufomi_device_alloc(t->ctx, cap * p->elems * sizeof(float), (void **) &grown)))
+            return process_fail(t, rc, "average: %s", ufomi_last_error());
+        if (p->count && (rc = ufomi_memcpy(t->ctx, grown, p->d_items, p->count * p->elems * sizeof(float), 2))) {
+            ufomi_device_free(t->ctx, grown);
+            return process_fail(t, rc, "average: %s", ufomi_last_error());
+        }
+        if (p->d_items)
+            ufomi_device_free(t->ctx, p->d_items);
+        p->d_items = grown;
+        p->cap = cap;
+    }
+    if ((rc = ufomi_memcpy(t->ctx, p->d_items + p->count * p->elems, in[0]->data, p->elems * sizeof(float), 2)))
+        return process_fail(t, rc, "average: %s", ufomi_last_error());
+    p->count++;
+    return 1;
+}
+
+/* generate (:125-151): `number` copies of the average */
+static int avg_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    AvgPriv *p = t->priv;
+    if (!p->count || p->generated >= (unsigned) t->v[0].num)
+        return 0;
+    int rc;
+    if (!p->averaged) {
+        if ((rc = ufomi_device_alloc(t->ctx, p->elems * sizeof(float), (void **) &p->d_avg)) ||
+            (rc = ufomi_average(t->ctx, p->d_items, p->elems, (int) p->count, 1, p->d_avg)))
+            return process_fail(t, rc, "average: %s", ufomi_last_error());
+        p->averaged = 1;
+    }
+    if ((rc = ufomi_memcpy(t->ctx, out->data, p->d_avg, p->elems * sizeof(float), 2)))
+        return process_fail(t, rc, "average: %s", ufomi_last_error());
+    p->generated++;
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+static void avg_finalize(UfomiTask *t)
+{
+    if (t->priv)
+        avg_free(t);
+}
+
+/* ------------------------------------------------------ center-of-rotation */
+
+/* center-of-rotation (ufo-center-of-rotation-task.c:232-246, init :250-257:
+ * angle-step pi / 180; center is read-only) */
+static const PropSpec cor_props[] = {
+    {"angle-step", PT_DOUBLE, 0.00001f, 180.0f, M_PI / 180.0, NULL, 0, NULL},
+    {"center", PT_DOUBLE, -DBL_MAX, DBL_MAX, 0.0, NULL, 0, NULL},
+};
+enum { COR_STEP, COR_CENTER };
+
+typedef struct {
+    float *stage;
+    size_t cap;
+} CorPriv;
+
+static int cor_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    (void) t;
+    (void) in;
+    (void) e;
+    req->n_dims = 0;
+    return UFOMI_OK;
+}
+
+/* process (:111-166): squared differences of the first row against the last
+ * row mirrored, for displacements -(W/2 - 1) .. W/2 - 1, the smallest score
+ * gives center = (W + index - W/2 + 1) / 2.  The reference's mirror index
+ * (width - x + 1) reaches one and two elements past the last row; those reads
+ * are 0 here. */
+static int cor_compute(UfomiTask *t, const float *sino, const UfomiRequisition *r)
+{
+    const unsigned width = (unsigned) r->dims[0], height = r->n_dims >= 2 ? (unsigned) r->dims[1] : 1;
+    if (width < 2)
+        return process_fail(t, UFOMI_ERR_ARG, "center-of-rotation: the sinogram needs at least 2 columns");
+    const float *p0 = sino, *p180 = sino + (size_t) (height - 1) * width;
+#define AT180(i) ((i) < width ? p180[(i)] : 0.0f)
+    const unsigned maxd = width / 2, n = maxd * 2 - 1;
+    float *scores = calloc(n, sizeof(float));
+    if (!scores)
+        return process_fail(t, UFOMI_ERR_MEMORY, "center-of-rotation: out of host memory");
+    for (int d = -((int) maxd) + 1; d < 0; d++) {
+        const unsigned index = (unsigned) (d + (int) maxd - 1), max_x = width - (unsigned) (-d);
+        for (unsigned x = 0; x < max_x; x++) {
+            const float diff = p0[x] - AT180(max_x - x + 1);
+            scores[index] += diff * diff;
+        }
+    }
+    for (unsigned d = 0; d < maxd; d++) {
+        const unsigned index = d + maxd - 1;
+        for (unsigned x = 0; x < width - d; x++) {
+            const float diff = p0[x + d] - AT180(width - x + 1);
+            scores[index] += diff * diff;
+        }
+    }
+#undef AT180
+    unsigned best = 0;
+    for (unsigned i = 1; i < n; i++)
+        if (scores[i] < scores[best])
+            best = i;
+    free(scores);
+    t->v[COR_CENTER].num = (width + best - maxd + 1) / 2.0;
+    return 1;
+}
+
+static int cor_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) out;
+    (void) req;
+    CorPriv *p = t->priv;
+    if (in[0]->dtype != UFOMI_F32)
+        return process_fail(t, UFOMI_ERR_UNSUPPORTED, "center-of-rotation: input must be float32");
+    const size_t n = ufomi_requisition_elements(&in[0]->req);
+    if (n > p->cap) {
+        free(p->stage);
+        p->stage = malloc(n * sizeof(float));
+        p->cap = p->stage ? n : 0;
+        if (!p->stage)
+            return process_fail(t, UFOMI_ERR_MEMORY, "center-of-rotation: out of host memory");
+    }
+    int rc = ufomi_memcpy(t->ctx, p->stage, in[0]->data, n * sizeof(float), 1);
+    if (rc)
+        return process_fail(t, rc, "center-of-rotation: %s", ufomi_last_error());
+    return cor_compute(t, p->stage, &in[0]->req);
+}
+
+static void cor_finalize(UfomiTask *t)
+{
+    CorPriv *p = t->priv;
+    if (p)
+        free(p->stage);
+}
+
+/* ---------------------------------------------------- stacked-backproject */
+
+/* stacked-backproject (ufo-stacked-backproject-task.c:612-690, init :693-718:
+ * axis-pos and angle-step are -1 after init; precision_values :37-42) */
+static const char *const precision_nicks[] = {"int8", "half", "single", NULL};
+static const PropSpec sbp_props[] = {
+    {"num-projections", PT_UINT, 0, 32768, 0, NULL, 0, NULL},
+    {"offset", PT_UINT, 0, 32768, 0, NULL, 0, NULL},
+    {"axis-pos", PT_DOUBLE, -1.0, 32768.0, -1.0, NULL, 0, NULL},
+    {"angle-step", PT_DOUBLE, -DBL_MAX, DBL_MAX, -1.0, NULL, 0, NULL},
+    {"angle-offset", PT_DOUBLE, 0.0, DBL_MAX, 0.0, NULL, 0, NULL},
+    {"precision-mode", PT_ENUM, 0, 2, 2, precision_nicks, 0, NULL},
+    {"roi-x", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"roi-y", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"roi-width", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+    {"roi-height", PT_UINT, 0, U32MAX, 0, NULL, 0, NULL},
+};
+enum { SBP_NPROJ, SBP_OFFSET, SBP_AXIS, SBP_STEP, SBP_AOFF, SBP_PREC, SBP_RX, SBP_RY, SBP_RW, SBP_RH };
+
+typedef struct {
+    unsigned n_projections, burst;
+} SbpPriv;
+
+/* get_requisition (:186-240): num-projections defaults to the first stack's
+ * sinogram height and stays; the output is [depth][roi-height][roi-width] */
+static int sbp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    SbpPriv *p = t->priv;
+    const UfomiRequisition *ir = &in[0]->req;
+    if (!p->n_projections)
+        p->n_projections = (unsigned) t->v[SBP_NPROJ].num ? (unsigned) t->v[SBP_NPROJ].num : (unsigned) ir->dims[1];
+    p->burst = (unsigned) ir->dims[1];
+    if (p->burst > p->n_projections)
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_PROJECTIONS,
+                          "Total number of projections (%u) must be greater than or equal to sinogram height (%u)",
+                          p->n_projections, p->burst);
+    req->n_dims = 3;
+    req->dims[0] = t->v[SBP_RW].num ? (size_t) t->v[SBP_RW].num : ir->dims[0];
+    req->dims[1] = t->v[SBP_RH].num ? (size_t) t->v[SBP_RH].num : ir->dims[0];
+    req->dims[2] = ir->n_dims == 3 ? ir->dims[2] : 1;
+    return UFOMI_OK;
+}
+
+/* process (:300-430): every slice of the stack in the chosen precision */
+static int sbp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    SbpPriv *p = t->priv;
+    if (in[0]->dtype != UFOMI_F32)
+        return process_fail(t, UFOMI_ERR_UNSUPPORTED, "stacked-backproject: input must be float32");
+    ufomi_bp_params bp;
+    ufomi_bp_params_init(&bp);
+    bp.num_projections = p->n_projections;
+    bp.offset = (unsigned) t->v[SBP_OFFSET].num;
+    bp.axis_pos = t->v[SBP_AXIS].num;
+    bp.angle_step = t->v[SBP_STEP].num;
+    bp.angle_offset = t->v[SBP_AOFF].num;
+    bp.roi_x = (unsigned) t->v[SBP_RX].num;
+    bp.roi_y = (unsigned) t->v[SBP_RY].num;
+    bp.roi_width = (unsigned) req->dims[0];
+    bp.roi_height = (unsigned) req->dims[1];
+    const UfomiRequisition *ir = &in[0]->req;
+    int rc = ufomi_bp_stacked(t->ctx, in[0]->data, (int) ir->dims[0], (int) ir->dims[1], (int) req->dims[2], &bp,
+                              (int) t->v[SBP_PREC].num, out->data);
+    if (rc)
+        return process_fail(t, rc, "stacked-backproject: %s", ufomi_last_error());
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
 /* -------------------------------------------------------- cone weighting */
 
 typedef struct {
@@ -700,9 +1040,9 @@ static int gbp_setup(UfomiTask *t, UfomiError *e)
     if (t->v[G_GMIN].num >= t->v[G_GMAX].num && (st == UFOMI_ST_UCHAR || st == UFOMI_ST_USHORT || st == UFOMI_ST_UINT))
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_GRAY_MAP,
                           "Gray mapping minimum must be less then the maximum");
-    if ((int) t->v[G_PARAM].num != 17)
+    if ((int) t->v[G_PARAM].num == 2)
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
-                          "general-backproject: only parameter=z is built");
+                          "general-backproject: axis-angle-z holds the tomographic angles, it cannot be swept");
     if ((int) t->v[G_COMPUTE].num != 0 || (int) t->v[G_RESULT].num != 1)
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
                           "general-backproject: compute-type and result-type float only");
@@ -777,7 +1117,8 @@ static int gbp_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
     if (!p->reconstructed) {
         ufomi_gbp_geometry g;
         gbp_fill_geometry(t, &g);
-        int rc = ufomi_gbp(t->ctx, p->d_stack, p->width, p->height, (int) p->count, &g, &p->params, p->d_volume);
+        int rc = ufomi_gbp_sweep(t->ctx, p->d_stack, p->width, p->height, (int) p->count, &g, &p->params,
+                                 (int) t->v[G_PARAM].num, p->d_volume);
         if (rc)
             return process_fail(t, rc, "general-backproject: %s", ufomi_last_error());
         p->reconstructed = 1;
@@ -1375,6 +1716,14 @@ static const TaskClass classes[] = {
      sizeof(CwPriv), cw_setup, cw_requisition, cw_process, NULL, NULL},
     {"general-backproject", gbp_props, NPROPS(gbp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
      sizeof(GbpPriv), gbp_setup, gbp_requisition, gbp_process, gbp_generate, gbp_finalize},
+    {"center-of-rotation", cor_props, NPROPS(cor_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_CPU,
+     sizeof(CorPriv), NULL, cor_requisition, cor_process, NULL, cor_finalize},
+    {"flat-field-correct", ffc_props, NPROPS(ffc_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU,
+     sizeof(FfcPriv), NULL, ffc_requisition, ffc_process, NULL, NULL},
+    {"average", avg_props, NPROPS(avg_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU, sizeof(AvgPriv),
+     avg_setup, avg_requisition, avg_process, avg_generate, avg_finalize},
+    {"stacked-backproject", sbp_props, NPROPS(sbp_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU,
+     sizeof(SbpPriv), NULL, sbp_requisition, sbp_process, NULL, NULL},
     {"transpose-projections", tp_props, NPROPS(tp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
      sizeof(TpPriv), tp_setup, tp_requisition, tp_process, tp_generate, tp_finalize},
     {"read", read_props, NPROPS(read_props), UFOMI_TASK_MODE_GENERATOR | UFOMI_TASK_MODE_CPU, sizeof(ReadPriv),
@@ -1441,18 +1790,23 @@ int ufomi_task_get_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition 
 
 unsigned ufomi_task_get_num_inputs(const UfomiTask *t)
 {
-    return t && !(t->cls->mode & UFOMI_TASK_MODE_GENERATOR) ? 1 : 0;
+    if (!t || (t->cls->mode & UFOMI_TASK_MODE_GENERATOR))
+        return 0;
+    return !strcmp(t->cls->name, "flat-field-correct") ? 3 : 1;
 }
 
 unsigned ufomi_task_get_num_dimensions(const UfomiTask *t, unsigned input)
 {
-    (void) input;
     if (!t || (t->cls->mode & UFOMI_TASK_MODE_GENERATOR))
         return 0;
     if (!strcmp(t->cls->name, "fft"))
         return (int) t->v[FFT_DIMS].num > 2 ? 3 : 2;
     if (!strcmp(t->cls->name, "ifft"))
         return (int) t->v[IFFT_DIMS].num > 2 ? 3 : 2;
+    if (!strcmp(t->cls->name, "stacked-backproject"))
+        return 3;
+    if (!strcmp(t->cls->name, "flat-field-correct") && input > 0 && t->v[FFC_SINO].num)
+        return 1;
     return 2;
 }
 
@@ -1482,14 +1836,20 @@ int ufomi_task_generate_host(UfomiTask *t, float *out, const UfomiRequisition *r
 
 int ufomi_task_process_host(UfomiTask *t, const float *in, const UfomiRequisition *req)
 {
-    if (!t || !in || !req || !t->ready || strcmp(t->cls->name, "write"))
+    if (!t || !in || !req || !t->ready)
         return 0;
     t->err[0] = '\0';
     t->err_status = 0;
-    if (write_task_write_host(t, in, req) != UFOMI_OK)
+    int ok;
+    if (!strcmp(t->cls->name, "write"))
+        ok = write_task_write_host(t, in, req) == UFOMI_OK;
+    else if (!strcmp(t->cls->name, "center-of-rotation"))
+        ok = cor_compute(t, in, req);
+    else
         return 0;
-    t->num_processed++;
-    return 1;
+    if (ok)
+        t->num_processed++;
+    return ok;
 }
 
 int ufomi_task_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)

@@ -64,6 +64,11 @@ int gbp_task_store_type(const UfomiTask *t);
 int read_task_generate_host(UfomiTask *t, float *dst, const UfomiRequisition *req);
 /* write: one item from host memory; UFOMI_OK or a status with t->err set. */
 int write_task_write_host(UfomiTask *t, const float *src, const UfomiRequisition *req);
+/* flat-field-correct over a batch: input sizes checked, then one launch. */
+int ffc_check_inputs(const UfomiTask *t, const UfomiRequisition *data, const UfomiRequisition *dark,
+                     const UfomiRequisition *flat, UfomiError *e);
+int ffc_task_run(UfomiTask *t, ufomi_ctx *ctx, const float *data, const UfomiRequisition *r, int n,
+                 const float *dark, int n_dark, const float *flat, int n_flat, float *out);
 /* write: closes the open file (the end of a stream). */
 void write_task_flush(UfomiTask *t);
 

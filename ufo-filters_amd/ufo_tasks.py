@@ -179,10 +179,16 @@ class Task:
         if self.L.ufomi_task_setup(self.h, ctx.h if ctx is not None else None, ctypes.byref(err)):
             raise TaskError(err)
 
+    @staticmethod
+    def _inputs(buf):
+        bufs = buf if isinstance(buf, (list, tuple)) else [buf]
+        return (ctypes.POINTER(Buffer) * len(bufs))(*[ctypes.pointer(b) for b in bufs])
+
     def get_requisition(self, buf=None):
-        """buf None for generators; an exhausted generator returns n_dims 0."""
+        """buf: a Buffer, a list of them for multi-input tasks, or None for generators; an
+        exhausted generator returns n_dims 0."""
         err, req = Error(), Requisition()
-        arr = (ctypes.POINTER(Buffer) * 1)(ctypes.pointer(buf)) if buf is not None else None
+        arr = self._inputs(buf) if buf is not None else None
         if self.L.ufomi_task_get_requisition(self.h, arr, ctypes.byref(req), ctypes.byref(err)):
             raise TaskError(err)
         return req
@@ -219,10 +225,15 @@ class Task:
             yield item
 
     def process(self, buf, out, req):
-        arr = (ctypes.POINTER(Buffer) * 1)(ctypes.pointer(buf))
+        """True when the item was taken.  A reductor may answer False without an error: it takes no
+        more input (ufo-core stops feeding it); any other False raises."""
+        arr = self._inputs(buf)
         ok = self.L.ufomi_task_process(self.h, arr, ctypes.byref(out), ctypes.byref(req))
         if not ok:
-            raise RuntimeError("%s.process failed: %s" % (self.name, self.L.ufomi_task_last_error(self.h).decode()))
+            msg = self.L.ufomi_task_last_error(self.h).decode()
+            if msg or not (self.mode & MODE_REDUCTOR):
+                raise RuntimeError("%s.process failed: %s" % (self.name, msg))
+        return bool(ok)
 
     def generate(self, out, req):
         return bool(self.L.ufomi_task_generate(self.h, ctypes.byref(out), ctypes.byref(req)))

@@ -33,7 +33,7 @@ EXPORTS = [
     "ufomi_version", "ufomi_filter_coefficients", "ufomi_fft_rows", "ufomi_filter_apply", "ufomi_ifft_rows",
     "ufomi_fft_nd", "ufomi_ifft_nd",
     "ufomi_ramp_filter", "ufomi_bp_params_init", "ufomi_bp_parallel",
-    "ufomi_cone_weight", "ufomi_transpose_projections", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
+    "ufomi_cone_weight", "ufomi_transpose_projections", "ufomi_bp_stacked", "ufomi_flat_field_correct", "ufomi_average", "ufomi_gbp_sweep", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
     "ufomi_gbp", "ufomi_phantom_ellipses", "ufomi_phantom_ellipsoids", "ufomi_slice_ellipsoids",
     "ufomi_sinogram_ellipses", "ufomi_add_noise", "ufomi_bench_lds_read", "ufomi_bench_copy",
     "ufomi_gbp_projection_rows", "ufomi_project_ellipsoids", "ufomi_device_alloc", "ufomi_device_free",
@@ -110,12 +110,17 @@ def load():
         "ufomi_ifft_rows": [vp, vp, ci, ci, ci, vp], "ufomi_ramp_filter": [vp, vp, ci, ci, ci, vp, ci, vp, ci],
         "ufomi_bp_params_init": [ctypes.POINTER(BpParams)],
         "ufomi_bp_parallel": [vp, vp, ci, ci, ci, ci, csz, ctypes.POINTER(BpParams), vp, csz],
+        "ufomi_bp_stacked": [vp, vp, ci, ci, ci, ctypes.POINTER(BpParams), ci, vp],
+        "ufomi_flat_field_correct": [vp, vp, ci, ci, ci, vp, ci, vp, ci, ci, ci, ci, ctypes.c_float, ctypes.c_float,
+                                     vp],
+        "ufomi_average": [vp, vp, csz, ci, ci, vp],
         "ufomi_cone_weight": [vp, vp, vp, ci, ci, ci, dp, dp, dp, dp, dp, ci],
         "ufomi_transpose_projections": [vp, vp, ci, ci, ci, ci, ci, vp],
         "ufomi_gbp_params_init": [ctypes.POINTER(GbpParams)], "ufomi_gbp_geometry_init": [ctypes.POINTER(GbpGeometry)],
         "ufomi_gbp_volume_shape": [ctypes.POINTER(GbpParams), ci, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                    ctypes.POINTER(ci)],
         "ufomi_gbp": [vp, vp, ci, ci, ci, ctypes.POINTER(GbpGeometry), ctypes.POINTER(GbpParams), vp],
+        "ufomi_gbp_sweep": [vp, vp, ci, ci, ci, ctypes.POINTER(GbpGeometry), ctypes.POINTER(GbpParams), ci, vp],
         "ufomi_phantom_ellipses": [ctypes.c_uint64, ci, cd, dp],
         "ufomi_phantom_ellipsoids": [ctypes.c_uint64, ci, cd, dp],
         "ufomi_slice_ellipsoids": [dp, ci, cd, dp],
@@ -326,6 +331,20 @@ class Context:
         check(self.L.ufomi_bp_parallel(self.h, _ptr(d_sino), dtype, W, P_in, n_slices, sino_pitch,
                                        ctypes.byref(p), _ptr(d_out), out_pitch))
 
+    def flat_field_correct(self, d_data, W, H, n, d_dark, n_dark, d_flat, n_flat, d_out, sinogram_input=False,
+                           absorption_correct=False, fix_nan_and_inf=False, dark_scale=1.0, flat_scale=1.0):
+        check(self.L.ufomi_flat_field_correct(self.h, _ptr(d_data), W, H, n, _ptr(d_dark), n_dark, _ptr(d_flat),
+                                              n_flat, int(sinogram_input), int(absorption_correct),
+                                              int(fix_nan_and_inf), dark_scale, flat_scale, _ptr(d_out)))
+
+    def average(self, d_items, elems, n, d_out, copies=1):
+        check(self.L.ufomi_average(self.h, _ptr(d_items), elems, n, copies, _ptr(d_out)))
+
+    def bp_stacked(self, d_sino, W, P_in, n_slices, params, precision, d_out):
+        """stacked-backproject of n_slices sinograms; precision 0 int8, 1 half, 2 single."""
+        check(self.L.ufomi_bp_stacked(self.h, _ptr(d_sino), W, P_in, n_slices, ctypes.byref(params), precision,
+                                      _ptr(d_out)))
+
     def cone_weight(self, d_in, d_out, W, H, n_proj, center_x, center_z, source_distance, detector_distance,
                     axis_angle_x=0.0):
         arrs = [np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)
@@ -343,6 +362,11 @@ class Context:
     def gbp(self, d_proj, W, H, P, geometry, params, d_vol):
         check(self.L.ufomi_gbp(self.h, _ptr(d_proj), W, H, P, ctypes.byref(geometry), ctypes.byref(params),
                                _ptr(d_vol)))
+
+    def gbp_sweep(self, d_proj, W, H, P, geometry, params, parameter, d_vol):
+        """general-backproject with `parameter` (index of GBP_PARAMETERS) swept over params.region."""
+        check(self.L.ufomi_gbp_sweep(self.h, _ptr(d_proj), W, H, P, ctypes.byref(geometry), ctypes.byref(params),
+                                     parameter, _ptr(d_vol)))
 
     # -- synthetic data
     def sinogram_phantom(self, ellipses, waves, radius, W, P, axis_pos, angle_step, d_out):
@@ -524,6 +548,13 @@ class Unet:
             self.close()
         except Exception:
             pass
+
+
+# the `parameter` enum of general-backproject, in order
+GBP_PARAMETERS = ["axis-angle-x", "axis-angle-y", "axis-angle-z", "volume-angle-x", "volume-angle-y", "volume-angle-z",
+                  "detector-angle-x", "detector-angle-y", "detector-angle-z", "detector-position-x",
+                  "detector-position-y", "detector-position-z", "source-position-x", "source-position-y",
+                  "source-position-z", "center-position-x", "center-position-z", "z"]
 
 
 def bp_params(num_projections=0, offset=0, axis_pos=-1.0, angle_step=-1.0, angle_offset=0.0, mode="texture",
