@@ -238,8 +238,10 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
     r = runs[first]
     # LDS bytes read per update: (v, d) float pairs, or half pairs with f16 half staging
     bpu = 4 if "true" in kernel else 8
+    # compulsory HBM bytes of one backprojection launch: the stored filtered sinograms once, the volume once
+    hbm_bytes = float(S) * P * W * (4 if first == "f32" else 2) + float(S) * W * W * 4
     return dict(updates=updates, wall=r["wall"], bp_ms=r["bp_ms"], kernel=kernel, bytes_per_update=bpu,
-                extra=extra, cpu=cpu, accuracy=accuracy, workload=workload,
+                hbm_bytes=hbm_bytes, extra=extra, cpu=cpu, accuracy=accuracy, workload=workload,
                 dtype="f32" if first == "f32" else "f32 accumulate, %s storage" % first,
                 config={"slices": S, "angles": P, "detector": W, "fft_size": N, "storage": first, "filter": filt_name},
                 traffic_key="%s %dx%dx%d %s" % (kernel, S, P, W, first),
@@ -333,6 +335,7 @@ def general(args, ctx, torch, np, ufomi, kind):
             "laminography: %d projections of %dx%d, tilt 30 deg, Ram-Lak (N %d) -> %dx%dx%d volume"
             % (P, H, W, N, nz, W, W))
     return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=4 * 4,
+                hbm_bytes=float(P) * H * W * 4 + float(nz) * W * W * 4,
                 extra={"weight_filter_ms": round(pre_ms, 3)}, cpu=cpu, accuracy=accuracy, workload=desc, dtype="f32",
                 config={"projections": P, "detector": [H, W], "volume": [nz, W, W], "fft_size": N},
                 traffic_key="%s %s" % (kernel, kind),
@@ -490,7 +493,9 @@ def batch(args, ctx, torch, np, ufomi):
              "autotune": {"tile": tile, "angle_chunk": chunk, "tuned_bp_ms": round(tune_ms, 3),
                           "seconds": round(tune_s, 2)},
              "items": info["items"], "workgroups": info["workgroups"], "filter_launches": info["filter_launches"]}
-    return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=8, extra=extra, cpu=cpu,
+    hbm_bytes = float(sum(P * W * 4 + rw * rh * 4 for W, P, rw, rh in shapes))
+    return dict(updates=updates, wall=wall, bp_ms=bp_ms, kernel=kernel, bytes_per_update=8, hbm_bytes=hbm_bytes,
+                extra=extra, cpu=cpu,
                 accuracy=accuracy, dtype="f32",
                 workload="uneven batched FBP: 256 requests, ROI log-uniform in [256,2048]^2, angles from "
                          "{360,720,1200,1800,3600}, detector width max(ROI), Ram-Lak zero-padded to 2^k >= 2W, "
@@ -734,6 +739,10 @@ def main():
         "kernel_gups": round(r["updates"] / bp_s / 1e9, 1),
         "peak_source": "measured: ufomi_bench_lds_read (ds_read_b128 on every CU)",
         "hbm_copy_GBps": round(hbm_copy / 1e9, 1),
+        # the HBM-compulsory fraction beside the on-chip one: inputs and output once per launch
+        "hbm_compulsory_bytes": r["hbm_bytes"],
+        "hbm_compulsory_GBps": round(r["hbm_bytes"] / bp_s / 1e9, 1),
+        "hbm_compulsory_frac_of_8TBps": round(r["hbm_bytes"] / bp_s / 8e12, 5),
     }
     variants = r["extra"].pop("storage_variants", None)
     roof.update(r["extra"])

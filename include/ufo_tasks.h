@@ -64,8 +64,9 @@ typedef struct UfomiTask UfomiTask;
 
 /* Task names: "fft", "filter", "ifft", "backproject", "general-backproject",
  * "cone-beam-projection-weight" (libufofilter<name>.so in the reference), and
- * "transpose-projections", "stacked-backproject", "flat-field-correct" (three
- * inputs), "average", "center-of-rotation", and
+ * "transpose-projections", "stacked-backproject", "lamino-backproject"
+ * (src/ufo-lamino-backproject-task.c: a reductor emitting one 3-D volume),
+ * "flat-field-correct" (three inputs), "average", "center-of-rotation", and
  * the file tasks "read" (src/ufo-read-task.c: a generator, TIFF / BigTIFF,
  * EDF and raw files) and "write" (src/ufo-write-task.c: a sink, TIFF / BigTIFF
  * and raw).  Returns NULL for an unknown name.

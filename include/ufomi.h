@@ -430,6 +430,41 @@ int ufomi_gbp_sweep(ufomi_ctx *ctx, const float *d_projections, int W, int H, in
                     const ufomi_gbp_geometry *geometry, const ufomi_gbp_params *params, int parameter,
                     void *d_volume);
 
+/* ------------------------------------------------------ lamino-backproject */
+
+/* Parameter swept along the volume's third axis (parameter_values,
+ * src/ufo-lamino-backproject-task.c:48-60). */
+enum { UFOMI_LAMINO_Z = 0, UFOMI_LAMINO_X_CENTER = 1, UFOMI_LAMINO_LAMINO_ANGLE = 2, UFOMI_LAMINO_ROLL_ANGLE = 3 };
+
+/* Properties of `lamino-backproject` (:688-783, defaults of init :796-831). */
+typedef struct {
+    unsigned num_projections;
+    double overall_angle;        /* default pi; normalisation |overall| / num_projections */
+    double tomo_angle;           /* -FLT_MAX (default): projection i at overall * i / num_projections */
+    double lamino_angle, roll_angle;
+    double center[2];            /* (x, y) rotation axes on the projection */
+    int projection_offset[2];    /* (x, y) of a cropped projection */
+    int x_region[3], y_region[3];/* (from, to, step), integers */
+    double region[3];            /* the swept parameter (from, to, step) */
+    double z;                    /* slice z when the parameter is not z */
+    int parameter;               /* UFOMI_LAMINO_* */
+    int addressing_mode;         /* ufomi_addressing, default clamp */
+} ufomi_lamino_params;
+
+void ufomi_lamino_params_init(ufomi_lamino_params *p);
+/* Volume dimensions of get_requisition (:247-262). */
+int ufomi_lamino_volume_shape(const ufomi_lamino_params *p, int *nx, int *ny, int *nz);
+/*
+ * `lamino-backproject` (src/ufo-lamino-backproject-task.c:285-480,
+ * src/kernels/templates/{z,center,lamino,roll}_template.in, definitions.in):
+ * P projections [H][W] (the first P of num_projections) -> volume
+ * [nz][ny][nx] float32.  Each projection's parallel-beam map (tomographic
+ * rotation, laminographic tilt, roll about the centre) is composed in FP64 and
+ * run by the general backprojector's kernel.
+ */
+int ufomi_lamino_bp(ufomi_ctx *ctx, const float *d_projections, int W, int H, int P,
+                    const ufomi_lamino_params *params, float *d_volume);
+
 /* ------------------------------------------------- synthetic phantom data */
 
 /*

@@ -19,6 +19,7 @@ ADDR = {"clamp": 0, "clamp_to_edge": 1}
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _lib = None
 
 
@@ -68,6 +69,9 @@ def lib():
                               ctypes.POINTER(OrcGbpFlags), c_int, c_int, c_int, c_int,
                               c_double, c_double, c_double, c_double, c_double, c_double,
                               c_double, c_int, c_int, _f64p]
+        L.orc_lamino.argtypes = [_f32p, c_int, c_int, c_int, c_uint, c_float, c_float, c_float, c_float,
+                                 c_float, c_float, c_int, c_int, _i32p, _i32p, _f32p, c_float, c_int, c_int,
+                                 c_int, c_int, c_int, c_int, c_int, _f64p]
         _lib = L
     return _lib
 
@@ -427,6 +431,37 @@ def gbp(projs, num_projections=None, overall_angle=2 * math.pi, region=(0.0, 0.0
     lib().orc_gbp(projs, w, h, p, geoms, ctypes.byref(fl), ADDR[addressing_mode], nx, ny, nz,
                   x0, xs, y0, ys, float(zstart), float(zstep), float(overall_angle),
                   0 if mode == "f64" else 1, int(burst), out)
+    return out
+
+
+LAMINO_PARAMETERS = ["z", "x-center", "lamino-angle", "roll-angle"]
+
+
+def lamino_shape(x_region, y_region, region):
+    """REGION_SIZE for the integer x/y regions and ceil((stop - start) / step) in float32 for the
+    swept region (src/ufo-lamino-backproject-task.c:41-43, 247-262)."""
+    size = lambda r: 0 if r[2] == 0 else int((r[1] - r[0] - 1) / r[2]) + 1  # C division truncates
+    r = np.asarray(region, np.float32)
+    return size(x_region), size(y_region), int(math.ceil(float((r[1] - r[0]) / r[2])))
+
+
+def lamino(projs, num_projections=None, overall_angle=math.pi, tomo_angle=-3.4028234663852886e38,
+           lamino_angle=0.0, roll_angle=0.0, center=(0.0, 0.0), projection_offset=(0, 0),
+           x_region=(0, 0, 0), y_region=(0, 0, 0), region=(0.0, 0.0, 0.0), z=0.0, parameter="z",
+           addressing_mode="clamp", mode="f64", burst=16):
+    """lamino-backproject (src/ufo-lamino-backproject-task.c:285-480 and the kernel templates),
+    FP64 (mode "f64") or float32 with bursts of 16 projections ("ref32").  projs: [P][H][W].
+    Returns [nz][ny][nx] float64."""
+    projs = np.ascontiguousarray(projs, dtype=np.float32)
+    p, h, w = projs.shape
+    num_projections = num_projections or p
+    nx, ny, nz = lamino_shape(x_region, y_region, region)
+    out = np.empty((nz, ny, nx), dtype=np.float64)
+    lib().orc_lamino(projs, w, h, p, num_projections, overall_angle, tomo_angle, lamino_angle, roll_angle,
+                     center[0], center[1], int(projection_offset[0]), int(projection_offset[1]),
+                     np.asarray(x_region, np.int32), np.asarray(y_region, np.int32),
+                     np.asarray(region, np.float32), z, LAMINO_PARAMETERS.index(parameter),
+                     ADDR[addressing_mode], nx, ny, nz, 0 if mode == "f64" else 1, burst, out)
     return out
 
 

@@ -36,6 +36,7 @@
  *                          lamino tilt 0 == tomography, brute force on tiny
  *                          inputs.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -727,4 +728,107 @@ void orc_gbp(const float *projs, int w, int h, int P, const orc_gbp_geom *g, con
             }
         }
     }
+}
+
+/*
+ * lamino-backproject (src/ufo-lamino-backproject-task.c:285-480 with the
+ * kernel templates src/kernels/templates/common.in, z_template.in,
+ * center_template.in, lamino_template.in, roll_template.in and the rotate()
+ * macro of definitions.in:1-6), one voxel at a time.
+ *   parameter 0 (z):            slice iz at z = region[0] + iz region[2]
+ *   parameter 1 (x-center):     x centre region[0] + iz region[2] - offset_x,
+ *                               rotate() pivots on the first value
+ *   parameter 2 (lamino-angle): lamino angle region[0] + iz region[2]
+ *   parameter 3 (roll-angle):   roll angle region[0] + iz region[2]
+ * mode 0: FP64 throughout; mode 1: float32 with the kernel's mad()s as fmaf,
+ * sums of `burst` projections added to the volume one burst at a time.
+ */
+void orc_lamino(const float *projs, int w, int h, int P, unsigned num_projections, float overall_angle,
+                float tomo_angle, float lamino_angle, float roll_angle, float center_x, float center_y,
+                int offset_x, int offset_y, const int *x_region, const int *y_region, const float *region,
+                float z, int parameter, int addr, int nx, int ny, int nz, int mode, int burst, double *out)
+{
+    const float norm32 = fabsf (overall_angle) / num_projections;
+    float *tomo = malloc (sizeof (float) * (size_t) P);
+    for (int i = 0; i < P; i++)
+        tomo[i] = tomo_angle > -FLT_MAX ? tomo_angle : overall_angle * (unsigned) i / num_projections;
+    #pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (int iz = 0; iz < nz; iz++) {
+        for (int iy = 0; iy < ny; iy++) {
+            for (int ix = 0; ix < nx; ix++) {
+                const size_t o = ((size_t) iz * ny + iy) * nx + ix;
+                if (mode == 0) {
+                    const double swept = (double) region[0] + iz * (double) region[2];
+                    const double x = x_region[0] + (double) ix * x_region[2];
+                    const double y = y_region[0] + (double) iy * y_region[2];
+                    const double vz = parameter == 0 ? swept : z;
+                    const double xc0 = (parameter == 1 ? (double) region[0] : (double) center_x) - offset_x;
+                    const double xc = parameter == 1 ? swept - offset_x : xc0;
+                    const double yc = (double) center_y - offset_y;
+                    const double l = parameter == 2 ? swept : lamino_angle;
+                    const double r = parameter == 3 ? swept : roll_angle;
+                    const double sl = sin (l), cl = cos (l), sr = sin (-r), cr = cos (-r);
+                    double acc = 0.0;
+                    for (int i = 0; i < P; i++) {
+                        const double t = tomo_angle > -FLT_MAX ? tomo_angle
+                                                               : (double) overall_angle * i / num_projections;
+                        double px = x * cos (t) + y * sin (t) + xc;
+                        double py = x * cl * sin (t) - y * cl * cos (t) + vz * sl + yc;
+                        px -= xc0;
+                        py -= yc;
+                        px = px * cr + py * sr;
+                        py = -px * sr + py * cr;
+                        px += xc0;
+                        py += yc;
+                        acc += tex2_d (projs + (size_t) i * w * h, w, h, px, py, addr);
+                    }
+                    out[o] = acc * fabs ((double) overall_angle) / num_projections;
+                } else {
+                    const float vx = fmaf ((float) ix, (float) x_region[2], (float) x_region[0]);
+                    const float vy = fmaf ((float) iy, (float) y_region[2], (float) y_region[0]);
+                    const float xc0 = (parameter == 1 ? region[0] : center_x) - offset_x;
+                    const float xc = parameter == 1 ? fmaf ((float) iz, region[2], xc0) : xc0;
+                    const float yc = center_y - offset_y;
+                    float sl = sinf (lamino_angle), cl = cosf (lamino_angle);
+                    float sr = sinf (-roll_angle), cr = cosf (-roll_angle);
+                    float vz = z;
+                    if (parameter == 0) vz = fmaf ((float) iz, region[2], region[0]);
+                    if (parameter == 2) {
+                        const float a = fmaf ((float) iz, region[2], region[0]);
+                        sl = sinf (a);
+                        cl = cosf (a);
+                    }
+                    if (parameter == 3) {
+                        const float a = fmaf ((float) iz, -region[2], -region[0]);
+                        sr = sinf (a);
+                        cr = cosf (a);
+                    }
+                    const float tmp = fmaf (vz, sl, yc), tx = vx * cl, ty = -vy * cl;
+                    float vol = 0.0f, result = 0.0f;
+                    const int full = (int) (num_projections / burst * burst);
+                    for (int i = 0; i < P; i++) {
+                        const float st = (float) sin ((double) tomo[i]), ct = (float) cos ((double) tomo[i]);
+                        float px = fmaf (vx, ct, fmaf (vy, st, xc));
+                        float py = fmaf (tx, st, fmaf (ty, ct, tmp));
+                        px -= xc0;
+                        py -= yc;
+                        px = px * cr + py * sr;
+                        py = -px * sr + py * cr;
+                        px += xc0;
+                        py += yc;
+                        const float s = tex2_f (projs + (size_t) i * w * h, w, h, px, py, addr);
+                        if (i >= full) {
+                            vol = i == 0 ? s * norm32 : vol + s * norm32;
+                        } else {
+                            result = i % burst == 0 ? s : result + s;
+                            if (i % burst == burst - 1)
+                                vol = i == burst - 1 ? result * norm32 : vol + result * norm32;
+                        }
+                    }
+                    out[o] = vol;
+                }
+            }
+        }
+    }
+    free (tomo);
 }

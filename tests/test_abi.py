@@ -109,6 +109,11 @@ REFERENCE_PROPERTIES = {
                             "source-position-y": -math.inf, "num-projections": 0, "compute-type": "float",
                             "result-type": "float", "store-type": "float", "overall-angle": 2 * math.pi,
                             "addressing-mode": "clamp", "gray-map-min": 0, "gray-map-max": 0},
+    "lamino-backproject": {"x-region": [0.0, 0.0, 0.0], "y-region": [0.0, 0.0, 0.0], "z": 0.0,
+                           "region": [0.0, 0.0, 0.0], "projection-offset": [0.0, 0.0], "center": [0.0, 0.0],
+                           "num-projections": 0, "overall-angle": float(np.float32(math.pi)),
+                           "tomo-angle": -float(np.finfo(np.float32).max), "lamino-angle": 0.0, "roll-angle": 0.0,
+                           "parameter": "z", "addressing-mode": "clamp"},
 }
 
 
@@ -161,6 +166,22 @@ def test_task_property_ranges_and_enums():
         gbp.set_property("no-such-property", 1)
     assert gbp.mode & ufo_tasks.MODE_REDUCTOR
     assert bp.mode & ufo_tasks.MODE_PROCESSOR
+
+
+def test_lamino_shape_and_region_property():
+    """REGION_SIZE on integer regions, ceil in float32 for the swept region; set_region
+    (ufo-lamino-backproject-task.c:124-136) ignores a region whose "from" exceeds "to"."""
+    p = ufomi.lamino_params(10, x_region=(-10, 10, 3), y_region=(0, 7, 2), region=(0.0, 1.0, 0.3))
+    assert ufomi.lamino_shape(p) == (4, 4, 7)
+    assert oracle.lamino_shape((-10, 10, 3), (0, 7, 2), (0.0, 1.0, 0.3)) == (7, 4, 4)
+    with pytest.raises(ufomi.UfomiError):
+        ufomi.lamino_shape(ufomi.lamino_params(10))
+    t = ufo_tasks.Task("lamino-backproject")
+    t.props.x_region = [-4, 4, 1]
+    t.props.x_region = [5, 1, 1]
+    assert t.props.x_region == [-4.0, 4.0, 1.0]
+    t.props.parameter = "roll-angle"
+    assert t.props.parameter == "roll-angle" and t.mode & ufo_tasks.MODE_REDUCTOR
 
 
 def test_unknown_task_and_pipeline_errors():

@@ -439,3 +439,137 @@ def test_stacked_oracle_single_is_backproject_of_negated_angles():
     rounded = oracle.stacked_bp_ref32(sinos.astype(np.float16).astype(np.float32), "single", axis_pos=15.25,
                                       roi_x=3, roi_width=24, roi_height=20)
     np.testing.assert_array_equal(half, rounded)
+
+
+# ------------------------------------------------------------ lamino-backproject
+
+def test_lamino_oracle_at_right_angle_is_backproject():
+    """lamino-angle pi/2, roll 0: slice z reads projection row z + y-centre, u = x cos t + y sin t + x-centre,
+    which is backproject's h = axis + bx cos + by sin with bx = x (axis 15.5, x-region from -15)."""
+    rng = np.random.default_rng(41)
+    p, h, w = 24, 5, 32
+    projs = rng.standard_normal((p, h, w)).astype(np.float32)
+    vol = oracle.lamino(projs, lamino_angle=math.pi / 2, center=(15.5, 0.5), x_region=(-15, 17, 1),
+                        y_region=(-15, 17, 1), region=(0.0, float(h), 1.0))
+    for r in range(h):
+        ref = oracle.bp_texture_f64(projs[:, r, :], axis_pos=15.5)
+        # float32 pi/2 leaves cos(l) = -4.4e-8: rows blend by at most 1e-6
+        assert rel_l2(vol[r], ref) < 2e-6
+
+
+def test_lamino_oracle_is_general_backproject_laminography():
+    """lamino-backproject(l) equals general-backproject with axis-angle-x = l - pi/2 and axis-angle-z = -t
+    (the latter pinned by the matrix brute force above)."""
+    rng = np.random.default_rng(42)
+    p, h, w = 12, 20, 24
+    projs = rng.standard_normal((p, h, w)).astype(np.float32)
+    lam, xc, yc = 0.6, 11.25, 9.5
+    vol = oracle.lamino(projs, lamino_angle=lam, center=(xc, yc), x_region=(-8, 9, 2), y_region=(-6, 6, 1),
+                        region=(-3.0, 3.0, 1.5))
+    lam32 = float(np.float32(lam))
+    pi32 = float(np.float32(math.pi))  # the float property
+    t = [-pi32 * i / p for i in range(p)]
+    ref = oracle.gbp(projs, overall_angle=pi32, axis_angle_x=[lam32 - math.pi / 2], axis_angle_z=t,
+                     center_position_x=[xc], center_position_z=[yc], x_region=(-8.0, 10.0, 2.0),
+                     y_region=(-6.0, 6.0, 1.0), region=(-3.0, 3.0, 1.5))
+    assert vol.shape == ref.shape == (4, 12, 9)
+    np.testing.assert_allclose(vol, ref, atol=1e-9)
+
+
+def test_lamino_oracle_roll_closed_forms():
+    """definitions.in:1-6 rotate() updates x before computing y from it.  Roll pi is a point reflection
+    about the centre either way; roll pi/2 maps (a, b) to (-b, -b) (a true rotation would give (-b, a)),
+    so projections linear in u - v (which bilinear sampling reproduces) reconstruct to a constant."""
+    rng = np.random.default_rng(43)
+    p, h, w = 8, 16, 16
+    projs = rng.standard_normal((p, h, w)).astype(np.float32)
+    kw = dict(lamino_angle=0.7, center=(8.0, 8.0), x_region=(-4, 4, 1), y_region=(-4, 4, 1), region=(-1.0, 1.0, 1.0))
+    flipped = oracle.lamino(projs[:, ::-1, ::-1], **kw)
+    # the float32 roll property leaves sin(pi) = -8.7e-8: sub-1e-6 pixel shifts
+    np.testing.assert_allclose(oracle.lamino(projs, roll_angle=math.pi, **kw), flipped, atol=1e-5)
+    u, v = np.meshgrid(np.arange(w), np.arange(h))
+    diag = np.broadcast_to(0.3 * (u - v) + 1.7, (p, h, w)).astype(np.float32)
+    kw2 = dict(kw, center=(8.5, 8.5), x_region=(-2, 2, 1), y_region=(-2, 2, 1))
+    rolled = oracle.lamino(diag, roll_angle=math.pi / 2, **kw2)
+    # every sample at u - v = 0 inside the image: 1.7 per projection, times pi / p
+    np.testing.assert_allclose(rolled, float(np.float32(math.pi)) * 1.7, atol=1e-5)
+    assert np.ptp(oracle.lamino(diag, roll_angle=0.0, **kw2)) > 1e-2
+
+
+def test_lamino_oracle_sweeps_are_slices():
+    """Each swept slice equals a z run with the parameter at that value; the x-center sweep's rotate()
+    pivots on the first centre, which at roll pi equals a z run with centre 2 c0 - c_k.  (Swept values are
+    float32 region values stepped in double, z runs take float32 properties: 1e-7 apart.)"""
+    rng = np.random.default_rng(44)
+    p, h, w = 10, 14, 18
+    projs = rng.standard_normal((p, h, w)).astype(np.float32)
+    base = dict(lamino_angle=0.5, roll_angle=0.1, center=(9.0, 7.0), x_region=(-5, 5, 1), y_region=(-4, 4, 1),
+                z=0.75)
+    zrun = lambda **kw: oracle.lamino(projs, **dict(base, region=(0.75, 1.75, 1.0), **kw))[0]
+    sw = oracle.lamino(projs, **dict(base, parameter="lamino-angle", region=(0.3, 0.9, 0.2)))
+    for k in range(3):
+        np.testing.assert_allclose(sw[k], zrun(lamino_angle=0.3 + 0.2 * k), atol=1e-6)
+    sw = oracle.lamino(projs, **dict(base, parameter="roll-angle", region=(-0.2, 0.2, 0.2)))
+    for k in range(2):
+        np.testing.assert_allclose(sw[k], zrun(roll_angle=-0.2 + 0.2 * k), atol=1e-6)
+    sw = oracle.lamino(projs, **dict(base, roll_angle=math.pi, parameter="x-center", region=(8.0, 10.0, 0.5)))
+    for k in range(4):
+        c = 8.0 + 0.5 * k
+        np.testing.assert_allclose(sw[k], zrun(roll_angle=math.pi, center=(2 * 8.0 - c, 7.0)), atol=1e-5)
+    ref32 = oracle.lamino(projs, **dict(base, parameter="x-center", region=(8.0, 10.0, 0.5)), mode="ref32")
+    f64 = oracle.lamino(projs, **dict(base, parameter="x-center", region=(8.0, 10.0, 0.5)))
+    assert rel_l2(ref32, f64) < 1e-5
+
+
+def test_lamino_oracle_ref32_bursts():
+    """float32 semantics with bursts of 16: close to FP64 with all projections; with fewer projections
+    than num-projections the reference launches its 16-wide kernel only on full bursts, so a trailing
+    partial burst is left out."""
+    rng = np.random.default_rng(45)
+    projs = rng.standard_normal((40, 12, 16)).astype(np.float32)
+    kw = dict(lamino_angle=0.4, center=(8.0, 6.0), x_region=(-6, 6, 1), y_region=(-6, 6, 1), region=(-1.0, 1.0, 1.0))
+    assert rel_l2(oracle.lamino(projs, mode="ref32", **kw), oracle.lamino(projs, **kw)) < 1e-5
+    part = oracle.lamino(projs[:20], num_projections=40, mode="ref32", **kw)
+    first16 = oracle.lamino(projs[:16], num_projections=40, **kw)
+    assert rel_l2(part, first16) < 1e-5
+
+
+# ------------------------------------------------------ flat-field-correct, average
+
+def test_ffc_oracle_closed_forms():
+    """Data built as dark + k (flat - dark) corrects to k, and as dark + (flat - dark) exp(-a) to a
+    with absorption; scales act on dark and flat; a sinogram-input row broadcasts over the rows;
+    zero denominators give NaN/Inf, or 0 with the fix."""
+    rng = np.random.default_rng(51)
+    dark = rng.uniform(90, 110, (6, 9))
+    flat = rng.uniform(900, 1000, (6, 9))
+    k = rng.uniform(0.1, 0.9, (3, 6, 9))
+    data = (dark + k * (flat - dark)).astype(np.float32)
+    np.testing.assert_allclose(oracle.flat_field_correct_ref32(data, dark, flat), k, rtol=1e-4)
+    att = -np.log(k)
+    np.testing.assert_allclose(oracle.flat_field_correct_ref32(data, dark, flat, absorption_correct=True), att,
+                               rtol=1e-4, atol=1e-5)
+    np.testing.assert_array_equal(oracle.flat_field_correct_ref32(data, dark, flat, dark_scale=0.5, flat_scale=2.0),
+                                  oracle.flat_field_correct_ref32(data, np.float32(dark) * np.float32(0.5),
+                                                                  np.float32(flat) * np.float32(2.0)))
+    row_d, row_f = dark[0], flat[0]
+    np.testing.assert_array_equal(oracle.flat_field_correct_ref32(data, row_d, row_f, sinogram_input=True),
+                                  oracle.flat_field_correct_ref32(data, np.tile(row_d, (6, 1)),
+                                                                  np.tile(row_f, (6, 1))))
+    z = data.copy()
+    z[0, 0, 0] = np.float32(dark[0, 0])
+    bad = oracle.flat_field_correct_ref32(z, dark, flat, absorption_correct=True)
+    assert np.isinf(bad[0, 0, 0])
+    assert oracle.flat_field_correct_ref32(z, dark, flat, absorption_correct=True, fix_nan_and_inf=True)[0, 0, 0] == 0
+
+
+def test_average_oracle_closed_forms():
+    """Integer-valued items average exactly to the mean; identical items average to themselves; the sum is
+    float32 in arrival order (1e8 + 1 - 1e8 loses the 1, as the reference's += does)."""
+    rng = np.random.default_rng(52)
+    items = rng.integers(-1000, 1000, (7, 5, 4)).astype(np.float32)
+    np.testing.assert_array_equal(oracle.average_ref32(items), (items.astype(np.float64).sum(0) / 7).astype(np.float32))
+    one = rng.standard_normal((5, 4)).astype(np.float32)
+    np.testing.assert_allclose(oracle.average_ref32(np.stack([one] * 4)), one, rtol=1e-7)
+    seq = np.array([[1e8], [1.0], [-1e8]], np.float32)
+    assert oracle.average_ref32(seq)[0] == 0.0

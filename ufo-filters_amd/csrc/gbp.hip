@@ -17,6 +17,8 @@
 // re-writes the volume every `burst` projections).
 #include "ufomi_internal.h"
 
+#include <cfloat>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -707,6 +709,127 @@ extern "C" int ufomi_gbp_sweep(ufomi_ctx *ctx, const float *d_proj, int W, int H
         *field = &value;
         *count = 1;
         if ((rc = ufomi_gbp(ctx, d_proj, W, H, P, &gk, &pk, (char *) d_vol + (size_t) k * slice))) return rc;
+    }
+    return UFOMI_OK;
+}
+
+extern "C" void ufomi_lamino_params_init(ufomi_lamino_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->overall_angle = M_PI;
+    p->tomo_angle = -FLT_MAX;
+    p->parameter = UFOMI_LAMINO_Z;
+    p->addressing_mode = UFOMI_ADDR_CLAMP;
+}
+
+extern "C" int ufomi_lamino_volume_shape(const ufomi_lamino_params *p, int *nx, int *ny, int *nz)
+{
+    if (!p || !nx || !ny || !nz)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid arguments");
+    // REGION_SIZE and get_requisition (ufo-lamino-backproject-task.c:41-43, 247-262)
+    auto size = [](const int *r) { return r[2] == 0 ? 0 : (r[1] - r[0] - 1) / r[2] + 1; };
+    *nx = size(p->x_region);
+    *ny = size(p->y_region);
+    const float start = (float) p->region[0], stop = (float) p->region[1], step = (float) p->region[2];
+    *nz = step == 0.0f ? 0 : (int) ceil((double) ((stop - start) / step));
+    if (*nx <= 0 || *ny <= 0 || *nz <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "lamino-backproject: x-region, y-region and region must each give at "
+                                         "least one voxel");
+    return UFOMI_OK;
+}
+
+// The map of lamino_template.in / z_template.in / center_template.in /
+// roll_template.in and rotate() (definitions.in) for tomographic angle t, as
+// rows over (x, y, z, 1):
+//   a = x cos t + y sin t + xc - xr,  b = x cos l sin t - y cos l cos t + z sin l,
+//   a' = a cos r + b sin r,  b' = -a' sin r + b cos r   (rotate() reuses the new a'),
+//   u = a' + xr,  v = b' + yc
+// with xc the (swept) x centre, xr the x centre rotate() pivots on (the first
+// value of a sweep) and r = -roll.
+static void lamino_rows(double t, double xc, double xr, double yc, double lam, double roll, double *m)
+{
+    const double ct = cos(t), st = sin(t), cl = cos(lam), sl = sin(lam), cr = cos(-roll), sr = sin(-roll);
+    const double a[4] = {ct, st, 0.0, xc - xr}, b[4] = {cl * st, -cl * ct, sl, 0.0};
+    for (int k = 0; k < 4; k++) {
+        const double a2 = a[k] * cr + b[k] * sr;
+        const double b2 = -a2 * sr + b[k] * cr;
+        m[k] = a2 + (k == 3 ? xr : 0.0);
+        m[4 + k] = b2 + (k == 3 ? yc : 0.0);
+        m[8 + k] = k == 3 ? 1.0 : 0.0;
+        m[12 + k] = k == 3 ? 1.0 : 0.0;
+    }
+}
+
+extern "C" int ufomi_lamino_bp(ufomi_ctx *ctx, const float *d_proj, int W, int H, int P, const ufomi_lamino_params *p,
+                               float *d_vol)
+{
+    int rc = ufomi_enter(ctx);
+    if (rc) return rc;
+    if (!d_proj || !d_vol || !p || W <= 0 || H <= 0 || P <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid lamino backprojection arguments");
+    if (!p->num_projections)
+        return ufomi_fail(UFOMI_ERR_NUM_PROJECTIONS, "Number of projections not set");
+    if (p->parameter < UFOMI_LAMINO_Z || p->parameter > UFOMI_LAMINO_ROLL_ANGLE)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown parameter %d", p->parameter);
+    const int addr = p->addressing_mode;
+    if (addr == UFOMI_ADDR_REPEAT || addr == UFOMI_ADDR_MIRRORED_REPEAT)
+        return ufomi_fail(UFOMI_ERR_UNSUPPORTED,
+                          "repeat addressing needs normalized coordinates; unnormalized sampling is undefined");
+    if (addr != UFOMI_ADDR_CLAMP && addr != UFOMI_ADDR_CLAMP_TO_EDGE && addr != UFOMI_ADDR_NONE)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown addressing mode %d", addr);
+    const int kaddr = addr == UFOMI_ADDR_CLAMP ? UFOMI_ADDR_CLAMP : UFOMI_ADDR_CLAMP_TO_EDGE;
+    int nx, ny, nz;
+    if ((rc = ufomi_lamino_volume_shape(p, &nx, &ny, &nz))) return rc;
+    if (nz > 65535)
+        return ufomi_fail(UFOMI_ERR_ARG, "at most 65535 slices per call");
+
+    // process (:336-381): tomographic angle tomo-angle if set, else overall * i / n in float
+    std::vector<double> tomo(P);
+    for (int i = 0; i < P; i++)
+        tomo[i] = p->tomo_angle > -FLT_MAX ? (double) (float) p->tomo_angle
+                                           : (double) ((float) p->overall_angle * (float) i / (float) p->num_projections);
+    const double xc0 = p->center[0] - p->projection_offset[0], yc = p->center[1] - p->projection_offset[1];
+    ufomi_gbp_params g;
+    ufomi_gbp_params_init(&g);
+    g.num_projections = (unsigned) P;
+    g.overall_angle = fabs(p->overall_angle) * P / p->num_projections;  // norm |overall| / num-projections
+    g.x_region[0] = p->x_region[0];
+    g.x_region[1] = p->x_region[1];
+    g.x_region[2] = p->x_region[2];
+    g.y_region[0] = p->y_region[0];
+    g.y_region[1] = p->y_region[1];
+    g.y_region[2] = p->y_region[2];
+    Flags f = {false, false, false, false, false, true};
+    std::vector<float> mats(16 * (size_t) P);
+    double m[16];
+    if (p->parameter == UFOMI_LAMINO_Z) {
+        for (int k = 0; k < 3; k++)
+            g.region[k] = p->region[k];
+        for (int i = 0; i < P; i++) {
+            lamino_rows(tomo[i], xc0, xc0, yc, p->lamino_angle, p->roll_angle, m);
+            for (int k = 0; k < 16; k++)
+                mats[16 * (size_t) i + k] = (float) m[k];
+        }
+        return gbp_launch(ctx, d_proj, W, H, (unsigned) P, mats, f, kaddr, &g, nx, ny, nz, d_vol);
+    }
+    // a swept parameter: slice k at z with the parameter at region[0] + k region[2]
+    g.region[0] = p->z;
+    g.region[1] = p->z + 1.0;
+    g.region[2] = 1.0;
+    const double xr = p->parameter == UFOMI_LAMINO_X_CENTER ? p->region[0] - p->projection_offset[0] : xc0;
+    for (int s = 0; s < nz; s++) {
+        const double v = p->region[0] + (double) s * p->region[2];
+        const double xc = p->parameter == UFOMI_LAMINO_X_CENTER ? v - p->projection_offset[0] : xc0;
+        const double lam = p->parameter == UFOMI_LAMINO_LAMINO_ANGLE ? v : p->lamino_angle;
+        const double roll = p->parameter == UFOMI_LAMINO_ROLL_ANGLE ? v : p->roll_angle;
+        for (int i = 0; i < P; i++) {
+            lamino_rows(tomo[i], xc, xr, yc, lam, roll, m);
+            for (int k = 0; k < 16; k++)
+                mats[16 * (size_t) i + k] = (float) m[k];
+        }
+        if ((rc = gbp_launch(ctx, d_proj, W, H, (unsigned) P, mats, f, kaddr, &g, nx, ny, 1,
+                             d_vol + (size_t) s * nx * ny)))
+            return rc;
     }
     return UFOMI_OK;
 }

@@ -481,20 +481,14 @@ static int run_batch(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const
                     goto out;
                 }
             }
-            /* general-backproject stores its volume in the store type */
-            const int st = gbp_task_store_type(t);
-            static const size_t st_size[] = {2, 4, 8, 1, 2, 4};
-            int nz = 0;
-            {
-                ufomi_gbp_params gp;
-                ufomi_gbp_params_init(&gp);
-                const PropValue *reg = &t->v[3];
-                for (int k = 0; k < 3; k++)
-                    gp.region[k] = reg->n > k ? reg->arr[k] : 0.0;
-                int nx, ny;
-                ufomi_gbp_volume_shape(&gp, (int) cur.buf.req.dims[0], &nx, &ny, &nz);
+            /* general-backproject emits its slices in the store type, lamino-backproject one volume */
+            int nz = 0, dtype = UFOMI_F32;
+            size_t esize = sizeof(float);
+            if ((rc = reductor_output_items(t, &cur.buf.req, &nz, &esize, &dtype))) {
+                task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, rc, "%s", ufomi_last_error());
+                goto out;
             }
-            if ((rc = batch_alloc(ctx, &next, &req, nz, st_size[st], UFOMI_F32, UFOMI_LAYOUT_REAL)))
+            if ((rc = batch_alloc(ctx, &next, &req, nz, esize, UFOMI_F32, UFOMI_LAYOUT_REAL)))
                 goto fail_alloc;
             int produced = 0;
             for (;;) {
@@ -509,7 +503,7 @@ static int run_batch(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const
                 goto out;
             }
             next.n = produced;
-            next.buf.dtype = st == UFOMI_ST_FLOAT ? UFOMI_F32 : (st == UFOMI_ST_HALF ? UFOMI_F16 : -1 - st);
+            next.buf.dtype = dtype;
         } else {
             if ((rc = ufomi_task_get_requisition(t, ins, &req, e)))
                 goto out;

@@ -9,6 +9,7 @@
  *   backproject                  src/ufo-backproject-task.c
  *   cone-beam-projection-weight  src/ufo-cone-beam-projection-weight-task.c
  *   general-backproject          src/ufo-general-backproject-task.c
+ *   lamino-backproject           src/ufo-lamino-backproject-task.c
  *   center-of-rotation           src/ufo-center-of-rotation-task.c
  *   flat-field-correct           src/ufo-flat-field-correct-task.c
  *   average                      src/ufo-average-task.c
@@ -231,6 +232,7 @@ static int parse_double(const char *s, double *out)
 }
 
 static int write_bits_property(UfomiTask *t, double bits);
+static int lamino_region_property(const char *name, const double *vals, int n);
 
 int ufomi_task_set_property(UfomiTask *t, const char *name, const char *value)
 {
@@ -298,6 +300,9 @@ int ufomi_task_set_property(UfomiTask *t, const char *name, const char *value)
             if (*p == ',')
                 p++;
         }
+        if (!strcmp(t->cls->name, "lamino-backproject") && (!strcmp(name, "x-region") || !strcmp(name, "y-region")) &&
+            !lamino_region_property(name, vals, n))
+            return UFOMI_OK;
         free(v->arr);
         v->arr = n ? malloc((size_t) n * sizeof(double)) : NULL;
         if (n)
@@ -1143,6 +1148,191 @@ static void gbp_finalize(UfomiTask *t)
     }
 }
 
+/* ----------------------------------------------------- lamino-backproject */
+
+static const char *const lamino_parameter_nicks[] = {"z", "x-center", "lamino-angle", "roll-angle", NULL};
+
+/* lamino-backproject (ufo-lamino-backproject-task.c:660-790, init :796-831) */
+static const PropSpec lbp_props[] = {
+    {"x-region", PT_DARRAY, -2147483648.0, 2147483647.0, 0, NULL, 3, NULL},
+    {"y-region", PT_DARRAY, -2147483648.0, 2147483647.0, 0, NULL, 3, NULL},
+    {"z", PT_FLOAT, -FLT_MAX, FLT_MAX, 0.0, NULL, 0, NULL},
+    {"region", PT_DARRAY, -FLT_MAX, FLT_MAX, 0, NULL, 3, NULL},
+    {"projection-offset", PT_DARRAY, -2147483648.0, 2147483647.0, 0, NULL, 2, NULL},
+    {"center", PT_DARRAY, -FLT_MAX, FLT_MAX, 0, NULL, 2, NULL},
+    {"num-projections", PT_UINT, 0, 32768, 0, NULL, 0, NULL},
+    {"overall-angle", PT_FLOAT, -FLT_MAX, FLT_MAX, M_PI, NULL, 0, NULL},
+    {"tomo-angle", PT_FLOAT, -FLT_MAX, FLT_MAX, -FLT_MAX, NULL, 0, NULL},
+    {"lamino-angle", PT_FLOAT, -FLT_MAX, FLT_MAX, 0.0, NULL, 0, NULL},
+    {"roll-angle", PT_FLOAT, -FLT_MAX, FLT_MAX, 0.0, NULL, 0, NULL},
+    {"parameter", PT_ENUM, 0, 3, 0, lamino_parameter_nicks, 0, NULL},
+    {"addressing-mode", PT_ENUM, 0, 4, 2, addr_nicks, 0, NULL},
+};
+enum { L_XREGION, L_YREGION, L_Z, L_REGION, L_OFFSET, L_CENTER, L_NPROJ, L_OVERALL, L_TOMO, L_LAMINO, L_ROLL,
+       L_PARAM, L_ADDR };
+
+typedef struct {
+    int width, height;
+    unsigned count, generated;
+    float *d_stack;       /* num_projections projections [H][W] */
+    ufomi_lamino_params params;
+    int nx, ny, nz;
+} LbpPriv;
+
+/* Element k of a value array, 0 past its end (no broadcasting). */
+static double elem_at(const PropValue *v, int k) { return v->n > k ? v->arr[k] : 0.0; }
+
+static void lbp_fill_params(const UfomiTask *t, ufomi_lamino_params *p)
+{
+    ufomi_lamino_params_init(p);
+    p->num_projections = (unsigned) t->v[L_NPROJ].num;
+    p->overall_angle = (float) t->v[L_OVERALL].num;
+    p->tomo_angle = (float) t->v[L_TOMO].num;
+    p->lamino_angle = (float) t->v[L_LAMINO].num;
+    p->roll_angle = (float) t->v[L_ROLL].num;
+    p->z = (float) t->v[L_Z].num;
+    for (int k = 0; k < 3; k++) {
+        p->x_region[k] = (int) elem_at(&t->v[L_XREGION], k);
+        p->y_region[k] = (int) elem_at(&t->v[L_YREGION], k);
+        p->region[k] = (float) elem_at(&t->v[L_REGION], k);
+    }
+    for (int k = 0; k < 2; k++) {
+        p->center[k] = (float) elem_at(&t->v[L_CENTER], k);
+        p->projection_offset[k] = (int) elem_at(&t->v[L_OFFSET], k);
+    }
+    p->parameter = (int) t->v[L_PARAM].num;
+    static const int addr_map[] = {UFOMI_ADDR_NONE, UFOMI_ADDR_CLAMP_TO_EDGE, UFOMI_ADDR_CLAMP, UFOMI_ADDR_REPEAT,
+                                   UFOMI_ADDR_MIRRORED_REPEAT};
+    p->addressing_mode = addr_map[(int) t->v[L_ADDR].num];
+}
+
+/* set_region (:124-136): a region whose "from" exceeds "to" is ignored with a
+ * warning; returns 0 then, 1 to store the value. */
+static int lamino_region_property(const char *name, const double *vals, int n)
+{
+    if (n >= 2 && (int) vals[0] > (int) vals[1]) {
+        fprintf(stderr, "warning: Invalid region [\"from\", \"to\", \"step\"]: [%d, %d, %d], \"from\" has to be "
+                        "less than or equal to \"to\" (%s)\n", (int) vals[0], (int) vals[1], n > 2 ? (int) vals[2] : 0,
+                name);
+        return 0;
+    }
+    return 1;
+}
+
+static int lbp_setup(UfomiTask *t, UfomiError *e)
+{
+    /* :174-182 */
+    if (!(unsigned) t->v[L_NPROJ].num)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_NUM_PROJECTIONS, "Number of projections has not been set");
+    if ((float) elem_at(&t->v[L_REGION], 2) == 0.0f)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_ARG, "Step in region is 0");
+    const int addr = (int) t->v[L_ADDR].num;
+    if (addr == 3 || addr == 4)
+        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
+                          "lamino-backproject: repeat addressing needs normalized coordinates");
+    return UFOMI_OK;
+}
+
+static int lbp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
+{
+    LbpPriv *p = t->priv;
+    const UfomiRequisition *ir = &in[0]->req;
+    if (!p->d_stack) {
+        lbp_fill_params(t, &p->params);
+        int rc = ufomi_lamino_volume_shape(&p->params, &p->nx, &p->ny, &p->nz);
+        if (rc)
+            return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, rc, "%s", ufomi_last_error());
+        p->width = (int) ir->dims[0];
+        p->height = ir->n_dims >= 2 ? (int) ir->dims[1] : 1;
+        const size_t proj_bytes = (size_t) p->params.num_projections * p->width * p->height * sizeof(float);
+        if ((rc = ufomi_device_alloc(t->ctx, proj_bytes, (void **) &p->d_stack)))
+            return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_MEMORY, "%s", ufomi_last_error());
+    } else if ((int) ir->dims[0] != p->width || (ir->n_dims >= 2 ? (int) ir->dims[1] : 1) != p->height) {
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_ARG,
+                          "lamino-backproject: projection size changed");
+    }
+    /* :247-262: one volume [nz][ny][nx] */
+    req->n_dims = 3;
+    req->dims[0] = (size_t) p->nx;
+    req->dims[1] = (size_t) p->ny;
+    req->dims[2] = (size_t) p->nz;
+    return UFOMI_OK;
+}
+
+static int lbp_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) out;
+    (void) req;
+    LbpPriv *p = t->priv;
+    if (p->count >= p->params.num_projections)
+        return process_fail(t, UFOMI_ERR_ARG, "lamino-backproject: more projections than num-projections");
+    const size_t plane = (size_t) p->width * p->height;
+    int rc = ufomi_memcpy(t->ctx, p->d_stack + p->count * plane, in[0]->data, plane * sizeof(float), 2);
+    if (rc)
+        return process_fail(t, rc, "lamino-backproject: %s", ufomi_last_error());
+    p->count++;
+    return 1;
+}
+
+static int lbp_generate(UfomiTask *t, UfomiBuffer *out, UfomiRequisition *req)
+{
+    (void) req;
+    LbpPriv *p = t->priv;
+    /* :480-496: the volume, once */
+    if (p->generated)
+        return 0;
+    /* process (:353, :430-475) runs its 16-projection kernel only on a full
+     * burst below num_projections / 16 * 16, the 1-projection kernel from
+     * there on: short of that, a trailing partial burst never reaches the
+     * volume. */
+    const unsigned full = p->params.num_projections / 16 * 16;
+    const unsigned used = p->count < full ? p->count / 16 * 16 : p->count;
+    if (!used) {
+        fprintf(stderr, "warning: lamino-backproject received %u projections, fewer than one burst of 16 out of "
+                        "%u: no volume\n", p->count, p->params.num_projections);
+        return 0;
+    }
+    int rc = ufomi_lamino_bp(t->ctx, p->d_stack, p->width, p->height, (int) used, &p->params,
+                             (float *) out->data);
+    if (rc)
+        return process_fail(t, rc, "lamino-backproject: %s", ufomi_last_error());
+    p->generated = 1;
+    out->layout = UFOMI_LAYOUT_REAL;
+    return 1;
+}
+
+static void lbp_finalize(UfomiTask *t)
+{
+    LbpPriv *p = t->priv;
+    if (p && t->ctx && p->d_stack)
+        ufomi_device_free(t->ctx, p->d_stack);
+}
+
+/* Output items and element size of a generic reductor stage. */
+int reductor_output_items(const UfomiTask *t, const UfomiRequisition *in, int *items, size_t *elem_size,
+                          int *dtype)
+{
+    static const size_t st_size[] = {2, 4, 8, 1, 2, 4};
+    *items = 1;
+    *elem_size = sizeof(float);
+    *dtype = UFOMI_F32;
+    if (!strcmp(t->cls->name, "general-backproject")) {
+        ufomi_gbp_params gp;
+        ufomi_gbp_params_init(&gp);
+        for (int k = 0; k < 3; k++)
+            gp.region[k] = elem_at(&t->v[G_REGION], k);
+        int nx, ny, nz = 0;
+        int rc = ufomi_gbp_volume_shape(&gp, (int) in->dims[0], &nx, &ny, &nz);
+        if (rc)
+            return rc;
+        const int st = (int) t->v[G_STORE].num;
+        *items = nz;
+        *elem_size = st_size[st];
+        *dtype = st == UFOMI_ST_FLOAT ? UFOMI_F32 : (st == UFOMI_ST_HALF ? UFOMI_F16 : -1 - st);
+    }
+    return UFOMI_OK;
+}
+
 /* --------------------------------------------------- transpose-projections */
 
 /* transpose-projections (ufo-transpose-projections-task.c:256-261) */
@@ -1724,6 +1914,8 @@ static const TaskClass classes[] = {
      avg_setup, avg_requisition, avg_process, avg_generate, avg_finalize},
     {"stacked-backproject", sbp_props, NPROPS(sbp_props), UFOMI_TASK_MODE_PROCESSOR | UFOMI_TASK_MODE_GPU,
      sizeof(SbpPriv), NULL, sbp_requisition, sbp_process, NULL, NULL},
+    {"lamino-backproject", lbp_props, NPROPS(lbp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
+     sizeof(LbpPriv), lbp_setup, lbp_requisition, lbp_process, lbp_generate, lbp_finalize},
     {"transpose-projections", tp_props, NPROPS(tp_props), UFOMI_TASK_MODE_REDUCTOR | UFOMI_TASK_MODE_GPU,
      sizeof(TpPriv), tp_setup, tp_requisition, tp_process, tp_generate, tp_finalize},
     {"read", read_props, NPROPS(read_props), UFOMI_TASK_MODE_GENERATOR | UFOMI_TASK_MODE_CPU, sizeof(ReadPriv),

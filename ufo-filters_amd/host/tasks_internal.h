@@ -59,6 +59,8 @@ int filter_task_coefficients(UfomiTask *t, size_t width, UfomiError *e);
 const float *filter_task_device_coefficients(const UfomiTask *t);
 void backproject_task_params(const UfomiTask *t, ufomi_bp_params *bp);
 int gbp_task_store_type(const UfomiTask *t);
+/* Output items (volume slices, or 1), element size and dtype of a generic reductor stage. */
+int reductor_output_items(const UfomiTask *t, const UfomiRequisition *in, int *items, size_t *elem_size, int *dtype);
 /* read: the next image into host memory (req from ufomi_task_get_requisition);
  * 1 = produced, 0 = no more images or an error (then t->err_status is set). */
 int read_task_generate_host(UfomiTask *t, float *dst, const UfomiRequisition *req);

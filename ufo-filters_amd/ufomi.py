@@ -33,7 +33,7 @@ EXPORTS = [
     "ufomi_version", "ufomi_filter_coefficients", "ufomi_fft_rows", "ufomi_filter_apply", "ufomi_ifft_rows",
     "ufomi_fft_nd", "ufomi_ifft_nd",
     "ufomi_ramp_filter", "ufomi_bp_params_init", "ufomi_bp_parallel",
-    "ufomi_cone_weight", "ufomi_transpose_projections", "ufomi_bp_stacked", "ufomi_flat_field_correct", "ufomi_average", "ufomi_gbp_sweep", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
+    "ufomi_cone_weight", "ufomi_transpose_projections", "ufomi_bp_stacked", "ufomi_flat_field_correct", "ufomi_average", "ufomi_gbp_sweep", "ufomi_lamino_params_init", "ufomi_lamino_volume_shape", "ufomi_lamino_bp", "ufomi_gbp_params_init", "ufomi_gbp_geometry_init", "ufomi_gbp_volume_shape",
     "ufomi_gbp", "ufomi_phantom_ellipses", "ufomi_phantom_ellipsoids", "ufomi_slice_ellipsoids",
     "ufomi_sinogram_ellipses", "ufomi_add_noise", "ufomi_bench_lds_read", "ufomi_bench_copy",
     "ufomi_gbp_projection_rows", "ufomi_project_ellipsoids", "ufomi_device_alloc", "ufomi_device_free",
@@ -88,6 +88,16 @@ class GbpParams(ctypes.Structure):
                 ("gray_map_min", ctypes.c_double), ("gray_map_max", ctypes.c_double)]
 
 
+class LaminoParams(ctypes.Structure):
+    _fields_ = [("num_projections", ctypes.c_uint), ("overall_angle", ctypes.c_double),
+                ("tomo_angle", ctypes.c_double), ("lamino_angle", ctypes.c_double), ("roll_angle", ctypes.c_double),
+                ("center", ctypes.c_double * 2), ("projection_offset", ctypes.c_int * 2),
+                ("x_region", ctypes.c_int * 3), ("y_region", ctypes.c_int * 3), ("region", ctypes.c_double * 3),
+                ("z", ctypes.c_double), ("parameter", ctypes.c_int), ("addressing_mode", ctypes.c_int)]
+
+
+LAMINO_PARAMETERS = ["z", "x-center", "lamino-angle", "roll-angle"]
+
 _lib = None
 
 
@@ -120,6 +130,10 @@ def load():
         "ufomi_gbp_volume_shape": [ctypes.POINTER(GbpParams), ci, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                    ctypes.POINTER(ci)],
         "ufomi_gbp": [vp, vp, ci, ci, ci, ctypes.POINTER(GbpGeometry), ctypes.POINTER(GbpParams), vp],
+        "ufomi_lamino_params_init": [ctypes.POINTER(LaminoParams)],
+        "ufomi_lamino_volume_shape": [ctypes.POINTER(LaminoParams), ctypes.POINTER(ci), ctypes.POINTER(ci),
+                                      ctypes.POINTER(ci)],
+        "ufomi_lamino_bp": [vp, vp, ci, ci, ci, ctypes.POINTER(LaminoParams), vp],
         "ufomi_gbp_sweep": [vp, vp, ci, ci, ci, ctypes.POINTER(GbpGeometry), ctypes.POINTER(GbpParams), ci, vp],
         "ufomi_phantom_ellipses": [ctypes.c_uint64, ci, cd, dp],
         "ufomi_phantom_ellipsoids": [ctypes.c_uint64, ci, cd, dp],
@@ -158,6 +172,7 @@ def load():
     L.ufomi_version.restype = ctypes.c_char_p
     L.ufomi_bp_params_init.restype = None
     L.ufomi_gbp_params_init.restype = None
+    L.ufomi_lamino_params_init.restype = None
     L.ufomi_gbp_geometry_init.restype = None
     _lib = L
     return L
@@ -245,6 +260,30 @@ def gbp_params(num_projections, overall_angle=2 * math.pi, region=(0, 0, 0), x_r
     p.gray_map_min = gray_map_min
     p.gray_map_max = gray_map_max
     return p
+
+
+def lamino_params(num_projections, **kw):
+    """LaminoParams from keyword arguments named like the task's properties ('-' -> '_')."""
+    p = LaminoParams()
+    load().ufomi_lamino_params_init(ctypes.byref(p))
+    p.num_projections = num_projections
+    for k, v in kw.items():
+        if k == "parameter":
+            v = LAMINO_PARAMETERS.index(v) if isinstance(v, str) else v
+        elif k == "addressing_mode":
+            v = ADDRESSING[v] if isinstance(v, str) else v
+        if k in ("center", "projection_offset", "x_region", "y_region", "region"):
+            getattr(p, k)[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def lamino_shape(params):
+    nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(load().ufomi_lamino_volume_shape(ctypes.byref(params), ctypes.byref(nx), ctypes.byref(ny),
+                                           ctypes.byref(nz)))
+    return nz.value, ny.value, nx.value
 
 
 def gbp_geometry(**arrays):
@@ -362,6 +401,10 @@ class Context:
     def gbp(self, d_proj, W, H, P, geometry, params, d_vol):
         check(self.L.ufomi_gbp(self.h, _ptr(d_proj), W, H, P, ctypes.byref(geometry), ctypes.byref(params),
                                _ptr(d_vol)))
+
+    def lamino_bp(self, d_proj, W, H, P, params, d_vol):
+        """lamino-backproject: projections [P][H][W] -> volume [nz][ny][nx]."""
+        check(self.L.ufomi_lamino_bp(self.h, _ptr(d_proj), W, H, P, ctypes.byref(params), _ptr(d_vol)))
 
     def gbp_sweep(self, d_proj, W, H, P, geometry, params, parameter, d_vol):
         """general-backproject with `parameter` (index of GBP_PARAMETERS) swept over params.region."""
