@@ -254,6 +254,9 @@ extern "C" int ufomi_fbp_batch_create(ufomi_ctx *ctx, const ufomi_fbp_request *r
         if ((double) roi_w + prm.roi_x > lim || (double) roi_h + prm.roi_y > lim || W > (1 << 20) ||
             fabs((double) axis) > lim)
             return bad(UFOMI_ERR_ARG, "region of interest or axis beyond 2^20 pixels", i, 0);
+        // the staging loads address one sinogram with 32-bit byte offsets
+        if ((double) P * W * 4.0 >= 4294967296.0)
+            return bad(UFOMI_ERR_ARG, "sinogram exceeds 4 GiB", i, 0);
 
         ufomi_bp_batch_req &r = b->reqs[i];
         r.sino = nullptr;  // set once the filtered buffer exists

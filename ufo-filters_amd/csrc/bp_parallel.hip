@@ -73,6 +73,16 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return b;
 }
 
+// Global-memory view of a type (loads with a uniform base pointer then take
+// the SGPR-base + 32-bit VGPR-offset form: no 64-bit address per load).
+template <typename T> using Glb = __attribute__((address_space(1))) T;
+
+// Raw stored value at byte offset b of a slice's filtered sinogram.
+template <typename R, typename Ptr> __device__ __forceinline__ R load_at(Ptr p, unsigned b)
+{
+    return *(const Glb<R> *) ((const Glb<char> *) p + b);
+}
+
 // Raw stored values of a filtered sinogram and their conversion to float.
 template <typename T> struct Stage;
 // Loads go through the caller's pointer type, which may carry the global
@@ -212,16 +222,20 @@ __device__ __forceinline__ void bp_tile(
             in0[m] = p < P && j >= 0 && j < W;
             in1[m] = p < P && j + 1 >= 0 && j + 1 < W;
             const int j0 = min(max(j, 0), W - 1), j1 = min(max(j + 1, 0), W - 1);
-            const long roff = (long) (p < P ? p : 0) * W;
+            // byte offsets in a slice's sinogram (< 4 GiB, checked on the host),
+            // shared by the S slices, whose base pointers are uniform
+            const unsigned rb = (unsigned) (p < P ? p : 0) * (unsigned) W;
+            const unsigned b0 = (rb + (unsigned) j0) * (unsigned) sizeof(R);
+            const unsigned b1 = (rb + (unsigned) j1) * (unsigned) sizeof(R);
 #pragma unroll
-            for (int s = 0; s < S; s++) r0[m][s] = Stage<InT>::load(src[s] + roff, j0);
+            for (int s = 0; s < S; s++) r0[m][s] = load_at<R>(src[s], b0);
             if constexpr (!kDpp) {
 #pragma unroll
-                for (int s = 0; s < S; s++) r1[m][s] = Stage<InT>::load(src[s] + roff, j1);
+                for (int s = 0; s < S; s++) r1[m][s] = load_at<R>(src[s], b1);
             } else if constexpr (kSeam) {
                 if (wave_last) {
 #pragma unroll
-                    for (int s = 0; s < S; s++) r1[m][s] = Stage<InT>::load(src[s] + roff, j1);
+                    for (int s = 0; s < S; s++) r1[m][s] = load_at<R>(src[s], b1);
                 }
             }
         }
@@ -391,7 +405,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         if (threadIdx.x == 0)
             next = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = next;
+        // uniform across the workgroup: the request's fields load into SGPRs
+        const int it = __builtin_amdgcn_readfirstlane(next);
         if (it >= n_items)
             break;
         const int2 w = items[it];
@@ -548,6 +563,9 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
         return ufomi_fail(UFOMI_ERR_ARG, "region of interest or axis beyond 2^20 pixels");
     if (!sino_pitch) sino_pitch = (size_t) P_in * W;
     if (!out_pitch) out_pitch = (size_t) roi_w * roi_h;
+    // the staging loads address one slice's sinogram with 32-bit byte offsets
+    if ((double) P_in * W * 4.0 >= 4294967296.0)
+        return ufomi_fail(UFOMI_ERR_ARG, "a sinogram of %d x %d exceeds 4 GiB", P_in, W);
 
     float2 *trig = nullptr;
     if ((rc = ufomi_bp_trig(ctx, prm->angle_offset, step, prm->offset, P_in, sin_sign, &trig))) return rc;

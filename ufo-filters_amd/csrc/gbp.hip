@@ -219,6 +219,9 @@ __device__ __forceinline__ void store_voxel(ST *__restrict__ vol, long o, float 
 // takes the global-memory path for that projection.
 // ---------------------------------------------------------------------------
 constexpr int kBoxW = 64, kBoxH = 48;  // texels per staged footprint
+// Row pitch of a staged box: 74 floats, 10 banks (of 64) apart, so the rows a
+// wave reads at once (each a window of ~9 banks) do not share banks.
+constexpr int kBoxS = kBoxW + 10;
 constexpr int kChunk = 32;             // projections per footprint chunk
 constexpr int kRowsPerThread = kBoxH / 4;
 
@@ -226,11 +229,40 @@ struct BoxGeom {
     int i_lo, j_lo, w, h;  // w = 0: global path
 };
 
+// A box read back from LDS is the same for every lane: say so, so that the
+// staging's row tests and row offsets are scalar.
+__device__ __forceinline__ BoxGeom uniform(BoxGeom g)
+{
+    g.i_lo = __builtin_amdgcn_readfirstlane(g.i_lo);
+    g.j_lo = __builtin_amdgcn_readfirstlane(g.j_lo);
+    g.w = __builtin_amdgcn_readfirstlane(g.w);
+    g.h = __builtin_amdgcn_readfirstlane(g.h);
+    return g;
+}
+
+template <typename T> using Glb = __attribute__((address_space(1))) T;
+
 template <int MODE> struct TileShape {
     static constexpr bool CONE = MODE == MODE_CONE_ZSEP || MODE == MODE_CONE;
     static constexpr int TX = CONE ? 16 : 32, TY = TX, TZ = CONE ? 16 : 8;
     static constexpr int KA = TX / 16, KB = TY / 16;
 };
+
+// Texel (row, col) of a staged box.  Rows and columns inside a box are small
+// and non-negative, so a full-rate 24-bit multiply-add forms the index (a plain
+// int multiply-add becomes a quarter-rate 64-bit one).
+__device__ __forceinline__ const float *box_texel(const float *bx, int row, int col)
+{
+    return bx + __umul24((unsigned) row, (unsigned) kBoxS) + (unsigned) col;
+}
+
+// Keeps a wave-uniform value in a VGPR: a VALU operation takes one SGPR
+// operand, so a product of two uniform values would need a move per use.
+__device__ __forceinline__ float in_vgpr(float v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
 
 // Projective (or affine) map of one projection at X = (x, y, z).
 template <bool CONE>
@@ -258,13 +290,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     constexpr int TX = TS::TX, TY = TS::TY, TZ = TS::TZ, KA = TS::KA, KB = TS::KB;
     // U (and for cone D, C) independent of z: checked on the host for the ZSEP modes
     constexpr bool ZSEP = MODE == MODE_CONE_ZSEP || MODE == MODE_PARALLEL_ZSEP;
-    __shared__ float box[2][kBoxH * kBoxW];
+    __shared__ float box[2][kBoxH * kBoxS];
     __shared__ BoxGeom bgs[2][kChunk];
 
     const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
     const int X0 = tx * TX, Y0 = ty * TY, Z0 = tz * TZ;
-    const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
+    // Wave w covers the 8 x 8 voxel block (w & 1, w >> 1) of the 16 x 16
+    // thread grid: its reads spread over ~4 rows of ~8 columns at any angle,
+    // which the box pitch puts in distinct banks.
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lx = ((wv & 1) << 3) | (lane & 7), ly = ((wv >> 1) << 3) | (lane >> 3);
 
     float xv[KA], yv[KB], z[TZ];
 #pragma unroll
@@ -311,11 +347,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     };
 
     // Staging registers: texel column tid % 64, rows tid / 64 + 4 k.
-    const int sc = tid & 63, sr0 = tid >> 6;
+    // (the wave index, hence every staged row of a wave, is uniform)
+    const int sc = tid & 63, sr0 = __builtin_amdgcn_readfirstlane(tid >> 6);
     float st[kRowsPerThread];
     const long plane = (long) W * H;
     auto stage_load = [&](int p, const BoxGeom &g) {
-        const float *img = proj + (long) p * plane;
+        const Glb<float> *img = (const Glb<float> *) (proj + (long) p * plane);
         int i = g.i_lo + sc;
         bool ci = sc < g.w;
         if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
@@ -323,24 +360,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
         } else {
             i = min(max(i, 0), W - 1);
         }
+        const unsigned ib = (unsigned) i * 4u;
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; k++) {
             const int r = sr0 + 4 * k;
             int j = g.j_lo + r;
-            bool ok = ci && r < g.h;
+            bool rok = r < g.h;
             if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
-                ok = ok && j >= 0 && j < H;
+                rok = rok && j >= 0 && j < H;
             } else {
                 j = min(max(j, 0), H - 1);
             }
-            st[k] = ok ? img[(long) j * W + i] : 0.0f;
+            // byte offsets inside one projection (< 4 GiB, checked on the host):
+            // a scalar row offset plus the lane's column
+            const unsigned rowb = (unsigned) j * (unsigned) W * 4u;
+            st[k] = (rok && ci) ? *(const Glb<float> *) ((const Glb<char> *) img + (rowb + ib)) : 0.0f;
         }
     };
     auto commit = [&](int buf, const BoxGeom &g) {
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; k++) {
             const int r = sr0 + 4 * k;
-            if (r < g.h) box[buf][r * kBoxW + sc] = st[k];
+            if (r < g.h) box[buf][r * kBoxS + sc] = st[k];
         }
     };
 
@@ -356,7 +397,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     footprints(0, 0);
     __syncthreads();
     {
-        const BoxGeom g0 = bgs[0][0];
+        const BoxGeom g0 = uniform(bgs[0][0]);
         if (g0.w > 0) {
             stage_load(0, g0);
             commit(0, g0);
@@ -366,12 +407,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     for (int p = 0; p < P; p++) {
         if (p % kChunk == 0 && p + kChunk < P + kChunk - 1)
             footprints(p + kChunk, ((p / kChunk) + 1) & 1);  // read a chunk later
-        const BoxGeom g = bgs[(p / kChunk) & 1][p % kChunk];
+        const BoxGeom g = uniform(bgs[(p / kChunk) & 1][p % kChunk]);
         const bool next = p + 1 < P;
         BoxGeom gn;
         gn.w = 0;
         if (next) {
-            gn = bgs[((p + 1) / kChunk) & 1][(p + 1) % kChunk];
+            gn = uniform(bgs[((p + 1) / kChunk) & 1][(p + 1) % kChunk]);
             if (gn.w > 0) stage_load(p + 1, gn);
         }
         const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1], md = mats[4 * p + 2], mc = mats[4 * p + 3];
@@ -395,14 +436,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
                         const float uu = fmaf(ub, rd, -ilo);
                         const int ci = (int) uu;
                         const float fa = __builtin_amdgcn_fractf(uu);
-                        const float dv = mv.z * rd, v0 = fmaf(vb, rd, -jlo);
+                        // (z[k] is uniform: keep the other factor in a VGPR)
+                        const float dv = CONE ? mv.z * rd : in_vgpr(mv.z), v0 = fmaf(vb, rd, -jlo);
 #pragma unroll
                         for (int k = 0; k < TZ; k++) {
                             const float vv = fmaf(dv, z[k], v0);
-                            const float *t = bx + (int) vv * kBoxW + ci;
+                            const float *t = box_texel(bx, (int) vv, ci);
                             const float fb = __builtin_amdgcn_fractf(vv);
                             const float top = fmaf(fa, t[1] - t[0], t[0]);
-                            const float bot = fmaf(fa, t[kBoxW + 1] - t[kBoxW], t[kBoxW]);
+                            const float bot = fmaf(fa, t[kBoxS + 1] - t[kBoxS], t[kBoxS]);
                             const float val = fmaf(fb, bot - top, top);
                             if constexpr (CONE)
                                 acc[a][b][k] = fmaf(val, w, acc[a][b][k]);
@@ -412,18 +454,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
                     } else {
                         const float db = fmaf(md.x, x, fmaf(md.y, y, md.w));
                         const float cb = fmaf(mc.x, x, fmaf(mc.y, y, mc.w));
+                        const float muz = in_vgpr(mu.z), mvz = in_vgpr(mv.z);
+                        const float mdz = CONE ? in_vgpr(md.z) : 0.0f, mcz = CONE ? in_vgpr(mc.z) : 0.0f;
 #pragma unroll
                         for (int k = 0; k < TZ; k++) {
-                            const float rd = __builtin_amdgcn_rcpf(fmaf(md.z, z[k], db));
-                            const float cf = __builtin_amdgcn_rcpf(fmaf(mc.z, z[k], cb));
-                            const float uu = fmaf(fmaf(mu.z, z[k], ub), rd, -ilo);
-                            const float vv = fmaf(fmaf(mv.z, z[k], vb), rd, -jlo);
-                            const float *t = bx + (int) vv * kBoxW + (int) uu;
+                            float rd = 1.0f, w = 1.0f;
+                            if constexpr (CONE) {
+                                rd = __builtin_amdgcn_rcpf(fmaf(mdz, z[k], db));
+                                const float cf = __builtin_amdgcn_rcpf(fmaf(mcz, z[k], cb));
+                                w = cf * cf;
+                            }
+                            const float uu = fmaf(fmaf(muz, z[k], ub), rd, -ilo);
+                            const float vv = fmaf(fmaf(mvz, z[k], vb), rd, -jlo);
+                            const float *t = box_texel(bx, (int) vv, (int) uu);
                             const float fa = __builtin_amdgcn_fractf(uu), fb = __builtin_amdgcn_fractf(vv);
                             const float top = fmaf(fa, t[1] - t[0], t[0]);
-                            const float bot = fmaf(fa, t[kBoxW + 1] - t[kBoxW], t[kBoxW]);
+                            const float bot = fmaf(fa, t[kBoxS + 1] - t[kBoxS], t[kBoxS]);
                             const float val = fmaf(fb, bot - top, top);
-                            acc[a][b][k] = fmaf(val, cf * cf, acc[a][b][k]);
+                            if constexpr (CONE)
+                                acc[a][b][k] = fmaf(val, w, acc[a][b][k]);
+                            else
+                                acc[a][b][k] += val;
                         }
                     }
                 }
@@ -545,6 +596,9 @@ int gbp_launch(ufomi_ctx *ctx, const float *d_proj, int W, int H, unsigned n, co
     const long n_tiles = (long) tiles_x * tiles_y * tiles_z;
     if (n_tiles > 0x7fffffffL)
         return ufomi_fail(UFOMI_ERR_ARG, "volume too large for one launch");
+    // the staging loads index a projection with 24-bit factors and a 32-bit offset
+    if (W >= (1 << 24) || H >= (1 << 24) || (long) W * H * 4 > 0xffffffffL)
+        return ufomi_fail(UFOMI_ERR_ARG, "projections of %d x %d texels are too large", W, H);
     dim3 grid((unsigned) n_tiles);
     auto go = [&](auto mode_c, auto addr_c, auto *vol) {
         constexpr int C = decltype(mode_c)::value;
