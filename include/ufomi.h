@@ -124,8 +124,9 @@ int ufomi_ifft_rows(ufomi_ctx *ctx, const float *d_in, int N, int P, int crop, f
  * ufomi_ifft_nd: interleaved complex [m2][m1][m0] in, inverse DFT over the
  * transformed axes normalised by their sizes, real part cropped to
  * crop_width x crop_height (0 = no crop) out, [m2][crop_height][crop_width].
- * Powers of two up to 8192 and other sizes up to 4096 (chirp-z) are built;
- * either runs one LDS pass per transformed axis.
+ * Powers of two up to 8192 and other sizes up to 4096 (chirp-z) are built on
+ * every axis, one LDS pass per transformed axis; 1-D transforms of power-of-
+ * two rows up to 2^20 points run as two passes (four-step split).
  */
 int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, const int n[3], int dimensions, const int m[3],
                  float *d_out);

@@ -166,3 +166,51 @@ def test_complex_input_and_unbuilt_sizes(ctx, torch_dev):
     assert rel_l2(out.cpu().numpy().view(np.complex64), ref) < 1e-5
     with pytest.raises(RuntimeError, match="not built"):
         _run(ctx, x[None], _task("fft", dimensions=1, size_x=5000))
+
+
+@pytest.mark.parametrize("width,size_x", [(10000, 16384), (16384, 16384), (20000, 32768), (9, 32768)])
+def test_long_rows_four_step(ctx, width, size_x):
+    """Power-of-two rows above 8192 points (two passes of the four-step split): forward against
+    numpy.fft in float64, round trip with a crop back to the input."""
+    rows = 3
+    x = np.random.default_rng(width).standard_normal((rows, width)).astype(np.float32)
+    spec = _run(ctx, [x], _task("fft", dimensions=1, size_x=size_x))[0]
+    got = spec.reshape(rows, size_x, 2)
+    got = got[..., 0] + 1j * got[..., 1]
+    ref = np.fft.fft(x.astype(np.float64), n=size_x, axis=-1)
+    assert rel_l2(got, ref) < 1e-5
+    back = _run(ctx, [x], _task("fft", dimensions=1, size_x=size_x), _task("ifft", dimensions=1,
+                                                                           crop_width=width))[0]
+    assert back.shape == (rows, width)
+    assert rel_l2(back, x) < 1e-5
+
+
+def test_long_rows_complex_input_and_filter_chain(ctx, torch_dev):
+    """Complex rows of 16384 points through the C ABI, and fft ! filter ! ifft at N = 16384 (not fused)
+    against the oracle's FP64 ramp chain."""
+    import torch
+    rows, w, n = 4, 9000, 16384
+    z = np.random.default_rng(5).standard_normal((rows, 2 * w)).astype(np.float32)
+    out = torch.empty((rows, 2 * n), device=torch_dev)
+    ctx.fft_nd(torch.from_numpy(z).to(torch_dev), (w, rows, 1), 1, (n, rows, 1), out, in_complex=True)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(rows, n, 2)
+    zc = z[:, 0::2].astype(np.float64) + 1j * z[:, 1::2]
+    assert rel_l2(got[..., 0] + 1j * got[..., 1], np.fft.fft(zc, n=n, axis=-1)) < 1e-5
+    # 65536 points (beyond the task's size-x range, through the C ABI), forward and cropped inverse
+    big = 65536
+    xr = np.random.default_rng(7).standard_normal((2, 40000)).astype(np.float32)
+    spec = torch.empty((2, 2 * big), device=torch_dev)
+    ctx.fft_nd(torch.from_numpy(xr).to(torch_dev), (40000, 2, 1), 1, (big, 2, 1), spec)
+    back = torch.empty((2, 40000), device=torch_dev)
+    ctx.ifft_nd(spec, (big, 2, 1), 1, back, crop_width=40000)
+    torch.cuda.synchronize()
+    sp = spec.cpu().numpy().reshape(2, big, 2)
+    assert rel_l2(sp[..., 0] + 1j * sp[..., 1], np.fft.fft(xr.astype(np.float64), n=big, axis=-1)) < 1e-5
+    assert rel_l2(back.cpu().numpy(), xr) < 1e-5
+    sino = np.random.default_rng(6).standard_normal((6, w)).astype(np.float32)
+    out = _run(ctx, [sino], _task("fft", dimensions=1, size_x=n), _task("filter"),
+               _task("ifft", dimensions=1, crop_width=w))[0]
+    coef = oracle.filter_coefficients(2 * n, "ramp-fromreal")
+    ref = oracle.ramp_chain(sino, n, coef, w)
+    assert rel_l2(out, ref) < 1e-5

@@ -253,8 +253,8 @@ extern "C" int ufomi_filter_coefficients(int type, unsigned width, float scale, 
         return ufomi_fail(UFOMI_ERR_ARG, "null coefficient buffer");
     if (width < 4 || !ufomi_is_pow2(width))
         return ufomi_fail(UFOMI_ERR_NOT_POW2, "Invalid filter input width %u, must be a power of two", width);
-    if (width / 2 > UFOMI_FFT_MAX)
-        return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "filter width %u exceeds 2 * %d", width, UFOMI_FFT_MAX);
+    if (width / 2 > (1u << 20))
+        return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "filter width %u exceeds 2 * 2^20", width);
 
     memset(c, 0, width * sizeof(float));
     c[0] = c[1] = (float) (0.5 / width);

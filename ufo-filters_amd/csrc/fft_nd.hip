@@ -45,6 +45,9 @@ struct Pass {
     int L, log2L;              // transform length in LDS
     int G, Ls;                 // lines per block, LDS line stride
     float scale;
+    long flat_in;              // > 0: element i of line a exists iff a in_a + i in_e < flat_in (not i < n_in)
+    long flat_out;             // > 0: element i of line a is stored iff a out_a + i out_e < flat_out
+    int tw_n;                  // > 0: output i of line a times exp(DIR 2 pi i (a i mod tw_n) / tw_n)
 };
 
 __device__ __forceinline__ float2 load_as_complex(const float *p, long o) { return make_float2(p[o], 0.0f); }
@@ -76,7 +79,8 @@ __global__ __launch_bounds__(kThreads) void axis_dft_kernel(const InT *__restric
         }
         const int a = a0 + gl;
         float2 v = make_float2(0.0f, 0.0f);
-        if (a < g.A && i < g.n_in) {
+        const bool inside = g.flat_in ? (long) a * g.in_a + (long) i * g.in_e < g.flat_in : i < g.n_in;
+        if (a < g.A && inside) {
             v = load_as_complex(in, (long) a * g.in_a + (long) b * g.in_b + (long) i * g.in_e);
             if (CHIRP)
                 v = cmul(v, chirp[i]);
@@ -106,11 +110,18 @@ __global__ __launch_bounds__(kThreads) void axis_dft_kernel(const InT *__restric
             i = idx - gl * g.n_out;
         }
         const int a = a0 + gl;
-        if (a >= g.A)
+        if (a >= g.A || (g.flat_out && (long) a * g.out_a + (long) i * g.out_e >= g.flat_out))
             continue;
         float2 y = lds[gl * g.Ls + i];
         if (CHIRP)
             y = cmul(y, chirp[i]);
+        if (g.tw_n) {
+            // (a i mod tw_n) / tw_n is exact in float for a power-of-two tw_n
+            const long q = ((long) a * i) & (g.tw_n - 1);
+            float sn, cs;
+            sincospif(2.0f * (float) q / (float) g.tw_n, &sn, &cs);
+            y = cmul(y, make_float2(cs, DIR * sn));
+        }
         const long o = (long) a * g.out_a + (long) b * g.out_b + (long) i * g.out_e;
         if (OUT_REAL)
             ((float *) out_)[o] = y.x * g.scale;
@@ -208,6 +219,8 @@ int make_pass(int axis, const int S[3], const int D[3], const int E[3], int n_in
     g.n_in = n_in;
     g.n_out = n_out;
     g.scale = 1.0f;
+    g.flat_in = g.flat_out = 0;
+    g.tw_n = 0;
     const long s1 = S[0], s2 = (long) S[0] * S[1], d1 = D[0], d2 = (long) D[0] * D[1];
     if (axis == 0) {
         g.A = E[1];
@@ -293,6 +306,69 @@ int setup_scratch(ufomi_ctx *ctx, PassSpec *ps, int np, int dir, size_t tmp_elem
     return UFOMI_OK;
 }
 
+// Power-of-two line lengths above kMaxL along axis 0 (1-D transforms):
+// N = N1 N2 with n = N2 n1 + n2 and k = k1 + N1 k2 (the "four-step" split),
+//   pass 1: for every n2, an N1-point transform over n1 (stride N2), times
+//           exp(dir 2 pi i n2 k1 / N), into [k1][n2];
+//   pass 2: for every k1, an N2-point transform over n2, written to k1 + N1 k2.
+// Row r of the input holds `valid` elements at pitch `in_pitch`; row r of the
+// output holds the first `keep` outputs at pitch `out_pitch`.
+constexpr long kMaxSplit = 1L << 20;
+
+void split_pass(Pass &g, int L)
+{
+    g.L = L;
+    g.log2L = ufomi_log2(L);
+    g.scale = 1.0f;
+    g.flat_in = g.flat_out = 0;
+    g.tw_n = 0;
+}
+
+int bb_of(int L) { return L >= 8192 ? 8 : std::min(4, L / 16); }
+
+template <int DIR, typename InT, bool OUT_REAL>
+int split_transform(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int N, long keep,
+                    long out_pitch, float scale, void *out)
+{
+    const int lg = ufomi_log2(N);
+    const int N1 = 1 << (lg / 2), N2 = N / N1;
+    if (rows <= 0)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid transform rows");
+    PassSpec p1{}, p2{};
+    Pass &a = p1.g, &b = p2.g;
+    split_pass(a, N1);
+    p1.BB = bb_of(N1);
+    a.G = 1024 * p1.BB / N1;
+    a.Ls = a.G > 1 ? N1 + 1 : N1;
+    a.A = N2, a.Bn = rows;
+    a.in_a = 1, a.in_e = N2, a.in_b = in_pitch;
+    a.n_in = N1, a.flat_in = valid;
+    a.out_a = 1, a.out_e = N2, a.out_b = N;
+    a.n_out = N1;
+    a.tw_n = N;
+    split_pass(b, N2);
+    p2.BB = bb_of(N2);
+    b.G = 1024 * p2.BB / N2;
+    b.Ls = b.G > 1 ? N2 + 1 : N2;
+    b.A = N1, b.Bn = rows;
+    b.in_a = N2, b.in_e = 1, b.in_b = N;
+    b.n_in = N2;
+    b.out_a = 1, b.out_e = N1, b.out_b = out_pitch;
+    b.n_out = N2, b.flat_out = keep;
+    b.scale = scale;
+    for (const Pass *g : {&a, &b}) {
+        const long blocks = (long) g->Bn * ((g->A + g->G - 1) / g->G);
+        if (blocks <= 0 || blocks >= (1L << 31))
+            return ufomi_fail(UFOMI_ERR_ARG, "transform of %d rows does not fit one launch", rows);
+    }
+    void *tmp = nullptr;
+    int rc = ufomi_scratch(ctx, (size_t) rows * N * sizeof(float2), &tmp);
+    if (rc) return rc;
+    if ((rc = launch_bb<DIR, false, InT, false>(ctx->stream, p1, in, tmp, nullptr, nullptr, ctx->twiddles)))
+        return rc;
+    return launch_bb<DIR, false, float2, OUT_REAL>(ctx->stream, p2, tmp, out, nullptr, nullptr, ctx->twiddles);
+}
+
 }  // namespace
 
 extern "C" int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, const int *n, int dimensions,
@@ -310,6 +386,14 @@ extern "C" int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, c
     // the radix-16 row kernels for plain 1-D power-of-two transforms of real rows
     if (dimensions == 1 && !in_complex && pow2(m[0]) && m[0] >= 16 && m[0] <= kMaxL && m[1] == n[1] && m[2] == n[2])
         return ufomi_fft_rows(ctx, d_in, n[0], n[1] * n[2], m[0], d_out);
+    // long power-of-two rows: two passes of the four-step split
+    if (dimensions == 1 && pow2(m[0]) && m[0] > kMaxL && m[0] <= kMaxSplit && m[1] == n[1] && m[2] == n[2]) {
+        const int rows = n[1] * n[2];
+        return in_complex ? split_transform<-1, float2, false>(ctx, d_in, n[0], n[0], rows, m[0], m[0], m[0], 1.0f,
+                                                               d_out)
+                          : split_transform<-1, float, false>(ctx, d_in, n[0], n[0], rows, m[0], m[0], m[0], 1.0f,
+                                                              d_out);
+    }
 
     // passes from axis 0 up; the last one writes the output extents on every axis
     PassSpec ps[3];
@@ -359,6 +443,9 @@ extern "C" int ufomi_ifft_nd(ufomi_ctx *ctx, const float *d_in, const int *m, in
     const int cw = crop_width ? crop_width : m[0], ch = crop_height ? crop_height : m[1];
     if (dimensions == 1 && pow2(m[0]) && m[0] >= 16 && m[0] <= kMaxL && (ch == m[1] || m[2] == 1))
         return ufomi_ifft_rows(ctx, d_in, m[0], ch * m[2], cw, d_out);
+    if (dimensions == 1 && pow2(m[0]) && m[0] > kMaxL && m[0] <= kMaxSplit && (ch == m[1] || m[2] == 1))
+        return split_transform<+1, float2, true>(ctx, d_in, m[0], m[0], ch * m[2], m[0], cw, cw,
+                                                 (float) (1.0 / m[0]), d_out);
 
     // passes from the highest transformed axis down to axis 0, which writes the real, cropped rows
     PassSpec ps[3];

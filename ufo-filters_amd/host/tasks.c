@@ -481,8 +481,12 @@ static int ifft_process(UfomiTask *t, UfomiBuffer **in, UfomiBuffer *out, UfomiR
 int fft_chain_fusable(const UfomiTask *fft, const UfomiTask *ifft, const UfomiBuffer *in)
 {
     const UfomiRequisition *r = &in->req;
-    const double sy = fft->v[FFT_SY].num, sz = fft->v[FFT_SZ].num;
-    return in->layout == UFOMI_LAYOUT_REAL && in->dtype == UFOMI_F32 && (int) fft->v[FFT_DIMS].num == 1 &&
+    const double sy = fft->v[FFT_SY].num, sz = fft->v[FFT_SZ].num, sx = fft->v[FFT_SX].num;
+    /* the fused kernel covers power-of-two transforms of 4..8192 points (fft_requisition's size) */
+    const size_t w = r->dims[0];
+    const size_t n = sx != 0 ? (size_t) sx : (fft->v[FFT_ZEROPAD].num != 0 && w > 1 ? 2 * closest_smaller_pow2(w - 1) : w);
+    return n >= 4 && n <= 8192 && is_pow2(n) && in->layout == UFOMI_LAYOUT_REAL && in->dtype == UFOMI_F32 &&
+           (int) fft->v[FFT_DIMS].num == 1 &&
            (int) ifft->v[IFFT_DIMS].num == 1 && (sy == 0 || r->n_dims < 2 || sy == (double) r->dims[1]) &&
            (sz == 0 || r->n_dims < 3 || sz == (double) r->dims[2]) &&
            (ifft->v[IFFT_CH].num == 0 || r->n_dims < 2 || ifft->v[IFFT_CH].num == (double) r->dims[1]);
