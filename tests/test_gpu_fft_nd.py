@@ -165,7 +165,7 @@ def test_complex_input_and_unbuilt_sizes(ctx, torch_dev):
     torch.cuda.synchronize()
     assert rel_l2(out.cpu().numpy().view(np.complex64), ref) < 1e-5
     with pytest.raises(RuntimeError, match="not built"):
-        _run(ctx, x[None], _task("fft", dimensions=1, size_x=5000))
+        _run(ctx, x[None], _task("fft", dimensions=2, size_x=5000))
 
 
 @pytest.mark.parametrize("width,size_x", [(10000, 16384), (16384, 16384), (20000, 32768), (9, 32768)])
@@ -214,3 +214,18 @@ def test_long_rows_complex_input_and_filter_chain(ctx, torch_dev):
     coef = oracle.filter_coefficients(2 * n, "ramp-fromreal")
     ref = oracle.ramp_chain(sino, n, coef, w)
     assert rel_l2(out, ref) < 1e-5
+
+
+@pytest.mark.parametrize("width,size_x", [(4500, 5000), (7000, 12345), (3, 20001)])
+def test_long_rows_chirp(ctx, width, size_x):
+    """Rows of other sizes above 4096 points: Bluestein over the four-step split (forward vs numpy in
+    float64; round trip with a crop back to the input)."""
+    rows = 3
+    x = np.random.default_rng(size_x).standard_normal((rows, width)).astype(np.float32)
+    spec = _run(ctx, [x], _task("fft", dimensions=1, size_x=size_x))[0]
+    got = spec.reshape(rows, size_x, 2)
+    ref = np.fft.fft(x.astype(np.float64), n=size_x, axis=-1)
+    assert rel_l2(got[..., 0] + 1j * got[..., 1], ref) < 1e-5
+    back = _run(ctx, [x], _task("fft", dimensions=1, size_x=size_x), _task("ifft", dimensions=1,
+                                                                           crop_width=width))[0]
+    assert rel_l2(back, x) < 1e-5

@@ -65,11 +65,11 @@ def test_round_trip_crop_like_reference(ctx):
 
 
 def test_not_built_paths_raise(ctx):
-    """Transform sizes past the built range (chirp-z above 4096; powers of two above 8192 except along
-    rows of 1-D transforms) fail loudly."""
+    """Transform sizes past the built range (chirp-z above 4096, powers of two above 8192, except along
+    the rows of 1-D transforms) fail loudly."""
     x = np.ones((1, 4, 9), np.float32)
     with pytest.raises(RuntimeError, match="not built"):
-        run_chain(ctx, x, make("fft", size_x=4097))
+        run_chain(ctx, x, make("fft", dimensions=2, size_x=4097, size_y=4))
     with pytest.raises(RuntimeError, match="not built"):
         run_chain(ctx, x, make("fft", dimensions=2, size_x=16, size_y=16384))
     with pytest.raises(RuntimeError, match="not built"):

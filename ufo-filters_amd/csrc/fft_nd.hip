@@ -327,8 +327,8 @@ void split_pass(Pass &g, int L)
 int bb_of(int L) { return L >= 8192 ? 8 : std::min(4, L / 16); }
 
 template <int DIR, typename InT, bool OUT_REAL>
-int split_transform(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int N, long keep,
-                    long out_pitch, float scale, void *out)
+int split_passes(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int N, long keep,
+                 long out_pitch, float scale, float2 *tmp, void *out)
 {
     const int lg = ufomi_log2(N);
     const int N1 = 1 << (lg / 2), N2 = N / N1;
@@ -361,12 +361,99 @@ int split_transform(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, i
         if (blocks <= 0 || blocks >= (1L << 31))
             return ufomi_fail(UFOMI_ERR_ARG, "transform of %d rows does not fit one launch", rows);
     }
+    int rc = launch_bb<DIR, false, InT, false>(ctx->stream, p1, in, tmp, nullptr, nullptr, ctx->twiddles);
+    if (rc) return rc;
+    return launch_bb<DIR, false, float2, OUT_REAL>(ctx->stream, p2, tmp, out, nullptr, nullptr, ctx->twiddles);
+}
+
+template <int DIR, typename InT, bool OUT_REAL>
+int split_transform(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int N, long keep,
+                    long out_pitch, float scale, void *out)
+{
     void *tmp = nullptr;
     int rc = ufomi_scratch(ctx, (size_t) rows * N * sizeof(float2), &tmp);
     if (rc) return rc;
-    if ((rc = launch_bb<DIR, false, InT, false>(ctx->stream, p1, in, tmp, nullptr, nullptr, ctx->twiddles)))
-        return rc;
-    return launch_bb<DIR, false, float2, OUT_REAL>(ctx->stream, p2, tmp, out, nullptr, nullptr, ctx->twiddles);
+    return split_passes<DIR, InT, OUT_REAL>(ctx, in, in_pitch, valid, rows, N, keep, out_pitch, scale, (float2 *) tmp,
+                                            out);
+}
+
+// Bluestein for rows longer than the LDS passes hold (the same mathematics as
+// the fused pass): premultiply by the chirp into L = 2^k >= 2M - 1 points,
+// forward split transform, product with the kernel spectrum, inverse split
+// transform, postmultiply by the chirp.
+constexpr int kEw = 256;
+
+template <typename InT>
+__global__ __launch_bounds__(kEw) void chirp_pre_kernel(const InT *__restrict__ in, long in_pitch, long valid, int M,
+                                                        int L, const float2 *__restrict__ c, float2 *__restrict__ out,
+                                                        long total)
+{
+    for (long idx = (long) blockIdx.x * kEw + threadIdx.x; idx < total; idx += (long) gridDim.x * kEw) {
+        const long r = idx / L, n = idx - r * L;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (n < M && n < valid)
+            v = cmul(load_as_complex(in, r * in_pitch + n), c[n]);
+        out[idx] = v;
+    }
+}
+
+__global__ __launch_bounds__(kEw) void cmul_rows_kernel(float2 *__restrict__ x, const float2 *__restrict__ k, int L,
+                                                        long total)
+{
+    for (long idx = (long) blockIdx.x * kEw + threadIdx.x; idx < total; idx += (long) gridDim.x * kEw)
+        x[idx] = cmul(x[idx], k[idx % L]);
+}
+
+template <bool OUT_REAL>
+__global__ __launch_bounds__(kEw) void chirp_post_kernel(const float2 *__restrict__ x, int L,
+                                                         const float2 *__restrict__ c, long keep, long out_pitch,
+                                                         float scale, void *__restrict__ out, long total)
+{
+    for (long idx = (long) blockIdx.x * kEw + threadIdx.x; idx < total; idx += (long) gridDim.x * kEw) {
+        const long r = idx / keep, k = idx - r * keep;
+        const float2 y = cmul(x[r * L + k], c[k]);
+        if (OUT_REAL)
+            ((float *) out)[r * out_pitch + k] = y.x * scale;
+        else
+            ((float2 *) out)[r * out_pitch + k] = make_float2(y.x * scale, y.y * scale);
+    }
+}
+
+unsigned ew_blocks(long total) { return (unsigned) std::min<long>((total + kEw - 1) / kEw, 1L << 16); }
+
+template <int DIR, typename InT, bool OUT_REAL>
+int split_chirp(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int M, long keep, long out_pitch,
+                float scale, void *out)
+{
+    const int L = (int) next_pow2(2L * M - 1);
+    if (rows <= 0 || keep > M)
+        return ufomi_fail(UFOMI_ERR_ARG, "invalid transform rows");
+    const size_t row_bytes = (size_t) rows * L * sizeof(float2);
+    const size_t tab = (((size_t) M + L) * sizeof(float2) + 255) / 256 * 256;
+    void *base = nullptr;
+    int rc = ufomi_scratch(ctx, tab + 3 * row_bytes, &base);
+    if (rc) return rc;
+    float2 *chirp = (float2 *) base, *kern = chirp + M;
+    float2 *A = (float2 *) ((char *) base + tab), *B = (float2 *) ((char *) A + row_bytes),
+           *T = (float2 *) ((char *) B + row_bytes);
+    {
+        std::vector<float2> h((size_t) M + L);
+        bluestein_tables(M, L, DIR, h.data(), h.data() + M);
+        if ((rc = ufomi_upload(ctx, chirp, h.data(), h.size() * sizeof(float2)))) return rc;
+    }
+    const long total = (long) rows * L;
+    hipLaunchKernelGGL(chirp_pre_kernel<InT>, dim3(ew_blocks(total)), dim3(kEw), 0, ctx->stream, (const InT *) in,
+                       in_pitch, valid, M, L, chirp, A, total);
+    UFOMI_CHECK_LAUNCH();
+    if ((rc = split_passes<-1, float2, false>(ctx, A, L, L, rows, L, L, L, 1.0f, T, B))) return rc;
+    hipLaunchKernelGGL(cmul_rows_kernel, dim3(ew_blocks(total)), dim3(kEw), 0, ctx->stream, B, kern, L, total);
+    UFOMI_CHECK_LAUNCH();
+    if ((rc = split_passes<+1, float2, false>(ctx, B, L, L, rows, L, L, L, 1.0f, T, A))) return rc;
+    const long tot_out = (long) rows * keep;
+    hipLaunchKernelGGL(chirp_post_kernel<OUT_REAL>, dim3(ew_blocks(tot_out)), dim3(kEw), 0, ctx->stream, A, L, chirp,
+                       keep, out_pitch, scale, out, tot_out);
+    UFOMI_CHECK_LAUNCH();
+    return UFOMI_OK;
 }
 
 }  // namespace
@@ -386,6 +473,12 @@ extern "C" int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, c
     // the radix-16 row kernels for plain 1-D power-of-two transforms of real rows
     if (dimensions == 1 && !in_complex && pow2(m[0]) && m[0] >= 16 && m[0] <= kMaxL && m[1] == n[1] && m[2] == n[2])
         return ufomi_fft_rows(ctx, d_in, n[0], n[1] * n[2], m[0], d_out);
+    // long non-power-of-two rows: Bluestein over the four-step split
+    if (dimensions == 1 && !pow2(m[0]) && m[0] > kMaxL / 2 && m[0] <= kMaxSplit / 2 && m[1] == n[1] && m[2] == n[2]) {
+        const int rows = n[1] * n[2];
+        return in_complex ? split_chirp<-1, float2, false>(ctx, d_in, n[0], n[0], rows, m[0], m[0], m[0], 1.0f, d_out)
+                          : split_chirp<-1, float, false>(ctx, d_in, n[0], n[0], rows, m[0], m[0], m[0], 1.0f, d_out);
+    }
     // long power-of-two rows: two passes of the four-step split
     if (dimensions == 1 && pow2(m[0]) && m[0] > kMaxL && m[0] <= kMaxSplit && m[1] == n[1] && m[2] == n[2]) {
         const int rows = n[1] * n[2];
@@ -443,6 +536,9 @@ extern "C" int ufomi_ifft_nd(ufomi_ctx *ctx, const float *d_in, const int *m, in
     const int cw = crop_width ? crop_width : m[0], ch = crop_height ? crop_height : m[1];
     if (dimensions == 1 && pow2(m[0]) && m[0] >= 16 && m[0] <= kMaxL && (ch == m[1] || m[2] == 1))
         return ufomi_ifft_rows(ctx, d_in, m[0], ch * m[2], cw, d_out);
+    if (dimensions == 1 && !pow2(m[0]) && m[0] > kMaxL / 2 && m[0] <= kMaxSplit / 2 && (ch == m[1] || m[2] == 1))
+        return split_chirp<+1, float2, true>(ctx, d_in, m[0], m[0], ch * m[2], m[0], cw, cw, (float) (1.0 / m[0]),
+                                             d_out);
     if (dimensions == 1 && pow2(m[0]) && m[0] > kMaxL && m[0] <= kMaxSplit && (ch == m[1] || m[2] == 1))
         return split_transform<+1, float2, true>(ctx, d_in, m[0], m[0], ch * m[2], m[0], cw, cw,
                                                  (float) (1.0 / m[0]), d_out);
