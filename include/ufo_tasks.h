@@ -9,8 +9,9 @@
  * header restates the interface in plain C with the same task names, property
  * names, types, ranges, defaults, requisition rules and error conditions, and
  * prefixes every symbol with ufomi_ so it can live in the same process as
- * ufo-core.  A GObject plugin that wants to be a literal drop-in forwards its
- * vfuncs to these functions.
+ * ufo-core.  A GObject plugin that wants to be a literal drop-in keeps its
+ * GType and property installation and forwards its UfoTaskIface vfuncs to
+ * these functions.
  *
  * Buffers hold device memory of the task's ufomi context.
  */

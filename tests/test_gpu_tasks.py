@@ -146,7 +146,8 @@ def test_general_backproject_reductor_lifecycle(ctx, torch_dev):
 
 
 def test_fdk_pipeline_vs_oracle(ctx, torch_dev):
-    """cone-beam-projection-weight ! fft ! filter ! ifft ! general-backproject (SURVEY call stack E)."""
+    """cone-beam-projection-weight ! fft ! filter ! ifft ! general-backproject: the FDK chain of the
+    reference's cone-beam examples."""
     import torch
     rng = np.random.default_rng(5)
     p, h, w, n = 24, 12, 24, 64
