@@ -144,7 +144,7 @@ template <int S, int TILE, int A, typename InT, bool H16 = false, typename InPtr
 __device__ __forceinline__ void bp_tile(
     InPtr __restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
     float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices, int slice0,
-    OutPtr __restrict__ out, long out_pitch, int x0, int y0)
+    OutPtr __restrict__ out, long out_pitch, int x0, int y0, int p_norm = 0)
 {
     static_assert(!H16 || (S % 4 == 0 && std::is_same<InT, __half>::value), "half staging: f16, 4 | S");
     using L = LdsT<S, H16>;
@@ -355,7 +355,8 @@ __device__ __forceinline__ void bp_tile(
         }
     }
 
-    const float fp = (float) P;
+    // pi / (sinogram height); an angle-split part divides by the whole height
+    const float fp = (float) (p_norm ? p_norm : P);
 #pragma unroll
     for (int ky = 0; ky < K; ky++) {
         const int y = y0 + ly + 16 * ky;
@@ -380,15 +381,39 @@ template <int S, int A, typename InT, bool H16>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(S == 8 && !H16 ? 2 : 4))) void
 bp_texture_kernel(const InT *__restrict__ sino, long sino_pitch, int W, int P, const float2 *__restrict__ trig,
                   float axis, float roi_x, float roi_y, int roi_w, int roi_h, int n_slices,
-                  float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y)
+                  float *__restrict__ out, long out_pitch, int tiles_x, int tiles_y, int p_chunk)
 {
     const int nb = tiles_x * tiles_y * ((n_slices + S - 1) / S);
     const int b = xcd_remap(blockIdx.x, nb);
     const int tiles = tiles_x * tiles_y;
     const int g = b / tiles;
     const int ty = (b % tiles) / tiles_x, tx = b % tiles_x;
+    if (p_chunk > 0) {
+        // angle split j = blockIdx.y: angles [j p_chunk, ...) into partial volume j
+        const int p0 = blockIdx.y * p_chunk;
+        bp_tile<S, kTile, A, InT, H16>(sino + (long) p0 * W, sino_pitch, W, min(p_chunk, P - p0), trig + p0, axis,
+                                       roi_x, roi_y, roi_w, roi_h, n_slices, g * S,
+                                       out + (long) blockIdx.y * n_slices * out_pitch, out_pitch, tx * kTile,
+                                       ty * kTile, P);
+        return;
+    }
     bp_tile<S, kTile, A, InT, H16>(sino, sino_pitch, W, P, trig, axis, roi_x, roi_y, roi_w, roi_h, n_slices, g * S,
                                    out, out_pitch, tx * kTile, ty * kTile);
+}
+
+// Sum of the partial volumes of an angle-split launch, in split order.
+__global__ __launch_bounds__(kThreads) void bp_sum_parts_kernel(const float *__restrict__ part, int n_split,
+                                                                long part_stride, long plane, int n_slices,
+                                                                float *__restrict__ out, long out_pitch)
+{
+    const long total = plane * n_slices;
+    for (long i = (long) blockIdx.x * kThreads + threadIdx.x; i < total; i += (long) gridDim.x * kThreads) {
+        float s = part[i];
+        for (int j = 1; j < n_split; j++)
+            s += part[j * part_stride + i];
+        const long sl = i / plane;
+        out[sl * out_pitch + (i - sl * plane)] = s;
+    }
 }
 
 // Persistent backprojection of a batch of single-slice requests of uneven
@@ -596,6 +621,33 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
     const long nb = (long) tiles_x * tiles_y * ((n_slices + S - 1) / S);
     if (nb > 0x7fffffffL)
         return ufomi_fail(UFOMI_ERR_ARG, "too many tiles");
+    // Few tiles (one 512^2 slice is 256 workgroups for 256 CUs): split the
+    // angles over up to 8 launches' worth of workgroups, at least 64 angles
+    // each, into partial volumes summed afterwards in a fixed order.
+    int n_split = 1, p_chunk = 0;
+    if (nb < 1024 && prm->mode == UFOMI_BP_TEXTURE) {
+        n_split = (int) std::max<long>(1, std::min<long>((1024 + nb - 1) / nb, std::min(8, P_in / 64)));
+        if (n_split > 1) {
+            p_chunk = (P_in + n_split - 1) / n_split;
+            n_split = (P_in + p_chunk - 1) / p_chunk;
+        }
+    }
+    const long plane = (long) roi_w * roi_h;
+    float *part = nullptr;
+    if (n_split > 1) {
+        const size_t bytes = (size_t) n_split * n_slices * plane * sizeof(float);
+        if (bytes > ctx->bp_part_bytes) {
+            if (ctx->bp_part) {
+                UFOMI_HIP(hipStreamSynchronize(ctx->stream));
+                UFOMI_HIP(hipFree(ctx->bp_part));
+                ctx->bp_part = nullptr;
+                ctx->bp_part_bytes = 0;
+            }
+            UFOMI_HIP(hipMalloc(&ctx->bp_part, bytes));
+            ctx->bp_part_bytes = bytes;
+        }
+        part = ctx->bp_part;
+    }
     auto go = [&](auto s_const, auto a_const, auto h_const, auto *in) {
         constexpr int SS = decltype(s_const)::value, AA = decltype(a_const)::value;
         constexpr bool HH = decltype(h_const)::value;
@@ -603,10 +655,18 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
         if constexpr (HH && !(std::is_same<InT, __half>::value && SS % 4 == 0)) {
             return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "half staging needs f16 storage and 4 | S");
         } else {
-            hipLaunchKernelGGL((bp_texture_kernel<SS, AA, InT, HH>), dim3((unsigned) nb), dim3(kThreads), 0,
-                               ctx->stream, in, (long) sino_pitch, W, P_in, trig, axis, (float) prm->roi_x,
-                               (float) prm->roi_y, roi_w, roi_h, n_slices, d_out, (long) out_pitch, tiles_x, tiles_y);
+            hipLaunchKernelGGL((bp_texture_kernel<SS, AA, InT, HH>), dim3((unsigned) nb, (unsigned) n_split),
+                               dim3(kThreads), 0, ctx->stream, in, (long) sino_pitch, W, P_in, trig, axis,
+                               (float) prm->roi_x, (float) prm->roi_y, roi_w, roi_h, n_slices,
+                               part ? part : d_out, part ? plane : (long) out_pitch, tiles_x, tiles_y, p_chunk);
             UFOMI_CHECK_LAUNCH();
+            if (part) {
+                const long total = plane * n_slices;
+                const unsigned blocks = (unsigned) std::min<long>((total + kThreads - 1) / kThreads, 4096);
+                hipLaunchKernelGGL(bp_sum_parts_kernel, dim3(blocks), dim3(kThreads), 0, ctx->stream, part, n_split,
+                                   (long) n_slices * plane, plane, n_slices, d_out, (long) out_pitch);
+                UFOMI_CHECK_LAUNCH();
+            }
             return (int) UFOMI_OK;
         }
     };

@@ -125,6 +125,8 @@ extern "C" int ufomi_destroy(ufomi_ctx *ctx)
     (void) hipFree(ctx->flags);
     if (ctx->lut)
         (void) hipFree(ctx->lut);
+    if (ctx->bp_part)
+        (void) hipFree(ctx->bp_part);
     if (ctx->scratch)
         hipFree(ctx->scratch);
     if (ctx->hscratch)

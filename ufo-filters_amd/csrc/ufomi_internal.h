@@ -28,6 +28,9 @@ struct ufomi_ctx {
     int lut_sign;            // +1: (sin, cos); -1: (-sin, cos)
     // parallel backprojection tile configuration (0 = automatic), ufomi_bp_configure
     int bp_slices, bp_chunk, bp_half;
+    // partial sums of angle-split backprojections of few tiles (grow-only)
+    float *bp_part;
+    size_t bp_part_bytes;
 };
 
 #define UFOMI_FFT_MAX 16384  // twiddle table length
