@@ -203,6 +203,9 @@ __device__ __forceinline__ void bp_tile(
     constexpr bool kSeam = kDpp && STRIDE == 128;
     R r0[PASSES][S], r1[(kSeam || !kDpp) ? PASSES : 1][S];
     bool in0[PASSES], in1[PASSES];
+    // the staged span of an angle lies inside the detector (no border masking);
+    // one angle row is one or two whole waves, so this is uniform per wave
+    bool inner[PASSES];
     const bool wave_last = kSeam && e == 63;
     // (sin, cos) of the angles the next fetch() stages, loaded one chunk ahead
     // so that no fetch waits for the trig table before issuing its loads.
@@ -221,6 +224,7 @@ __device__ __forceinline__ void bp_tile(
             const int j = sp[m].lo + e;
             in0[m] = p < P && j >= 0 && j < W;
             in1[m] = p < P && j + 1 >= 0 && j + 1 < W;
+            inner[m] = p < P && sp[m].lo >= 0 && sp[m].lo + STRIDE + 1 <= W;
             const int j0 = min(max(j, 0), W - 1), j1 = min(max(j + 1, 0), W - 1);
             // byte offsets in a slice's sinogram (< 4 GiB, checked on the host),
             // shared by the S slices, whose base pointers are uniform
@@ -245,21 +249,39 @@ __device__ __forceinline__ void bp_tile(
 #pragma unroll
         for (int m = 0; m < PASSES; m++) {
             const int pa = arow + ROWS * m;
+            if (inner[m]) {
 #pragma unroll
-            for (int s = 0; s < S; s++) {
-                const float v0 = in0[m] ? Stage<InT>::cvt(r0[m][s]) : 0.0f;
-                // entry e + 1 of the same angle sits in the next lane of this wave
-                float v1;
-                if constexpr (!kDpp) {
-                    v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
-                } else {
-                    v1 = next_lane(v0);
-                    if constexpr (kSeam) {
-                        if (wave_last) v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
+                for (int s = 0; s < S; s++) {
+                    const float v0 = Stage<InT>::cvt(r0[m][s]);
+                    float v1;
+                    if constexpr (!kDpp) {
+                        v1 = Stage<InT>::cvt(r1[m][s]);
+                    } else {
+                        v1 = next_lane(v0);
+                        if constexpr (kSeam) {
+                            if (wave_last) v1 = Stage<InT>::cvt(r1[m][s]);
+                        }
                     }
+                    sv[s] = v0;
+                    sd[s] = v1 - v0;
                 }
-                sv[s] = v0;
-                sd[s] = v1 - v0;
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    const float v0 = in0[m] ? Stage<InT>::cvt(r0[m][s]) : 0.0f;
+                    // entry e + 1 of the same angle sits in the next lane of this wave
+                    float v1;
+                    if constexpr (!kDpp) {
+                        v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
+                    } else {
+                        v1 = next_lane(v0);
+                        if constexpr (kSeam) {
+                            if (wave_last) v1 = in1[m] ? Stage<InT>::cvt(r1[m][s]) : 0.0f;
+                        }
+                    }
+                    sv[s] = v0;
+                    sd[s] = v1 - v0;
+                }
             }
             if constexpr (H16) {
 #pragma unroll
@@ -388,17 +410,12 @@ bp_texture_kernel(const InT *__restrict__ sino, long sino_pitch, int W, int P, c
     const int tiles = tiles_x * tiles_y;
     const int g = b / tiles;
     const int ty = (b % tiles) / tiles_x, tx = b % tiles_x;
-    if (p_chunk > 0) {
-        // angle split j = blockIdx.y: angles [j p_chunk, ...) into partial volume j
-        const int p0 = blockIdx.y * p_chunk;
-        bp_tile<S, kTile, A, InT, H16>(sino + (long) p0 * W, sino_pitch, W, min(p_chunk, P - p0), trig + p0, axis,
-                                       roi_x, roi_y, roi_w, roi_h, n_slices, g * S,
-                                       out + (long) blockIdx.y * n_slices * out_pitch, out_pitch, tx * kTile,
-                                       ty * kTile, P);
-        return;
-    }
-    bp_tile<S, kTile, A, InT, H16>(sino, sino_pitch, W, P, trig, axis, roi_x, roi_y, roi_w, roi_h, n_slices, g * S,
-                                   out, out_pitch, tx * kTile, ty * kTile);
+    // angle split j = blockIdx.y (p_chunk > 0): angles [j p_chunk, ...) into partial volume j
+    const int p0 = p_chunk > 0 ? (int) blockIdx.y * p_chunk : 0;
+    const int np = p_chunk > 0 ? min(p_chunk, P - p0) : P;
+    float *o = p_chunk > 0 ? out + (long) blockIdx.y * n_slices * out_pitch : out;
+    bp_tile<S, kTile, A, InT, H16>(sino + (long) p0 * W, sino_pitch, W, np, trig + p0, axis, roi_x, roi_y, roi_w,
+                                   roi_h, n_slices, g * S, o, out_pitch, tx * kTile, ty * kTile, P);
 }
 
 // Sum of the partial volumes of an angle-split launch, in split order.
