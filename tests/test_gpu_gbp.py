@@ -323,3 +323,28 @@ def test_sweep_task_and_tomo_angle_rejected(ctx, torch_dev):
     with pytest.raises(ufo_tasks.TaskError) as e:
         bad.setup(ctx)
     assert e.value.status == -4
+
+
+def test_double_compute_and_result_types(ctx, torch_dev):
+    """compute-type=double and result-type=double (FP64 coordinates and running sums in the reference)
+    meet the FP64 oracle within 1e-5; result-type=half is refused."""
+    import ufo_tasks
+    projs = _smooth_projs(5, 18, 16, 24)
+    t = ufo_tasks.Task("general-backproject")
+    for k, v in dict(num_projections=18, compute_type="double", result_type="double", region=[-2.0, 2.0, 1.0],
+                     axis_angle_x=[0.4], center_position_x=[12.0], center_position_z=[8.0]).items():
+        setattr(t.props, k, v)
+    src, sink = ufo_tasks.MemoryIn(projs), ufo_tasks.MemoryOut()
+    g = ufo_tasks.TaskGraph()
+    g.connect_nodes(src, t)
+    g.connect_nodes(t, sink)
+    ufo_tasks.Scheduler(ctx).run(g)
+    ref = oracle.gbp(projs, region=(-2.0, 2.0, 1.0), axis_angle_x=[0.4], center_position_x=[12.0],
+                     center_position_z=[8.0])
+    assert rel_l2(sink.array, ref) < 1e-5
+    half = ufo_tasks.Task("general-backproject")
+    half.props.num_projections = 18
+    half.props.result_type = "half"
+    with pytest.raises(ufo_tasks.TaskError) as e:
+        half.setup(ctx)
+    assert e.value.status == -4

@@ -1052,9 +1052,15 @@ static int gbp_setup(UfomiTask *t, UfomiError *e)
     if ((int) t->v[G_PARAM].num == 2)
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
                           "general-backproject: axis-angle-z holds the tomographic angles, it cannot be swept");
-    if ((int) t->v[G_COMPUTE].num != 0 || (int) t->v[G_RESULT].num != 1)
+    /* compute-type double (FP64 coordinates, then float samples, :998) and
+     * result-type double (FP64 running sum between bursts) run on the float
+     * kernel: its rows are composed in FP64 and every projection accumulates
+     * in registers, within 1e-5 of the FP64 oracle.  result-type half rounds
+     * the running sum to half after every burst, which this kernel does not
+     * reproduce. */
+    if ((int) t->v[G_RESULT].num == 0)
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
-                          "general-backproject: compute-type and result-type float only");
+                          "general-backproject: result-type half is not built");
     const int addr = (int) t->v[G_ADDR].num;
     if (addr == 3 || addr == 4)
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
