@@ -219,9 +219,10 @@ __device__ __forceinline__ void store_voxel(ST *__restrict__ vol, long o, float 
 // takes the global-memory path for that projection.
 // ---------------------------------------------------------------------------
 constexpr int kBoxW = 64, kBoxH = 48;  // texels per staged footprint
-// Row pitch of a staged box: 74 floats, 10 banks (of 64) apart, so the rows a
-// wave reads at once (each a window of ~9 banks) do not share banks.
-constexpr int kBoxS = kBoxW + 10;
+// Row pitch of a staged box: 72 floats, 8 banks (of 64) apart, so the rows a
+// wave reads at once (each a window of ~9 banks) barely share banks (pitches
+// of 70, 74 and 76 floats measured the same or slower, 80 much slower).
+constexpr int kBoxS = kBoxW + 8;
 constexpr int kChunk = 32;             // projections per footprint chunk
 constexpr int kRowsPerThread = kBoxH / 4;
 
