@@ -141,28 +141,35 @@ def test_tile_configurations_bitwise(ctx, torch_dev, slices, chunk):
     sinos = rng.standard_normal((11, 70, 64)).astype(np.float32)
     ref = _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75)
     try:
+        ctx.bp_configure(0, 0, -1)  # bf16 reference with float staging (16-bit staging rounds more)
+        ref16 = _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75, dtype=ufomi.BF16)
         ctx.bp_configure(slices, chunk, -1)
         got = _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75)
         got16 = _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75, dtype=ufomi.BF16)
     finally:
         ctx.bp_configure(0, 0, 0)
     np.testing.assert_array_equal(got, ref)
-    np.testing.assert_array_equal(got16, _bp(ctx, torch_dev, sinos, n_slices=11, axis_pos=31.75, dtype=ufomi.BF16))
+    np.testing.assert_array_equal(got16, ref16)
 
 
+@pytest.mark.parametrize("dtype,tol", [(ufomi.F16, 1e-3)])
 @pytest.mark.parametrize("slices", [4, 8])
-def test_half_staging(ctx, torch_dev, slices):
-    """f16 storage with half (v, d) staging and v_dot2 interpolation: one more f16
-    rounding (the sample difference and the weight), still within the f16 bar."""
+def test_half_staging(ctx, torch_dev, slices, dtype, tol):
+    """f16 storage with half (v, d) staging and v_dot2_f32_f16 interpolation: one more f16
+    rounding of the sample difference and the weight, still within the f16 bar, and close to
+    the float-staged result."""
     rng = np.random.default_rng(13)
     sinos = rng.standard_normal((9, 64, 80)).astype(np.float32)
     ref = np.stack([oracle.bp_texture_f64(s) for s in sinos])
     try:
         ctx.bp_configure(slices, 8, 1)
-        got = _bp(ctx, torch_dev, sinos, n_slices=9, dtype=ufomi.F16)
+        got = _bp(ctx, torch_dev, sinos, n_slices=9, dtype=dtype)
+        ctx.bp_configure(slices, 8, -1)
+        plain = _bp(ctx, torch_dev, sinos, n_slices=9, dtype=dtype)
     finally:
         ctx.bp_configure(0, 0, 0)
-    assert rel_l2(got, ref) < 1e-3
+    assert rel_l2(got, ref) < tol
+    assert rel_l2(got, plain) < tol / 2
     with pytest.raises(ufomi.UfomiError):
         ctx.bp_configure(3, 0, 0)
     with pytest.raises(ufomi.UfomiError):
