@@ -229,3 +229,13 @@ def test_long_rows_chirp(ctx, width, size_x):
     back = _run(ctx, [x], _task("fft", dimensions=1, size_x=size_x), _task("ifft", dimensions=1,
                                                                            crop_width=width))[0]
     assert rel_l2(back, x) < 1e-5
+
+
+def test_reference_shell_test_177(ctx):
+    """tests/test-177.sh: a 9x9 array of ones through fft dimensions=2 auto-zeropadding=True, then
+    ifft dimensions=2 crop-width=9 crop-height=9; the maximum absolute error must be <= 1e-6."""
+    x = np.ones((9, 9), np.float32)
+    out = _run(ctx, [x], _task("fft", dimensions=2, auto_zeropadding=True),
+               _task("ifft", dimensions=2, crop_width=9, crop_height=9))[0]
+    assert out.shape == (9, 9)
+    assert np.abs(out - 1.0).max() <= 1e-6
