@@ -245,6 +245,16 @@ template <int N> struct Plan {
     static constexpr int radix(int p) { return (REM && p == 0) ? REM : 16; }
 };
 
+// LDS slot of element i of a transform's exchange buffer.  A radix-16 pass
+// stores at stride 16 (the first pass: 16 j + r), which puts the lanes of a
+// wave 32 dwords apart, on the same few banks; one pad element after every 16
+// spreads them.  N = 8192 stays unpadded (its buffer would pass 64 KiB).
+template <int N> struct Lds {
+    static constexpr bool PAD = N <= 4096;
+    static constexpr int STRIDE = PAD ? N + N / 16 : N;  // elements per transform
+    static __device__ __forceinline__ int at(int i) { return PAD ? i + (i >> 4) : i; }
+};
+
 // One Stockham pass in registers: v holds this thread's butterflies of radix R
 // (values v[q*R + r] = x[j + r N/R], j = t + q TF); stride Ns of the pass.
 template <int N, int R, int DIR>
@@ -276,7 +286,7 @@ __device__ __forceinline__ void pass_store(float2 *x, const float2 *v, int t, in
         const int idx = (j - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++)
-            x[idx + r * Ns] = v[q * R + r];
+            x[Lds<N>::at(idx + r * Ns)] = v[q * R + r];
     }
 }
 
@@ -291,7 +301,7 @@ __device__ __forceinline__ void pass_load(const float2 *x, float2 *v, int t)
         const int j = t + q * P::TF;
 #pragma unroll
         for (int r = 0; r < R; r++)
-            v[q * R + r] = x[j + r * (N / R)];
+            v[q * R + r] = x[Lds<N>::at(j + r * (N / R))];
     }
 }
 

@@ -193,7 +193,7 @@ __device__ __forceinline__ void ramp16_block(
     constexpr int R0 = P::radix(0), RL = P::radix(P::NPASS - 1);
     extern __shared__ float2 lds[];
     const int f = threadIdx.x / P::TF, t = threadIdx.x % P::TF;
-    float2 *x = lds + (size_t) f * N;
+    float2 *x = lds + (size_t) f * fft16::Lds<N>::STRIDE;
     if (pair && 2 * blk * P::G >= rows)
         return;  // whole block beyond the last pair
     const long unit = blk * P::G + f;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kThreads) void fft16_rows_kernel(const float *__res
     constexpr int R0 = P::radix(0), RL = P::radix(P::NPASS - 1);
     extern __shared__ float2 lds[];
     const int f = threadIdx.x / P::TF, t = threadIdx.x % P::TF;
-    float2 *x = lds + (size_t) f * N;
+    float2 *x = lds + (size_t) f * fft16::Lds<N>::STRIDE;
     const long r = (long) blockIdx.x * P::G + f;
     const bool valid = r < rows;
     const float *a = in + r * W;
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kThreads) void ifft16_rows_kernel(const float2 *__r
     constexpr int R0 = P::radix(0), RL = P::radix(P::NPASS - 1);
     extern __shared__ float2 lds[];
     const int f = threadIdx.x / P::TF, t = threadIdx.x % P::TF;
-    float2 *x = lds + (size_t) f * N;
+    float2 *x = lds + (size_t) f * fft16::Lds<N>::STRIDE;
     const long r = (long) blockIdx.x * P::G + f;
     const bool valid = r < rows;
     float2 v[P::V];
@@ -334,7 +334,7 @@ int launch_by_n(int N, K k)
 template <int N> size_t lds16_bytes()
 {
     using P = fft16::Plan<N>;
-    return P::NPASS > 1 ? (size_t) P::G * N * sizeof(float2) : 0;
+    return P::NPASS > 1 ? (size_t) P::G * fft16::Lds<N>::STRIDE * sizeof(float2) : 0;
 }
 
 }  // namespace
