@@ -220,3 +220,23 @@ def test_scheduler_read_write_tasks(ctx, torch_dev, tmp_path):
         im.seek(i)
         ref = oracle.bp_texture_ref32(sinos[i], axis_pos=axis)
         assert rel_l2(np.asarray(im), ref) < 1e-5
+
+
+def test_wide_detector_fbp_pipeline(ctx, torch_dev):
+    """A detector wider than 4096 px: `fft size-x=16384 ! filter ! ifft ! backproject` as one
+    ufo-launch description runs the unfused chain (four-step transforms) and then backprojects
+    a 64 x 64 region of interest; equal to the FP64 oracle's ramp chain and backprojection."""
+    import torch
+    w, p, n = 8200, 16, 16384
+    rng = np.random.default_rng(17)
+    raw = rng.standard_normal((p, w + 32))
+    k = np.exp(-0.5 * (np.arange(-16, 17) / 6.0) ** 2)
+    sino = np.stack([np.convolve(r, k, mode="valid")[:w] for r in raw]).astype(np.float32)
+    desc = ("fft dimensions=1 size-x=%d ! filter ! ifft dimensions=1 crop-width=%d ! "
+            "backproject axis-pos=4100.5 roi-x=4068 roi-y=4068 roi-width=64 roi-height=64" % (n, w))
+    raw_out, n_out, shape = ufo_tasks.Pipeline(desc).run(ctx, torch.from_numpy(sino[None]).to(torch_dev))
+    assert n_out == 1 and shape == (64, 64)
+    got = raw_out[:64 * 64 * 4].view(torch.float32).reshape(64, 64).cpu().numpy()
+    filt = oracle.ramp_chain(sino, n, oracle.filter_coefficients(2 * n, "ramp-fromreal"), w)
+    ref = oracle.bp_texture_f64(filt, axis_pos=4100.5, roi_x=4068, roi_y=4068, roi_width=64, roi_height=64)
+    assert rel_l2(got, ref) < 1e-5

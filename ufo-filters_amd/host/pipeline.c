@@ -509,6 +509,8 @@ static int run_batch(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const
                 goto out;
             if ((rc = batch_alloc(ctx, &next, &req, cur.n, sizeof(float), UFOMI_F32, ufomi_task_output_layout(t))))
                 goto fail_alloc;
+            /* the layout the task gives its outputs (filter keeps its input's) */
+            UfomiBufferLayout layout = ufomi_task_output_layout(t);
             for (int i = 0; i < cur.n; i++) {
                 UfomiBuffer v = item_view(&cur, i), o = item_view(&next, i), *vi[1] = {&v};
                 if (!ufomi_task_process(t, vi, &o, &req)) {
@@ -517,8 +519,9 @@ static int run_batch(UfomiPipeline *pl, ufomi_ctx *ctx, const float *d_in, const
                                     ufomi_task_last_error(t));
                     goto out;
                 }
+                layout = o.layout;
             }
-            next.buf.layout = ufomi_task_output_layout(t);
+            next.buf.layout = layout;
         }
         if (cur_owned)
             ufomi_buffer_release(ctx, &cur.buf);
