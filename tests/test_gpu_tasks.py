@@ -240,3 +240,21 @@ def test_wide_detector_fbp_pipeline(ctx, torch_dev):
     filt = oracle.ramp_chain(sino, n, oracle.filter_coefficients(2 * n, "ramp-fromreal"), w)
     ref = oracle.bp_texture_f64(filt, axis_pos=4100.5, roi_x=4068, roi_y=4068, roi_width=64, roi_height=64)
     assert rel_l2(got, ref) < 1e-5
+
+
+def test_2d_fft_filter_ifft_pipeline(ctx, torch_dev):
+    """fft dimensions=2 ! filter ! ifft dimensions=2 as one description (unfused): the filter
+    multiplies every row of the 2-D spectrum by its coefficients; against numpy in float64."""
+    import torch
+    h, w, n = 12, 30, 32
+    x = np.random.default_rng(23).standard_normal((h, w)).astype(np.float32)
+    desc = ("fft dimensions=2 size-x=%d size-y=16 ! filter filter=butterworth ! ifft dimensions=2 crop-width=%d "
+            "crop-height=%d" % (n, w, h))
+    raw, n_out, shape = ufo_tasks.Pipeline(desc).run(ctx, torch.from_numpy(x[None]).to(torch_dev))
+    assert n_out == 1 and shape == (h, w)
+    got = raw[:w * h * 4].view(torch.float32).reshape(h, w).cpu().numpy()
+    coef = oracle.filter_coefficients(2 * n, "butterworth").astype(np.float64)
+    spec = np.fft.fft2(np.pad(x.astype(np.float64), ((0, 16 - h), (0, n - w))))
+    spec = spec.real * coef[0::2][None, :] + 1j * spec.imag * coef[1::2][None, :]
+    ref = np.fft.ifft2(spec).real[:h, :w]
+    assert rel_l2(got, ref) < 1e-5
