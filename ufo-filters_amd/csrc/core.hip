@@ -150,10 +150,14 @@ extern "C" int ufomi_device_alloc(ufomi_ctx *ctx, size_t bytes, void **out)
     if (!out)
         return ufomi_fail(UFOMI_ERR_ARG, "null output pointer");
     *out = nullptr;
-    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+    const hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
         (void) hipGetLastError();
         *out = nullptr;
-        return ufomi_fail(UFOMI_ERR_MEMORY, "cannot allocate %zu bytes of device memory", bytes);
+        size_t free_b = 0, total_b = 0;
+        (void) hipMemGetInfo(&free_b, &total_b);
+        return ufomi_fail(UFOMI_ERR_MEMORY, "cannot allocate %zu bytes of device memory (%s; %zu of %zu bytes free)",
+                          bytes, hipGetErrorString(e), free_b, total_b);
     }
     return UFOMI_OK;
 }
