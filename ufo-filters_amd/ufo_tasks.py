@@ -412,7 +412,12 @@ class Pipeline:
         last = self.L.ufomi_task_name(self.L.ufomi_pipeline_task(self.h, n_tasks - 1)).decode()
         out, cap = None, 0
         if last != "write":
-            cap = out_bytes or (items.numel() * 4 * 8 if items is not None else 1 << 30)
+            # without out_bytes: room for 8x the input, but at most half the free device
+            # memory (the stages' own batches need the rest)
+            cap = out_bytes
+            if not cap:
+                free, _ = torch.cuda.mem_get_info(ctx.device)
+                cap = min(items.numel() * 4 * 8 if items is not None else 1 << 30, free // 2)
             out = torch.empty(cap, dtype=torch.uint8, device="cuda:%d" % ctx.device)
         n_out, oreq, err = ctypes.c_int(), Requisition(), Error()
         rc = self.L.ufomi_pipeline_run(self.h, ctx.h, ptr, reqp, n, out.data_ptr() if out is not None else None, cap,
