@@ -434,6 +434,23 @@ def gbp(projs, num_projections=None, overall_angle=2 * math.pi, region=(0.0, 0.0
     return out
 
 
+class gbp_planted:
+    """Context manager that switches on planted mistakes of orc_gbp's FP64 mode (bit mask of
+    orc_gbp_plant in ufo_oracle.c), for tests that show the geometry checks catch them."""
+
+    def __init__(self, mask):
+        self.mask = int(mask)
+
+    def __enter__(self):
+        self._var = ctypes.c_int.in_dll(lib(), "orc_gbp_plant")
+        self._var.value = self.mask
+        return self
+
+    def __exit__(self, *exc):
+        self._var.value = 0
+        return False
+
+
 LAMINO_PARAMETERS = ["z", "x-center", "lamino-angle", "roll-angle"]
 
 

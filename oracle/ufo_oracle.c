@@ -34,7 +34,14 @@
  *   orc_gbp_*              reduction to orc_bp_texture for untilted parallel
  *                          geometry, point-source closed form for cone,
  *                          lamino tilt 0 == tomography, brute force on tiny
- *                          inputs.
+ *                          inputs; every geometry branch (tilted detector,
+ *                          shifted detector and source, volume and axis
+ *                          rotation, cone and parallel) against an
+ *                          independent ray-plane model with Rodrigues rotation
+ *                          matrices and against hand-derived closed forms of
+ *                          the projected coordinates
+ *                          (tests/test_oracle_gbp_geometry.py), with planted
+ *                          mistakes (orc_gbp_plant) shown to fail them.
  */
 #include <float.h>
 #include <math.h>
@@ -548,6 +555,18 @@ static inline float tex2_f(const float *img, int w, int h, float x, float y, int
 }
 
 /*
+ * Planted mistakes for the pinning tests (0 = the faithful restatement).  A test
+ * sets a bit, checks that the independent geometry checks now FAIL, and clears it:
+ *   bit 0: the inverse detector rotation about x keeps +sin (minus sign dropped);
+ *   bit 1: the shifted-source term of the perpendicular cone branch has its sign flipped;
+ *   bit 2: the volume rotation runs rotate_x, rotate_y, rotate_z (order reversed);
+ *   bit 3: the detector normal is rotated x, y, z (order reversed);
+ *   bit 4: the cone weight uses the source distance alone ((S_y - D_y) / S_y).
+ * Only the FP64 mode reads it.
+ */
+int orc_gbp_plant = 0;
+
+/*
  * One voxel of the generated `backproject` kernel for the z parameter
  * (general_bp_body.in:1-32 filled by make_kernel, :1069-1180), FP64.
  * (x, y, z) is voxel_0 before magnification: x_region.x + idx * x_region.y etc.
@@ -567,17 +586,29 @@ static double gbp_voxel_f64(const float *projs, int w, int h, int P, const orc_g
             v0.x *= m; v0.y *= m; v0.z *= m;
         }
         if (!fl->perpendicular_detector) {
-            n = rz_d (sin (q->det_az), cos (q->det_az), n);
-            n = ry_d (sin (q->det_ay), cos (q->det_ay), n);
-            n = rx_d (sin (q->det_ax), cos (q->det_ax), n);
+            if (orc_gbp_plant & 8) {
+                n = rx_d (sin (q->det_ax), cos (q->det_ax), n);
+                n = ry_d (sin (q->det_ay), cos (q->det_ay), n);
+                n = rz_d (sin (q->det_az), cos (q->det_az), n);
+            } else {
+                n = rz_d (sin (q->det_az), cos (q->det_az), n);
+                n = ry_d (sin (q->det_ay), cos (q->det_ay), n);
+                n = rx_d (sin (q->det_ax), cos (q->det_ax), n);
+            }
             det_off = -(q->dx * n.x + q->dy * n.y + q->dz * n.z);
         } else if (!fl->parallel_beam) {
             project_tmp = q->dy - q->sy;
         }
         if (fl->with_volume) {
-            v0 = rz_d (sin (q->vol_z), cos (q->vol_z), v0);
-            v0 = ry_d (sin (q->vol_y), cos (q->vol_y), v0);
-            v0 = rx_d (sin (q->vol_x), cos (q->vol_x), v0);
+            if (orc_gbp_plant & 4) {
+                v0 = rx_d (sin (q->vol_x), cos (q->vol_x), v0);
+                v0 = ry_d (sin (q->vol_y), cos (q->vol_y), v0);
+                v0 = rz_d (sin (q->vol_z), cos (q->vol_z), v0);
+            } else {
+                v0 = rz_d (sin (q->vol_z), cos (q->vol_z), v0);
+                v0 = ry_d (sin (q->vol_y), cos (q->vol_y), v0);
+                v0 = rx_d (sin (q->vol_x), cos (q->vol_x), v0);
+            }
         }
         if (!(fl->perpendicular_detector || fl->parallel_beam))
             tmp_tr = -(det_off + (q->sx * n.x + q->sy * n.y + q->sz * n.z));
@@ -588,11 +619,13 @@ static double gbp_voxel_f64(const float *projs, int w, int h, int P, const orc_g
             v = rx_d (sin (q->axis_x), cos (q->axis_x), v);
         }
         if (!fl->parallel_beam)
-            coeff = (q->sy - q->dy) / (q->sy - v.y);
+            coeff = (orc_gbp_plant & 16) ? (q->sy - q->dy) / q->sy : (q->sy - q->dy) / (q->sy - v.y);
         if (fl->perpendicular_detector) {
             if (!fl->parallel_beam) {
                 double t = project_tmp / (v.y - q->sy);
-                if (fl->shifted_source) {
+                if (fl->shifted_source && (orc_gbp_plant & 2)) {
+                    v.x = t * (v.x + q->sx) + q->sx; v.y = t * (v.y - q->sy) + q->sy; v.z = t * (v.z + q->sz) + q->sz;
+                } else if (fl->shifted_source) {
                     v.x = t * (v.x - q->sx) + q->sx; v.y = t * (v.y - q->sy) + q->sy; v.z = t * (v.z - q->sz) + q->sz;
                 } else {
                     v.x = t * v.x + q->sx; v.y = t * v.y + q->sy; v.z = t * v.z + q->sz;
@@ -610,7 +643,7 @@ static double gbp_voxel_f64(const float *projs, int w, int h, int P, const orc_g
         if (!fl->perpendicular_detector || fl->shifted_detector) {
             v.x -= q->dx; v.y -= q->dy; v.z -= q->dz;
             if (!fl->perpendicular_detector) {
-                v = rx_d (-sin (q->det_ax), cos (q->det_ax), v);
+                v = rx_d ((orc_gbp_plant & 1) ? sin (q->det_ax) : -sin (q->det_ax), cos (q->det_ax), v);
                 v = ry_d (-sin (q->det_ay), cos (q->det_ay), v);
                 v = rz_d (-sin (q->det_az), cos (q->det_az), v);
             }

@@ -348,3 +348,19 @@ def test_double_compute_and_result_types(ctx, torch_dev):
     with pytest.raises(ufo_tasks.TaskError) as e:
         half.setup(ctx)
     assert e.value.status == -4
+
+
+@pytest.mark.parametrize("k", range(9))
+def test_variants_vs_ray_model(ctx, torch_dev, k):
+    # The product against the independent ray-plane model (tests/test_oracle_gbp_geometry.py) on
+    # random per-projection geometries: tilted and shifted detectors, shifted sources, volume
+    # rotations and axis tilts, parallel and cone beam.
+    import test_oracle_gbp_geometry as G
+    rng = np.random.default_rng(300 + k)
+    p = 6
+    projs = _smooth_projs(300 + k, p, 64, 64) + np.float32(1.0)
+    geo = G._random_geometry(rng, p, *G.VARIANTS[k])
+    got = _gbp(ctx, torch_dev, projs, overall_angle=2 * math.pi, **G.MODEL_GRID, **geo)
+    ref = G.ray_model_volume(projs, geo, 2 * math.pi, **G.MODEL_GRID)
+    assert got.shape == ref.shape
+    assert rel_l2(got, ref) < 1e-5
