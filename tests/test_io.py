@@ -356,3 +356,44 @@ def test_task_modes_and_properties():
         r.props.step = 0
     with pytest.raises(Exception):
         r.props.type = "hdf5"
+
+
+def _bigtiff(path, strip_count, strip_value, width=4, height=4):
+    """A little-endian BigTIFF of one 8-bit page with a hand-set StripOffsets entry."""
+    import struct
+    entries = [(256, 3, 1, width), (257, 3, 1, height), (258, 3, 1, 8), (278, 3, 1, 1),
+               (273, 16, strip_count, strip_value)]
+    ifd = struct.pack("<Q", len(entries)) + b"".join(struct.pack("<HHQQ", *e) for e in entries) + struct.pack("<Q", 0)
+    with open(path, "wb") as f:
+        f.write(b"II" + struct.pack("<HHHQ", 43, 8, 0, 16) + ifd + bytes(width * height))
+
+
+@pytest.mark.parametrize("count,value", [
+    (1 << 61, 0x4141414141414141),  # count x 8 bytes wraps to 0: must not be taken as inline
+    (1 << 61, 0xFFFFFFFFFFFFFFF0),  # offset + index x 8 overflows
+    (4, 0x7FFFFFFFFFFFFFF9),        # offset past any file
+])
+def test_bigtiff_crafted_strip_offsets_rejected(tmp_path, count, value):
+    _bigtiff(tmp_path / "evil.tif", count, value)
+    r = _read(tmp_path / "evil.tif")
+    with pytest.raises((U.TaskError, RuntimeError)) as e:  # setup-time or read-time refusal
+        r.get_requisition()
+        list(r.items())
+    assert "TIFF" in str(e.value)
+
+
+def test_bigtiff_hand_made_reads(tmp_path):
+    # the same writer with valid strips (4 rows of 4 bytes after the 16 + 8 + 5 x 20 + 8 byte header and IFD)
+    _bigtiff(tmp_path / "ok.tif", 4, 0)
+    import struct
+    data = bytearray(open(tmp_path / "ok.tif", "rb").read())
+    pixels = 16 + 8 + 5 * 20 + 8
+    strips = len(data)
+    data += struct.pack("<4Q", *(pixels + 4 * i for i in range(4)))
+    data[16 + 8 + 4 * 20 + 12: 16 + 8 + 4 * 20 + 20] = struct.pack("<Q", strips)
+    data[pixels: pixels + 16] = bytes(range(16))
+    open(tmp_path / "ok.tif", "wb").write(bytes(data))
+    r = _read(tmp_path / "ok.tif")
+    r.get_requisition()
+    (img,) = list(r.items())
+    np.testing.assert_array_equal(img, np.arange(16, dtype=np.float32).reshape(4, 4))

@@ -9,6 +9,7 @@
 #include "io.h"
 
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdlib.h>
@@ -78,10 +79,16 @@ static int tiff_value(FILE *fp, int be, int big, const unsigned char *e, uint64_
     if (!size || index >= count)
         return -1;
     unsigned char buf[8];
-    if (count * size <= inline_cap) {
+    /* count comes from the file: compare by division so that no product can wrap */
+    if (count <= inline_cap / size) {
         memcpy(buf, val + index * size, size);
     } else {
-        const uint64_t at = rdn(val, big ? 8 : 4, be) + index * size;
+        const uint64_t base = rdn(val, big ? 8 : 4, be);
+        if (index > (UINT64_MAX - base) / size)
+            return -1;
+        const uint64_t at = base + index * size;
+        if (at > (uint64_t) LONG_MAX - size)
+            return -1;
         if (fseek(fp, (long) at, SEEK_SET) || fread(buf, 1, size, fp) != size)
             return -1;
     }
@@ -174,6 +181,8 @@ static int tiff_open(ImgFile *f, char *err, size_t errlen)
     }
     size_t cap = 64;
     f->ifd = malloc(cap * sizeof(long));
+    if (!f->ifd)
+        return fail(err, errlen, "out of host memory");
     const uint64_t fsz = (uint64_t) file_size(f->fp);
     const int es = f->big ? 20 : 12, cn = f->big ? 8 : 2, on = f->big ? 8 : 4;
     while (ofs > 0 && ofs < fsz) {
@@ -186,8 +195,11 @@ static int tiff_open(ImgFile *f, char *err, size_t errlen)
                 break;
         }
         if (f->n_images == cap) {
+            long *grown = realloc(f->ifd, 2 * cap * sizeof(long));
+            if (!grown)
+                return fail(err, errlen, "out of host memory");
+            f->ifd = grown;
             cap *= 2;
-            f->ifd = realloc(f->ifd, cap * sizeof(long));
         }
         f->ifd[f->n_images++] = (long) ofs;
         unsigned char bc[8], bo[8];
@@ -233,9 +245,16 @@ static int tiff_load_page(ImgFile *f, unsigned page, char *err, size_t errlen)
     if (count < need)
         return fail(err, errlen, "TIFF page %u has %llu strips, needs %llu", page, (unsigned long long) count,
                     (unsigned long long) need);
+    if (need > (uint64_t) file_size(f->fp))
+        return fail(err, errlen, "TIFF page %u has more strips than the file has bytes", page);
     free(f->strips);
+    f->cached_page = -1;  /* the strips of the page cached before are gone */
     f->n_strips = (unsigned) need;
     f->strips = malloc(f->n_strips * sizeof(long));
+    if (!f->strips) {
+        f->n_strips = 0;
+        return fail(err, errlen, "out of host memory");
+    }
     for (unsigned i = 0; i < f->n_strips; i++) {
         uint64_t v;
         if (tiff_value(f->fp, f->swap, f->big, e, i, &v))

@@ -289,6 +289,23 @@ class TaskGraph:
 
 
 _TORCH_DT = {0: "float32", 1: "float16", 2: "bfloat16"}
+_DT_CODE = {v: k for k, v in _TORCH_DT.items()}
+
+
+def _device_tensor(tensor, device, what, dtypes=("float32",)):
+    """The checks every tensor passes before its data pointer goes to the C layer: on the
+    context's device, one of `dtypes`, contiguous.  Returns the dtype code of the buffer."""
+    import torch
+    if not isinstance(tensor, torch.Tensor):
+        raise TypeError("%s must be a torch tensor, not %s" % (what, type(tensor).__name__))
+    if not tensor.is_cuda or tensor.device.index != device:
+        raise ValueError("%s must be on cuda:%d (it is on %s)" % (what, device, tensor.device))
+    name = str(tensor.dtype).replace("torch.", "")
+    if name not in dtypes:
+        raise ValueError("%s must be %s (it is %s)" % (what, " or ".join(dtypes), name))
+    if not tensor.is_contiguous():
+        raise ValueError("%s must be contiguous" % what)
+    return _DT_CODE[name]
 
 
 def _req_shape(req):
@@ -315,12 +332,12 @@ class Scheduler:
 
         def make_buf(tensor, layout):
             b = Buffer()
+            b.dtype = _device_tensor(tensor, self.ctx.device, "a task's buffer", tuple(_TORCH_DT.values()))
             b.data = tensor.data_ptr()
             b.req.n_dims = tensor.dim()
             for i, s in enumerate(reversed(tensor.shape)):
                 b.req.dims[i] = s
             b.layout = layout
-            b.dtype = 0
             return b
 
         def push(stage, tensor, layout):
@@ -358,7 +375,7 @@ class Scheduler:
         reducer_in = {}
         dev = torch.device("cuda:%d" % self.ctx.device)
         for item in (src.items() if isinstance(src, Task) else src.array):
-            x = torch.from_numpy(item).to(dev)
+            x = torch.from_numpy(np.ascontiguousarray(item, dtype=np.float32)).to(dev)
             buf = make_buf(x, LAYOUT_REAL)
             for k, t in enumerate(tasks):
                 if t.mode & MODE_REDUCTOR:
@@ -396,14 +413,18 @@ class Pipeline:
     def used_fused_filter(self):
         return bool(self.L.ufomi_pipeline_used_fused_filter(self.h))
 
-    def run(self, ctx, items=None, out_capacity_items=None, out_bytes=None):
-        """items: torch tensor [n][h][w] float32 on the device (None when the first
-        stage is `read`); returns (tensor of bytes or None when the last stage is
-        `write`, n_out, shape)."""
+    def run(self, ctx, items=None, out_bytes=None):
+        """items: torch tensor [n][h][w] float32, contiguous, on the context's device (None
+        when the first stage is `read`); returns (tensor of bytes or None when the last stage
+        is `write`, n_out, shape).  out_bytes sizes the output buffer (default: 8x the input,
+        at most half the free device memory)."""
         import torch
         req = Requisition()
         n, ptr, reqp = 0, None, None
         if items is not None:
+            _device_tensor(items, ctx.device, "items")
+            if items.dim() != 3:
+                raise ValueError("items must be [n][h][w] (it has %d dimensions)" % items.dim())
             n, h, w = items.shape
             req.n_dims = 2
             req.dims[0], req.dims[1], req.dims[2] = w, h, 1

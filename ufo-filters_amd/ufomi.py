@@ -184,12 +184,25 @@ def check(rc):
     return rc
 
 
-def _dptr(a):
-    return ctypes.cast(np.ascontiguousarray(a, dtype=np.float64).ctypes.data, ctypes.POINTER(ctypes.c_double))
+class _dptr:
+    """A float64 array argument: the converted copy lives as long as this object, which ctypes
+    keeps for the duration of the call it is passed to."""
+
+    def __init__(self, a):
+        self.a = np.ascontiguousarray(a, dtype=np.float64)
+        self._as_parameter_ = self.a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    """Device pointer of a torch tensor; refuses host tensors and non-contiguous views, whose
+    pointer would make the kernels read host memory or the wrong elements."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor, got one on %s" % t.device)
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor (got strides %s for shape %s)" % (t.stride(), tuple(t.shape)))
+    return ctypes.c_void_p(t.data_ptr())
 
 
 # ------------------------------------------------------------------ host-only
@@ -485,8 +498,8 @@ class FbpBatch:
             sino, out = r["sinogram"], r["slice"]
             p, w = sino.shape
             props = {k: v for k, v in r.items() if k not in ("sinogram", "slice", "fft_size")}
-            arr[i].sinogram = sino.data_ptr()
-            arr[i].slice = out.data_ptr()
+            arr[i].sinogram = _ptr(sino).value
+            arr[i].slice = _ptr(out).value
             arr[i].width = w
             arr[i].num_angles = p
             arr[i].fft_size = r.get("fft_size", 0)
