@@ -389,7 +389,14 @@ typedef struct {
     int store_type;              /* ufomi_store_type, default float */
     int addressing_mode;         /* default clamp */
     double gray_map_min, gray_map_max;
+    int compute_type;            /* UFOMI_CT_FLOAT (default) or UFOMI_CT_DOUBLE */
+    int result_type;             /* UFOMI_RT_HALF, UFOMI_RT_FLOAT (default) or UFOMI_RT_DOUBLE */
+    unsigned burst;              /* projections per running sum (0 = 8); matters for result-type half */
 } ufomi_gbp_params;
+
+/* compute-type and result-type nicks of ufo-general-backproject-task.c:367-378 */
+enum { UFOMI_CT_FLOAT = 0, UFOMI_CT_DOUBLE = 1 };
+enum { UFOMI_RT_HALF = 0, UFOMI_RT_FLOAT = 1, UFOMI_RT_DOUBLE = 2 };
 
 void ufomi_gbp_params_init(ufomi_gbp_params *p);
 void ufomi_gbp_geometry_init(ufomi_gbp_geometry *g);
@@ -415,6 +422,18 @@ int ufomi_gbp_volume_shape(const ufomi_gbp_params *p, int W, int *nx, int *ny, i
  * is composed in FP64 into a projective 4x4 map on the host; the kernel
  * accumulates all P projections per voxel in registers (no per-burst volume
  * read-modify-write).
+ * compute-type double, result-type half or double (:367-378, 1075-1150) and the
+ * addressing modes repeat / mirrored_repeat run a second kernel:
+ *   - compute-type double evaluates the projective maps, the cone weight and
+ *     the bilinear weights in FP64 (the reference's double kernel rounds the
+ *     sample coordinate to float and interpolates in float);
+ *   - the running sum of every burst of params->burst projections (0 = 8) is
+ *     rounded to the result type after each projection, and norm * sum is
+ *     added to the volume in the store type once per burst, as the reference's
+ *     burst launches do (integer stores map the whole sum once);
+ *   - repeat / mirrored_repeat apply OpenCL's normalised-coordinate rules
+ *     (OpenCL 1.2, section 8.2) to u / W and v / H; the reference's sampler is
+ *     unnormalised, where OpenCL leaves these modes undefined.
  */
 int ufomi_gbp(ufomi_ctx *ctx, const float *d_projections, int W, int H, int P,
               const ufomi_gbp_geometry *geometry, const ufomi_gbp_params *params, void *d_volume);
@@ -425,7 +444,10 @@ int ufomi_gbp(ufomi_ctx *ctx, const float *d_projections, int W, int H, int P,
  * sweeps that parameter over region (from, to, step) while voxels lie at
  * z = params->z; slice k replaces the parameter by region[0] + k region[2]
  * for every projection (make_parameter_assignment, :755-775).  parameter z
- * is ufomi_gbp.  axis-angle-z holds the tomographic angles and is rejected.
+ * is ufomi_gbp.  axis-angle-z follows the reference's generated kernel,
+ * which never reads the swept value: without per-projection geometry arrays
+ * that kernel does not build (UFOMI_ERR_UNSUPPORTED here); with them every
+ * slice is the reconstruction at z.
  */
 enum { UFOMI_GBP_PARAMETER_Z = 17 };
 int ufomi_gbp_sweep(ufomi_ctx *ctx, const float *d_projections, int W, int H, int P,

@@ -15,7 +15,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 FILTERS = {"ramp": 0, "ramp-fromreal": 1, "butterworth": 2, "faris-byer": 3, "hamming": 4, "bh3": 5,
            "shepp-logan": 6}
-ADDR = {"clamp": 0, "clamp_to_edge": 1}
+ADDR = {"clamp": 0, "clamp_to_edge": 1, "repeat": 2, "mirrored_repeat": 3}
+RESULT_TYPES = {"half": 0, "float": 1, "double": 2}
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -68,7 +69,9 @@ def lib():
         L.orc_gbp.argtypes = [_f32p, c_int, c_int, c_int, ctypes.POINTER(OrcGbpGeom),
                               ctypes.POINTER(OrcGbpFlags), c_int, c_int, c_int, c_int,
                               c_double, c_double, c_double, c_double, c_double, c_double,
-                              c_double, c_int, c_int, _f64p]
+                              c_double, c_int, c_int, c_int, _f64p]
+        L.orc_round_half.argtypes = [c_double]
+        L.orc_round_half.restype = c_double
         L.orc_lamino.argtypes = [_f32p, c_int, c_int, c_int, c_uint, c_float, c_float, c_float, c_float,
                                  c_float, c_float, c_int, c_int, _i32p, _i32p, _f32p, c_float, c_int, c_int,
                                  c_int, c_int, c_int, c_int, c_int, _f64p]
@@ -371,10 +374,16 @@ def _almost_zero(vals):
     return all(abs(v) < 1e-7 for v in vals)
 
 
+def round_half(x):
+    """Nearest IEEE binary16 value of x (ties to even), as a Python float."""
+    return lib().orc_round_half(float(x))
+
+
 def gbp(projs, num_projections=None, overall_angle=2 * math.pi, region=(0.0, 0.0, 0.0),
         x_region=(0.0, 0.0, 0.0), y_region=(0.0, 0.0, 0.0), addressing_mode="clamp", mode="f64",
-        burst=8, **geometry):
-    """general-backproject, parameter z, in FP64 (mode "f64") or float32 semantics ("ref32").
+        burst=8, result_type="float", **geometry):
+    """general-backproject, parameter z, in FP64 (mode "f64") or float32 semantics ("ref32",
+    with the per-burst running sum in result_type: half, float or double).
 
     projs: [P][H][W] float32.  Geometry keyword arguments use the property names with
     '-' replaced by '_' (e.g. source_position_y=[-2500.0]); one value broadcasts.
@@ -430,7 +439,7 @@ def gbp(projs, num_projections=None, overall_angle=2 * math.pi, region=(0.0, 0.0
     out = np.empty((nz, ny, nx), dtype=np.float64)
     lib().orc_gbp(projs, w, h, p, geoms, ctypes.byref(fl), ADDR[addressing_mode], nx, ny, nz,
                   x0, xs, y0, ys, float(zstart), float(zstep), float(overall_angle),
-                  0 if mode == "f64" else 1, int(burst), out)
+                  0 if mode == "f64" else 1, int(burst), RESULT_TYPES[result_type], out)
     return out
 
 

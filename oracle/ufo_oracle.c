@@ -507,14 +507,93 @@ static inline v3f rx_f(float s, float c, v3f p) { v3f r = { p.x, fmaf (c, p.y, -
 static inline v3f ry_f(float s, float c, v3f p) { v3f r = { fmaf (c, p.x, s * p.z), p.y, fmaf (-s, p.x, c * p.z) }; return r; }
 static inline v3f rz_f(float s, float c, v3f p) { v3f r = { fmaf (c, p.x, -(s * p.y)), fmaf (s, p.x, c * p.y), p.z }; return r; }
 
-/* Bilinear read_imagef, unnormalised coordinates, addressing clamp (0 border) or clamp_to_edge. */
-enum { ORC_ADDR_CLAMP = 0, ORC_ADDR_CLAMP_TO_EDGE = 1 };
+/*
+ * Round to the nearest IEEE binary16 value (ties to even), returned as a
+ * double: 11 significant bits for normal halves, a quantum of 2^-24 below 2^-14,
+ * infinity from 65520 on.  Pinned against numpy's float16 conversion.
+ */
+double orc_round_half(double x)
+{
+    if (!isfinite (x) || x == 0.0)
+        return x;
+    const double a = fabs (x);
+    if (a >= 65520.0)
+        return copysign (INFINITY, x);
+    int e;
+    frexp (a, &e);  /* a = m 2^e, m in [0.5, 1) */
+    const double q = e - 1 >= -14 ? ldexp (1.0, e - 11) : ldexp (1.0, -24);
+    return copysign (nearbyint (a / q) * q, x);
+}
+
+/* Bilinear read_imagef, unnormalised coordinates, addressing clamp (0 border),
+ * clamp_to_edge, repeat or mirrored_repeat.  OpenCL defines the two repeat
+ * modes for normalised coordinates only (OpenCL 1.2 spec, section 8.2); the
+ * reference's sampler is unnormalised (ufo-general-backproject-task.c:1450),
+ * so here they apply the spec's normalised formulas to s = x / w, t = y / h. */
+enum { ORC_ADDR_CLAMP = 0, ORC_ADDR_CLAMP_TO_EDGE = 1, ORC_ADDR_REPEAT = 2, ORC_ADDR_MIRRORED_REPEAT = 3 };
+
+/* texel indices and weight of one axis for the repeat modes, FP64 */
+static inline void repeat_axis_d(double x, int w, int addr, long *i0, long *i1, double *a)
+{
+    double s = x / w, u;
+    if (addr == ORC_ADDR_REPEAT) {
+        u = (s - floor (s)) * w;
+    } else {
+        double sp = 2.0 * nearbyint (0.5 * s);
+        u = fabs (s - sp) * w;
+    }
+    const double f = floor (u - 0.5);
+    *i0 = (long) f;
+    *i1 = *i0 + 1;
+    *a = (u - 0.5) - f;
+    if (addr == ORC_ADDR_REPEAT) {
+        if (*i0 < 0) *i0 += w;
+        if (*i1 > w - 1) *i1 -= w;
+    } else {
+        if (*i0 < 0) *i0 = 0;
+        if (*i1 > w - 1) *i1 = w - 1;
+    }
+}
+
+static inline void repeat_axis_f(float x, int w, int addr, int *i0, int *i1, float *a)
+{
+    float s = x / (float) w, u;
+    if (addr == ORC_ADDR_REPEAT) {
+        u = (s - floorf (s)) * (float) w;
+    } else {
+        float sp = 2.0f * rintf (0.5f * s);
+        u = fabsf (s - sp) * (float) w;
+    }
+    const float f = floorf (u - 0.5f);
+    *i0 = (int) f;
+    *i1 = *i0 + 1;
+    *a = (u - 0.5f) - f;
+    if (addr == ORC_ADDR_REPEAT) {
+        if (*i0 < 0) *i0 += w;
+        if (*i1 > w - 1) *i1 -= w;
+    } else {
+        if (*i0 < 0) *i0 = 0;
+        if (*i1 > w - 1) *i1 = w - 1;
+    }
+}
 
 static inline double tex2_d(const float *img, int w, int h, double x, double y, int addr)
 {
+    if (!isfinite (x) || !isfinite (y))
+        return 0.0;
+    if (addr == ORC_ADDR_REPEAT || addr == ORC_ADDR_MIRRORED_REPEAT) {
+        long i0, i1, j0, j1;
+        double a, b;
+        repeat_axis_d (x, w, addr, &i0, &i1, &a);
+        repeat_axis_d (y, h, addr, &j0, &j1, &b);
+        return (1 - a) * (1 - b) * img[j0 * w + i0] + a * (1 - b) * img[j0 * w + i1] +
+               (1 - a) * b * img[j1 * w + i0] + a * b * img[j1 * w + i1];
+    }
     double u = x - 0.5, v = y - 0.5;
     double fu = floor (u), fv = floor (v);
     double a = u - fu, b = v - fv;
+    fu = fmax (fmin (fu, 1e9), -1e9);  /* far outside stays far outside, on its side */
+    fv = fmax (fmin (fv, 1e9), -1e9);
     long i0 = (long) fu, j0 = (long) fv;
     double t[4];
     for (int q = 0; q < 4; q++) {
@@ -532,9 +611,24 @@ static inline double tex2_d(const float *img, int w, int h, double x, double y, 
 
 static inline float tex2_f(const float *img, int w, int h, float x, float y, int addr)
 {
+    if (!isfinite (x) || !isfinite (y))
+        return 0.0f;
+    if (addr == ORC_ADDR_REPEAT || addr == ORC_ADDR_MIRRORED_REPEAT) {
+        int i0, i1, j0, j1;
+        float a, b;
+        repeat_axis_f (x, w, addr, &i0, &i1, &a);
+        repeat_axis_f (y, h, addr, &j0, &j1, &b);
+        float r0 = (1 - a) * (1 - b) * img[(size_t) j0 * w + i0];
+        float r1 = a * (1 - b) * img[(size_t) j0 * w + i1];
+        float r2 = (1 - a) * b * img[(size_t) j1 * w + i0];
+        float r3 = a * b * img[(size_t) j1 * w + i1];
+        return ((r0 + r1) + r2) + r3;
+    }
     float u = x - 0.5f, v = y - 0.5f;
     float fu = floorf (u), fv = floorf (v);
     float a = u - fu, b = v - fv;
+    fu = fmaxf (fminf (fu, 1e9f), -1e9f);
+    fv = fmaxf (fminf (fv, 1e9f), -1e9f);
     int i0 = (int) fu, j0 = (int) fv;
     float t[4];
     for (int q = 0; q < 4; q++) {
@@ -656,12 +750,17 @@ static double gbp_voxel_f64(const float *projs, int w, int h, int P, const orc_g
 
 /* Same voxel in float32 with the template's operation order and the burst-wise
  * accumulation of ufo_general_backproject_task_process (:1610-1706):
- * volume = norm*sum(burst 0); volume += norm*sum(burst k). */
+ * volume = norm*sum(burst 0); volume += norm*sum(burst k).
+ * result_type (general_bp_body.in:7, 21-30 with rtype): 0 half -- the running
+ * sum `half result += float` is the float sum rounded to half after every
+ * projection; 1 float; 2 double -- norm * result then in double, cast to the
+ * float volume. */
 static float gbp_voxel_ref32(const float *projs, int w, int h, int P, const orc_gbp_geom *g,
                              const orc_gbp_flags *fl, int addr, float x, float y, float z,
-                             float norm, int burst)
+                             float norm, int burst, int result_type)
 {
-    float volume = 0.0f, result = 0.0f;
+    float volume = 0.0f;
+    double result = 0.0;
     for (int i = 0; i < P; i++) {
         const orc_gbp_geom *q = g + i;
         const float sx = (float) q->sx, sy = (float) q->sy, sz = (float) q->sz;
@@ -721,13 +820,20 @@ static float gbp_voxel_ref32(const float *projs, int w, int h, int P, const orc_
             }
         }
         float val = tex2_f (projs + (size_t) i * w * h, w, h, v.x + (float) q->cx, v.z + (float) q->cz, addr);
-        result += fl->parallel_beam ? val : val * coeff * coeff;
+        const float contrib = fl->parallel_beam ? val : val * coeff * coeff;
+        if (result_type == 0)
+            result = orc_round_half ((double) ((float) result + contrib));
+        else if (result_type == 1)
+            result = (double) ((float) result + contrib);
+        else
+            result += (double) contrib;
         if ((i + 1) % burst == 0 || i + 1 == P) {
+            const float part = result_type == 2 ? (float) ((double) norm * result) : norm * (float) result;
             if (i + 1 <= burst)
-                volume = norm * result;
+                volume = part;
             else
-                volume += norm * result;
-            result = 0.0f;
+                volume += part;
+            result = 0.0;
         }
     }
     return volume;
@@ -737,11 +843,12 @@ static float gbp_voxel_ref32(const float *projs, int w, int h, int P, const orc_
  * Volume [nz][ny][nx] for the z parameter: voxel_0 = (x0 + ix*dx, y0 + iy*dy,
  * z_k) with z_k = region_start + k * region_step (create_regions, :38-76) and
  * x0/dx, y0/dy from compute_slice_region (:246-259).  norm = |overall| / P
- * (set_static_args, :292).  mode 0: FP64 result * norm; mode 1: ref32.
+ * (set_static_args, :292).  mode 0: FP64 result * norm; mode 1: ref32 with the
+ * running sum in result_type (0 half, 1 float, 2 double).
  */
 void orc_gbp(const float *projs, int w, int h, int P, const orc_gbp_geom *g, const orc_gbp_flags *fl,
              int addr, int nx, int ny, int nz, double x0, double xs, double y0, double ys,
-             double z0, double zs, double overall_angle, int mode, int burst, double *out)
+             double z0, double zs, double overall_angle, int mode, int burst, int result_type, double *out)
 {
     const double norm = fabs (overall_angle) / P;
     #pragma omp parallel for collapse(2) schedule(dynamic, 4)
@@ -756,7 +863,8 @@ void orc_gbp(const float *projs, int w, int h, int P, const orc_gbp_geom *g, con
                     float x = fmaf ((float) ix, (float) xs, (float) x0);
                     float y = fmaf ((float) iy, (float) ys, (float) y0);
                     float z = (float) (z0 + iz * zs);
-                    out[o] = gbp_voxel_ref32 (projs, w, h, P, g, fl, addr, x, y, z, (float) norm, burst);
+                    out[o] = gbp_voxel_ref32 (projs, w, h, P, g, fl, addr, x, y, z, (float) norm, burst,
+                                              result_type);
                 }
             }
         }

@@ -28,13 +28,14 @@ TORCH_STORE = {"half": "float16", "float": "float32", "double": "float64", "ucha
 
 def _gbp(ctx, torch_dev, projs, store_type="float", addressing_mode="clamp", overall_angle=2 * math.pi,
          region=(0, 0, 0), x_region=(0, 0, 0), y_region=(0, 0, 0), gray_map_min=0.0, gray_map_max=0.0,
-         num_projections=None, **geometry):
+         num_projections=None, compute_type="float", result_type="float", burst=0, **geometry):
     import torch
     p, h, w = projs.shape
     n = num_projections if num_projections is not None else p
     params = ufomi.gbp_params(n, overall_angle=overall_angle, region=region, x_region=x_region,
                               y_region=y_region, store_type=store_type, addressing_mode=addressing_mode,
-                              gray_map_min=gray_map_min, gray_map_max=gray_map_max)
+                              gray_map_min=gray_map_min, gray_map_max=gray_map_max, compute_type=compute_type,
+                              result_type=result_type, burst=burst)
     geom = ufomi.gbp_geometry(**geometry)
     shape = ufomi.gbp_volume_shape(params, w)
     vol = torch.zeros(shape, dtype=getattr(torch, TORCH_STORE[store_type]), device=torch_dev)
@@ -176,8 +177,8 @@ def test_errors(ctx, torch_dev):
         _gbp(ctx, torch_dev, projs, store_type="uchar", gray_map_min=1.0, gray_map_max=1.0)
     assert e.value.code == -8
     with pytest.raises(ufomi.UfomiError) as e:
-        _gbp(ctx, torch_dev, projs, addressing_mode="repeat")
-    assert e.value.code == -4
+        _gbp(ctx, torch_dev, projs, burst=129)
+    assert e.value.code == -1
 
 
 def test_cone_weight(ctx, torch_dev):
@@ -299,7 +300,7 @@ def test_parameter_sweep(ctx, torch_dev, parameter, region, extra):
         assert rel_l2(got[k], ref[0]) < 1e-5, (k, rel_l2(got[k], ref[0]))
 
 
-def test_sweep_task_and_tomo_angle_rejected(ctx, torch_dev):
+def test_sweep_task(ctx, torch_dev):
     import ufo_tasks
     projs = _smooth_projs(22, 16, 12, 16)
     t = ufo_tasks.Task("general-backproject")
@@ -317,37 +318,99 @@ def test_sweep_task_and_tomo_angle_rejected(ctx, torch_dev):
         ref = oracle.gbp(projs, region=(0.5, 1.5, 1.0), source_position_y=[-40.0], detector_position_y=[30.0],
                          center_position_x=[7.0 + 0.5 * k], center_position_z=[6.0])
         assert rel_l2(got[k], ref[0]) < 1e-5
-    bad = ufo_tasks.Task("general-backproject")
-    bad.props.num_projections = 16
-    bad.props.parameter = "axis-angle-z"
-    with pytest.raises(ufo_tasks.TaskError) as e:
-        bad.setup(ctx)
-    assert e.value.status == -4
 
-
-def test_double_compute_and_result_types(ctx, torch_dev):
-    """compute-type=double and result-type=double (FP64 coordinates and running sums in the reference)
-    meet the FP64 oracle within 1e-5; result-type=half is refused."""
+def _run_task(ctx, projs, **props):
     import ufo_tasks
-    projs = _smooth_projs(5, 18, 16, 24)
     t = ufo_tasks.Task("general-backproject")
-    for k, v in dict(num_projections=18, compute_type="double", result_type="double", region=[-2.0, 2.0, 1.0],
-                     axis_angle_x=[0.4], center_position_x=[12.0], center_position_z=[8.0]).items():
+    for k, v in props.items():
         setattr(t.props, k, v)
     src, sink = ufo_tasks.MemoryIn(projs), ufo_tasks.MemoryOut()
     g = ufo_tasks.TaskGraph()
     g.connect_nodes(src, t)
     g.connect_nodes(t, sink)
     ufo_tasks.Scheduler(ctx).run(g)
+    return sink.array
+
+
+def test_axis_angle_z_parameter(ctx, torch_dev):
+    # The reference's generated kernel never reads the swept axis-angle-z: without per-projection
+    # geometry it does not build; with it every slice is the reconstruction at z.
+    import ufo_tasks
+    projs = _smooth_projs(23, 16, 12, 16)
+    base = dict(num_projections=16, parameter="axis-angle-z", region=[0.0, 1.5, 0.5], z=0.5,
+                center_position_z=[6.0])
+    with pytest.raises(ufo_tasks.TaskError) as e:
+        _run_task(ctx, projs, center_position_x=[8.0], **base)
+    assert e.value.status == -4 and "Error creating backprojection kernels" in str(e.value)
+    cx = list(8.0 + 0.1 * np.arange(16))
+    got = _run_task(ctx, projs, center_position_x=cx, **base)
+    assert got.shape == (3, 16, 16)
+    ref = oracle.gbp(projs, region=(0.5, 1.5, 1.0), center_position_x=cx, center_position_z=[6.0])
+    for k in range(3):
+        assert rel_l2(got[k], ref[0]) < 1e-5
+
+
+def test_double_compute_and_result_types(ctx, torch_dev):
+    """compute-type=double and result-type=double through the task: FP64 maps, FP64 bilinear
+    weights and FP64 running sums meet the FP64 oracle to FP64 rounding with a double volume."""
+    projs = _smooth_projs(5, 18, 16, 24)
+    geo = dict(region=[-2.0, 2.0, 1.0], axis_angle_x=[0.4], center_position_x=[12.0], center_position_z=[8.0])
     ref = oracle.gbp(projs, region=(-2.0, 2.0, 1.0), axis_angle_x=[0.4], center_position_x=[12.0],
                      center_position_z=[8.0])
-    assert rel_l2(sink.array, ref) < 1e-5
-    half = ufo_tasks.Task("general-backproject")
-    half.props.num_projections = 18
-    half.props.result_type = "half"
-    with pytest.raises(ufo_tasks.TaskError) as e:
-        half.setup(ctx)
-    assert e.value.status == -4
+    got = _run_task(ctx, projs, num_projections=18, compute_type="double", result_type="double", **geo)
+    assert rel_l2(got, ref) < 1e-6  # float volume (the task layer hands out float32 items)
+    got64 = _gbp(ctx, torch_dev, projs, compute_type="double", result_type="double", store_type="double",
+                 region=(-2.0, 2.0, 1.0), axis_angle_x=[0.4], center_position_x=[12.0], center_position_z=[8.0])
+    assert got64.dtype == np.float64
+    assert rel_l2(got64, ref) < 1e-11
+    got32 = _gbp(ctx, torch_dev, projs, store_type="double", region=(-2.0, 2.0, 1.0), axis_angle_x=[0.4],
+                 center_position_x=[12.0], center_position_z=[8.0])
+    assert 1e-10 < rel_l2(got32, ref) < 1e-5  # the float kernel stays at float rounding
+
+
+@pytest.mark.parametrize("k", [0, 3, 5, 7])
+def test_double_compute_on_variants(ctx, torch_dev, k):
+    # FP64 compute on the tilted / shifted / rotated / cone variants against the FP64 oracle
+    import test_oracle_gbp_geometry as G
+    rng = np.random.default_rng(400 + k)
+    p = 6
+    projs = _smooth_projs(400 + k, p, 64, 64) + np.float32(1.0)
+    geo = G._random_geometry(rng, p, *G.VARIANTS[k])
+    got = _gbp(ctx, torch_dev, projs, compute_type="double", result_type="double", store_type="double",
+               **G.MODEL_GRID, **geo)
+    ref = oracle.gbp(projs, **G.MODEL_GRID, **geo)
+    assert rel_l2(got, ref) < 1e-11
+
+
+@pytest.mark.parametrize("burst", [0, 3, 16])
+@pytest.mark.parametrize("cone", [False, True])
+def test_result_type_half(ctx, torch_dev, burst, cone):
+    """result-type half: the running sum of every burst rounded to binary16 after each projection.
+    Against the oracle's float32 semantics with the same rounding (a rounding flip from a float32
+    difference moves one voxel by a half ulp), and visibly away from the FP64 result."""
+    projs = _smooth_projs(31, 24, 20, 24) + np.float32(1.0)
+    geo = dict(center_position_x=[12.0], center_position_z=[10.0], region=(-3.0, 3.0, 1.0))
+    if cone:
+        geo.update(source_position_y=[-60.0], detector_position_y=[60.0])
+    got = _gbp(ctx, torch_dev, projs, result_type="half", burst=burst, **geo)
+    emu = oracle.gbp(projs, mode="ref32", result_type="half", burst=burst or 8, **geo)
+    f64 = oracle.gbp(projs, **geo)
+    err_emu, err64 = rel_l2(got, emu), rel_l2(got, f64)
+    print("result-type half burst %d cone %s: vs emulation %.3g, vs FP64 %.3g" % (burst, cone, err_emu, err64))
+    assert err_emu < 2e-4
+    assert err64 > 3 * err_emu
+
+
+@pytest.mark.parametrize("addressing_mode", ["repeat", "mirrored_repeat"])
+@pytest.mark.parametrize("cone", [False, True])
+def test_repeat_addressing(ctx, torch_dev, addressing_mode, cone):
+    # a volume wider than the detector, so samples wrap (repeat) or mirror
+    projs = _smooth_projs(41, 12, 16, 20) + np.float32(1.0)
+    geo = dict(center_position_x=[10.0], center_position_z=[8.0], region=(-12.0, 12.0, 4.0),
+               x_region=(-24.0, 24.0, 1.5), y_region=(-24.0, 24.0, 1.5), addressing_mode=addressing_mode)
+    if cone:
+        geo.update(source_position_y=[-80.0], detector_position_y=[40.0])
+    _check(ctx, torch_dev, projs, **geo)
 
 
 @pytest.mark.parametrize("k", range(9))

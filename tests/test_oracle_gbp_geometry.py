@@ -380,3 +380,79 @@ def test_planted_geometry_mistakes_are_caught(bit):
     # and with the plant cleared, the oracle is back to the faithful restatement
     got, ref = _model_vs_oracle(100 + 3, VARIANTS[3])
     np.testing.assert_allclose(got, ref, atol=1e-9, rtol=0)
+
+
+# ------------------------------------------------------------ result type and addressing
+
+def test_round_half_vs_numpy_float16():
+    # numpy converts float64 to binary16 with its own round-to-nearest-even code
+    rng = np.random.default_rng(3)
+    x = np.concatenate([
+        rng.standard_normal(4000) * np.exp(rng.uniform(-20, 11, 4000)),   # normals and subnormals
+        (np.arange(-2048, 2048) + 0.5) * 2.0 ** -10,                       # ties at the 2^-10 quantum of [1, 2)
+        [65504.0, 65519.99, 65520.0, -65520.0, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26, 0.0, -0.0],
+    ])
+    got = np.array([oracle.round_half(v) for v in x])
+    with np.errstate(over="ignore"):  # 65520 and beyond become infinity, as they should
+        ref = x.astype(np.float16).astype(np.float64)
+    np.testing.assert_array_equal(got, ref)
+
+
+def _sample_line(values, xs, addressing_mode, mode="f64"):
+    """Parallel beam, one projection at theta = 0, one detector row: voxel x samples u = x + cx."""
+    w = len(values)
+    projs = np.asarray(values, np.float32)[None, None, :]
+    cx = 0.0
+    out = [oracle.gbp(projs, num_projections=1, overall_angle=1.0, axis_angle_z=[0.0], center_position_x=[cx],
+                      center_position_z=[0.5], x_region=(x, x + 1.0, 1.0), y_region=(0.0, 1.0, 1.0),
+                      addressing_mode=addressing_mode, mode=mode)[0, 0, 0] for x in xs]
+    return np.array(out)
+
+
+@pytest.mark.parametrize("mode", ["f64", "ref32"])
+def test_repeat_addressing_invariants(mode):
+    rng = np.random.default_rng(8)
+    w = 12
+    vals = rng.uniform(-1, 1, w)
+    inner = np.linspace(0.6, w - 0.6, 23)
+    # inside the image every mode is the plain linear interpolation
+    base = _sample_line(vals, inner, "clamp_to_edge", mode)
+    for m in ("repeat", "mirrored_repeat"):
+        np.testing.assert_allclose(_sample_line(vals, inner, m, mode), base, atol=1e-6)
+    # repeat: period w, and at the seam u = 0 half of the last and half of the first texel
+    xs = np.linspace(-30.3, 30.1, 41)
+    np.testing.assert_allclose(_sample_line(vals, xs + w, "repeat", mode), _sample_line(vals, xs, "repeat", mode),
+                               atol=2e-5)
+    np.testing.assert_allclose(_sample_line(vals, [0.0, float(w)], "repeat", mode),
+                               [(vals[-1] + vals[0]) / 2] * 2, atol=1e-6)
+    # mirrored repeat: even about 0 and about w, period 2 w, edge texels held at the mirror lines
+    np.testing.assert_allclose(_sample_line(vals, -xs, "mirrored_repeat", mode),
+                               _sample_line(vals, xs, "mirrored_repeat", mode), atol=2e-5)
+    np.testing.assert_allclose(_sample_line(vals, 2 * w - xs, "mirrored_repeat", mode),
+                               _sample_line(vals, xs, "mirrored_repeat", mode), atol=2e-5)
+    np.testing.assert_allclose(_sample_line(vals, [0.0, float(w)], "mirrored_repeat", mode), [vals[0], vals[-1]],
+                               atol=1e-6)
+
+
+@pytest.mark.parametrize("rt", ["half", "double"])
+def test_result_types_ref32(rt):
+    # half: the running sum is rounded to binary16 after every projection (a burst sum of
+    # values near 1 is then a multiple of 2^-10 .. 2^-7), so it differs from float by about
+    # the half rounding; double differs from float only by float rounding
+    rng = np.random.default_rng(12)
+    p = 20
+    projs = rng.uniform(0.5, 1.5, (p, 24, 24)).astype(np.float32)
+    geo = dict(center_position_x=[12.0], center_position_z=[12.0], region=(-2.0, 2.0, 2.0),
+               x_region=(-6.0, 6.0, 1.0), y_region=(-6.0, 6.0, 1.0))
+    f64 = oracle.gbp(projs, **geo)
+    f32 = oracle.gbp(projs, mode="ref32", **geo)
+    got = oracle.gbp(projs, mode="ref32", result_type=rt, **geo)
+    if rt == "double":
+        assert oracle.rel_l2(got, f64) < 1e-6
+    else:
+        err = oracle.rel_l2(got, f64)
+        assert 1e-5 < err < 3e-3, err
+        assert oracle.rel_l2(f32, f64) < 1e-6
+        # with bursts of one projection every burst sum is a single rounded sample
+        one = oracle.gbp(projs, mode="ref32", result_type=rt, burst=1, **geo)
+        assert oracle.rel_l2(one, f64) < err

@@ -531,6 +531,155 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     }
 }
 
+// ---------------------------------------------------------------------------
+// Precise kernel: compute-type double, result-type half or double, and the
+// repeat addressing modes.  One thread per voxel, global-memory gathers (these
+// are options, not the hot path):
+//   * CT = double keeps the composed FP64 rows and evaluates the projective map,
+//     the cone weight and the bilinear weights in FP64 (the reference's double
+//     kernel computes the chain in FP64 but rounds the sample coordinate to float
+//     and interpolates in float, so this is at least as accurate);
+//   * the running sum of each burst of `burst` projections is rounded to the
+//     result type after every projection (`rtype result += ...`,
+//     general_bp_body.in:7, 21) and norm * result is added to the volume in the
+//     store type once per burst (:26-30), as the reference's burst launches do;
+//     integer store types take the gray mapping of the whole sum once, as the
+//     LDS kernel does;
+//   * repeat and mirrored_repeat apply the OpenCL 1.2 normalised-coordinate
+//     rules (section 8.2) to s = u / W, t = v / H; the reference's sampler uses
+//     unnormalised coordinates, for which OpenCL leaves these modes undefined.
+// ---------------------------------------------------------------------------
+enum { RT_HALF = 0, RT_FLOAT = 1, RT_DOUBLE = 2 };
+
+// double -> binary16 with one rounding: truncate to float, mark an inexact
+// truncation in the last bit (round to odd), then round to nearest even.
+__device__ __forceinline__ float round_half(double d)
+{
+    float f = __double2float_rz(d);
+    if ((double) f != d)
+        f = __uint_as_float(__float_as_uint(f) | 1u);
+    return __half2float(__float2half_rn(f));
+}
+__device__ __forceinline__ float round_half(float f) { return __half2float(__float2half_rn(f)); }
+
+template <typename CT> __device__ __forceinline__ void axis_taps(CT x, int w, int addr, int &i0, int &i1, CT &a,
+                                                                 bool &c0, bool &c1)
+{
+    c0 = c1 = true;
+    if (addr == UFOMI_ADDR_REPEAT || addr == UFOMI_ADDR_MIRRORED_REPEAT) {
+        const CT s = x / (CT) w;
+        CT u;
+        if (addr == UFOMI_ADDR_REPEAT) {
+            u = (s - floor(s)) * (CT) w;
+        } else {
+            const CT sp = (CT) 2 * rint((CT) 0.5 * s);
+            u = fabs(s - sp) * (CT) w;
+        }
+        const CT f = floor(u - (CT) 0.5);
+        i0 = (int) f;
+        i1 = i0 + 1;
+        a = (u - (CT) 0.5) - f;
+        if (addr == UFOMI_ADDR_REPEAT) {
+            if (i0 < 0) i0 += w;
+            if (i1 > w - 1) i1 -= w;
+        } else {
+            i0 = max(i0, 0);
+            i1 = min(i1, w - 1);
+        }
+        i0 = min(max(i0, 0), w - 1);  // u within [0, w]; keeps every index inside for any rounding
+        i1 = min(max(i1, 0), w - 1);
+        return;
+    }
+    // far coordinates stay far on their side (no overflow in the conversion)
+    const CT f = fmin(fmax(floor(x - (CT) 0.5), (CT) -1e9), (CT) 1e9);
+    a = (x - (CT) 0.5) - f;
+    i0 = (int) f;
+    i1 = i0 + 1;
+    if (addr == UFOMI_ADDR_CLAMP) {
+        c0 = i0 >= 0 && i0 < w;
+        c1 = i1 >= 0 && i1 < w;
+    } else {
+        i0 = min(max(i0, 0), w - 1);
+        i1 = min(max(i1, 0), w - 1);
+    }
+}
+
+template <typename CT>
+__device__ __forceinline__ CT sample_precise(const float *__restrict__ img, int W, int H, CT x, CT y, int addr)
+{
+    if (!isfinite(x) || !isfinite(y))
+        return (CT) 0;
+    int i0, i1, j0, j1;
+    CT a, b;
+    bool ci0, ci1, cj0, cj1;
+    axis_taps(x, W, addr, i0, i1, a, ci0, ci1);
+    axis_taps(y, H, addr, j0, j1, b, cj0, cj1);
+    const float *r0 = img + (long) (cj0 ? j0 : 0) * W, *r1 = img + (long) (cj1 ? j1 : 0) * W;
+    const CT t00 = (cj0 && ci0) ? (CT) r0[i0] : (CT) 0, t10 = (cj0 && ci1) ? (CT) r0[i1] : (CT) 0;
+    const CT t01 = (cj1 && ci0) ? (CT) r1[i0] : (CT) 0, t11 = (cj1 && ci1) ? (CT) r1[i1] : (CT) 0;
+    const CT top = t00 + a * (t10 - t00), bot = t01 + a * (t11 - t01);
+    return top + b * (bot - top);
+}
+
+template <bool CONE, typename CT, int RT>
+__global__ __launch_bounds__(256) void gbp_precise_kernel(
+    const float *__restrict__ proj, int W, int H, int P, const CT *__restrict__ rows, CT x0, CT xs, CT y0, CT ys,
+    const CT *__restrict__ zs, int nx, int ny, int nz, CT norm, int addr, int burst, int st, float gmin, float gscale,
+    float gmax, void *__restrict__ vol)
+{
+    // the sum of a burst: in the wider of the compute and result types
+    using SumT = typename std::conditional<std::is_same<CT, double>::value || RT == RT_DOUBLE, double, float>::type;
+    const long n = (long) nx * ny * nz;
+    const long plane = (long) W * H;
+    for (long i = (long) blockIdx.x * 256 + threadIdx.x; i < n; i += (long) gridDim.x * 256) {
+        const int ix = (int) (i % nx), iy = (int) ((i / nx) % ny), iz = (int) (i / ((long) nx * ny));
+        const CT x = x0 + (CT) ix * xs, y = y0 + (CT) iy * ys, z = zs[iz];
+        SumT r = 0;
+        float vf = 0.0f;   // volume in float (store float or half, the latter rounded to half)
+        double vd = 0.0;   // volume in double (store double) or the whole sum (integer stores)
+        for (int p = 0; p < P; p++) {
+            const CT *m = rows + 16 * (long) p;
+            CT u = m[0] * x + m[1] * y + m[2] * z + m[3];
+            CT v = m[4] * x + m[5] * y + m[6] * z + m[7];
+            CT w2 = 1;
+            if constexpr (CONE) {
+                const CT d = m[8] * x + m[9] * y + m[10] * z + m[11];
+                u = u / d;
+                v = v / d;
+                const CT c = (CT) 1 / (m[12] * x + m[13] * y + m[14] * z + m[15]);
+                w2 = c * c;
+            }
+            const CT contrib = sample_precise<CT>(proj + p * plane, W, H, u, v, addr) * w2;
+            const SumT sum = (SumT) r + (SumT) contrib;
+            if constexpr (RT == RT_HALF)
+                r = (SumT) round_half(sum);
+            else if constexpr (RT == RT_FLOAT)
+                r = (SumT) (float) sum;
+            else
+                r = sum;
+            if ((p + 1) % burst == 0 || p + 1 == P) {
+                const SumT part = (SumT) norm * r;
+                const bool first = p + 1 <= burst;
+                if (st == UFOMI_ST_FLOAT)
+                    vf = first ? (float) part : vf + (float) part;
+                else if (st == UFOMI_ST_HALF)
+                    vf = first ? round_half(part) : round_half(vf + round_half(part));
+                else
+                    vd = first ? (double) part : vd + (double) part;
+                r = 0;
+            }
+        }
+        switch (st) {
+        case UFOMI_ST_FLOAT: ((float *) vol)[i] = vf; break;
+        case UFOMI_ST_HALF: ((__half *) vol)[i] = __float2half_rn(vf); break;
+        case UFOMI_ST_DOUBLE: ((double *) vol)[i] = vd; break;
+        case UFOMI_ST_UCHAR: store_voxel((uint8_t *) vol, i, (float) vd, gmin, gscale, gmax); break;
+        case UFOMI_ST_USHORT: store_voxel((uint16_t *) vol, i, (float) vd, gmin, gscale, gmax); break;
+        default: store_voxel((uint32_t *) vol, i, (float) vd, gmin, gscale, gmax); break;
+        }
+    }
+}
+
 double at(const double *a, int n, int i, double dflt) { return (!a || n <= 0) ? dflt : (n == 1 ? a[0] : a[i]); }
 
 bool almost_zero(const double *a, int n)
@@ -634,6 +783,62 @@ int gbp_launch(ufomi_ctx *ctx, const float *d_proj, int W, int H, unsigned n, co
     }
 }
 
+// The precise kernel over FP64 rows (16 per projection) for one parameter-z call.
+int gbp_precise_launch(ufomi_ctx *ctx, const float *d_proj, int W, int H, unsigned n, const std::vector<double> &rows,
+                       bool parallel, int kaddr, const ufomi_gbp_params *prm, int nx, int ny, int nz, void *d_vol)
+{
+    int rc;
+    const int st = prm->store_type;
+    const bool integer_store = st == UFOMI_ST_UCHAR || st == UFOMI_ST_USHORT || st == UFOMI_ST_UINT;
+    const bool f64 = prm->compute_type == UFOMI_CT_DOUBLE;
+    const int burst = prm->burst ? (int) prm->burst : 8;
+    double zstart = 0.0, zstep = 1.0;
+    if (!(fabs(prm->region[2]) < 1e-7)) {
+        zstart = prm->region[0];
+        zstep = prm->region[2];
+    }
+    const double x0 = prm->x_region[2] ? prm->x_region[0] : -((double) nx / 2);
+    const double xs = prm->x_region[2] ? prm->x_region[2] : 1.0;
+    const double y0 = prm->y_region[2] ? prm->y_region[0] : -((double) ny / 2);
+    const double ys = prm->y_region[2] ? prm->y_region[2] : 1.0;
+    const double norm = fabs(prm->overall_angle) / n;
+    const double maxval = st == UFOMI_ST_UCHAR ? 255.0 : st == UFOMI_ST_USHORT ? 65535.0 : 4294967295.0;
+    const float gscale = integer_store ? (float) (maxval / (prm->gray_map_max - prm->gray_map_min)) : 0.0f;
+
+    auto run = [&](auto ct_tag) {
+        using CT = decltype(ct_tag);
+        std::vector<CT> tab(rows.size() + (size_t) nz);
+        for (size_t i = 0; i < rows.size(); i++) tab[i] = (CT) rows[i];
+        for (int k = 0; k < nz; k++) tab[rows.size() + k] = (CT) (zstart + k * zstep);
+        void *d_tab = nullptr;
+        int rc2;
+        if ((rc2 = ufomi_scratch(ctx, tab.size() * sizeof(CT), &d_tab))) return rc2;
+        if ((rc2 = ufomi_upload(ctx, d_tab, tab.data(), tab.size() * sizeof(CT)))) return rc2;
+        const CT *d_rows = (const CT *) d_tab, *d_zs = d_rows + rows.size();
+        const long voxels = (long) nx * ny * nz;
+        const unsigned grid = (unsigned) std::min<long>((voxels + 255) / 256, 16384);
+        auto go = [&](auto cone_c, auto rt_c) {
+            constexpr bool CONE = decltype(cone_c)::value;
+            constexpr int RT = decltype(rt_c)::value;
+            hipLaunchKernelGGL((gbp_precise_kernel<CONE, CT, RT>), dim3(grid), dim3(256), 0, ctx->stream, d_proj, W,
+                               H, (int) n, d_rows, (CT) x0, (CT) xs, (CT) y0, (CT) ys, d_zs, nx, ny, nz, (CT) norm,
+                               kaddr, burst, st, (float) prm->gray_map_min, gscale, (float) maxval, d_vol);
+            UFOMI_CHECK_LAUNCH();
+            return (int) UFOMI_OK;
+        };
+        auto by_rt = [&](auto cone_c) {
+            switch (prm->result_type) {
+            case UFOMI_RT_HALF: return go(cone_c, std::integral_constant<int, RT_HALF>());
+            case UFOMI_RT_DOUBLE: return go(cone_c, std::integral_constant<int, RT_DOUBLE>());
+            default: return go(cone_c, std::integral_constant<int, RT_FLOAT>());
+            }
+        };
+        return parallel ? by_rt(std::false_type()) : by_rt(std::true_type());
+    };
+    rc = f64 ? run(0.0) : run(0.0f);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" void ufomi_gbp_params_init(ufomi_gbp_params *p)
@@ -642,6 +847,9 @@ extern "C" void ufomi_gbp_params_init(ufomi_gbp_params *p)
     p->overall_angle = 2.0 * M_PI;
     p->store_type = UFOMI_ST_FLOAT;
     p->addressing_mode = UFOMI_ADDR_CLAMP;
+    p->compute_type = UFOMI_CT_FLOAT;
+    p->result_type = UFOMI_RT_FLOAT;
+    p->burst = 0;
 }
 
 extern "C" void ufomi_gbp_geometry_init(ufomi_gbp_geometry *g) { memset(g, 0, sizeof(*g)); }
@@ -684,17 +892,27 @@ extern "C" int ufomi_gbp(ufomi_ctx *ctx, const float *d_proj, int W, int H, int 
     if (integer_store && prm->gray_map_min >= prm->gray_map_max)
         return ufomi_fail(UFOMI_ERR_GRAY_MAP, "Gray mapping minimum must be less then the maximum");
     int addr = prm->addressing_mode;
-    if (addr == UFOMI_ADDR_REPEAT || addr == UFOMI_ADDR_MIRRORED_REPEAT)
-        return ufomi_fail(UFOMI_ERR_UNSUPPORTED,
-                          "repeat addressing needs normalized coordinates; unnormalized sampling is undefined");
-    if (addr != UFOMI_ADDR_CLAMP && addr != UFOMI_ADDR_CLAMP_TO_EDGE && addr != UFOMI_ADDR_NONE)
+    if (addr < UFOMI_ADDR_NONE || addr > UFOMI_ADDR_MIRRORED_REPEAT)
         return ufomi_fail(UFOMI_ERR_ARG, "unknown addressing mode %d", addr);
+    if (prm->compute_type != UFOMI_CT_FLOAT && prm->compute_type != UFOMI_CT_DOUBLE)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown compute type %d", prm->compute_type);
+    if (prm->result_type < UFOMI_RT_HALF || prm->result_type > UFOMI_RT_DOUBLE)
+        return ufomi_fail(UFOMI_ERR_ARG, "unknown result type %d", prm->result_type);
+    if (prm->burst > 128)
+        return ufomi_fail(UFOMI_ERR_ARG, "burst %u above 128", prm->burst);
     // "none" leaves out-of-range reads undefined in OpenCL; clamp to the edge here.
-    const int kaddr = addr == UFOMI_ADDR_CLAMP ? UFOMI_ADDR_CLAMP : UFOMI_ADDR_CLAMP_TO_EDGE;
+    const int kaddr = addr == UFOMI_ADDR_NONE ? UFOMI_ADDR_CLAMP_TO_EDGE : addr;
+    const bool precise = prm->compute_type == UFOMI_CT_DOUBLE || prm->result_type != UFOMI_RT_FLOAT ||
+                         kaddr == UFOMI_ADDR_REPEAT || kaddr == UFOMI_ADDR_MIRRORED_REPEAT;
 
     std::vector<double> rows(16 * (size_t) n);
     int parallel = 0;
     if ((rc = ufomi_gbp_projection_rows(g, prm, rows.data(), &parallel))) return rc;
+    if (precise) {
+        int nx, ny, nz;
+        if ((rc = ufomi_gbp_volume_shape(prm, W, &nx, &ny, &nz))) return rc;
+        return gbp_precise_launch(ctx, d_proj, W, H, n, rows, parallel != 0, kaddr, prm, nx, ny, nz, d_vol);
+    }
     std::vector<float> mats(rows.size());
     for (size_t i = 0; i < rows.size(); i++)
         mats[i] = (float) rows[i];
@@ -739,9 +957,42 @@ extern "C" int ufomi_gbp_sweep(ufomi_ctx *ctx, const float *d_proj, int W, int H
         return ufomi_gbp(ctx, d_proj, W, H, P, g, prm, d_vol);
     if (!g || !prm || !d_vol)
         return ufomi_fail(UFOMI_ERR_ARG, "invalid general backprojection arguments");
-    if (parameter == 2)
-        return ufomi_fail(UFOMI_ERR_UNSUPPORTED,
-                          "axis-angle-z holds the tomographic angles; it is not a sweep parameter");
+    if (parameter == 2) {
+        // The generated kernel rotates by its tomographic-angle arguments and never
+        // reads axis_angle_z.  Without per-projection geometry (the scalar kernel,
+        // general_bp_header_scalar.in) the assignment `axis_angle_z = region[idz]`
+        // (make_parameter_assignment, :762-764) names an undeclared variable and the
+        // kernel fails to build; with it (:1103-1116, the vectorized kernel) the
+        // variable is declared and unused, so every slice is the reconstruction at z.
+        const unsigned np = prm->num_projections;
+        const int counts[] = {g->n_axis_angle_x, g->n_axis_angle_y, g->n_volume_angle_x, g->n_volume_angle_y,
+                              g->n_volume_angle_z, g->n_detector_angle_x, g->n_detector_angle_y,
+                              g->n_detector_angle_z, g->n_detector_position_x, g->n_detector_position_y,
+                              g->n_detector_position_z, g->n_source_position_x, g->n_source_position_y,
+                              g->n_source_position_z, g->n_center_position_x, g->n_center_position_z};
+        bool vectorized = false;
+        for (int c : counts)
+            vectorized = vectorized || (np > 0 && c == (int) np);
+        if (!vectorized)
+            return ufomi_fail(UFOMI_ERR_UNSUPPORTED,
+                              "parameter axis-angle-z without per-projection geometry: the reference's kernel "
+                              "assigns the undeclared axis_angle_z and does not build");
+        int nx, ny, nz, rc;
+        if ((rc = ufomi_gbp_volume_shape(prm, W, &nx, &ny, &nz))) return rc;
+        static const size_t st_bytes[] = {2, 4, 8, 1, 2, 4};
+        if (prm->store_type < UFOMI_ST_HALF || prm->store_type > UFOMI_ST_UINT)
+            return ufomi_fail(UFOMI_ERR_ARG, "unknown store type %d", prm->store_type);
+        const size_t slice = (size_t) nx * ny * st_bytes[prm->store_type];
+        ufomi_gbp_params pk = *prm;
+        pk.region[0] = prm->z;
+        pk.region[1] = prm->z + 1.0;
+        pk.region[2] = 1.0;
+        if ((rc = ufomi_gbp(ctx, d_proj, W, H, P, g, &pk, d_vol))) return rc;
+        for (int k = 1; k < nz; k++)
+            UFOMI_HIP(hipMemcpyAsync((char *) d_vol + (size_t) k * slice, d_vol, slice, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        return UFOMI_OK;
+    }
     ufomi_gbp_geometry gk = *g;
     const double **field;
     int *count;

@@ -1023,6 +1023,9 @@ static void gbp_fill_params(const UfomiTask *t, ufomi_gbp_params *p)
     p->addressing_mode = addr_map[(int) t->v[G_ADDR].num];
     p->gray_map_min = t->v[G_GMIN].num;
     p->gray_map_max = t->v[G_GMAX].num;
+    p->compute_type = (int) t->v[G_COMPUTE].num;
+    p->result_type = (int) t->v[G_RESULT].num;
+    p->burst = (unsigned) t->v[G_BURST].num;
 }
 
 static void gbp_fill_geometry(const UfomiTask *t, ufomi_gbp_geometry *g)
@@ -1049,29 +1052,37 @@ static int gbp_setup(UfomiTask *t, UfomiError *e)
     if (t->v[G_GMIN].num >= t->v[G_GMAX].num && (st == UFOMI_ST_UCHAR || st == UFOMI_ST_USHORT || st == UFOMI_ST_UINT))
         return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_GRAY_MAP,
                           "Gray mapping minimum must be less then the maximum");
-    if ((int) t->v[G_PARAM].num == 2)
-        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
-                          "general-backproject: axis-angle-z holds the tomographic angles, it cannot be swept");
-    /* compute-type double (FP64 coordinates, then float samples, :998) and
-     * result-type double (FP64 running sum between bursts) run on the float
-     * kernel: its rows are composed in FP64 and every projection accumulates
-     * in registers, within 1e-5 of the FP64 oracle.  result-type half rounds
-     * the running sum to half after every burst, which this kernel does not
-     * reproduce. */
-    if ((int) t->v[G_RESULT].num == 0)
-        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
-                          "general-backproject: result-type half is not built");
-    const int addr = (int) t->v[G_ADDR].num;
-    if (addr == 3 || addr == 4)
-        return task_error(e, UFOMI_TASK_ERROR_SETUP, UFOMI_ERR_UNSUPPORTED,
-                          "general-backproject: repeat addressing needs normalized coordinates");
+    /* compute-type double, result-type half / double and the repeat addressing
+     * modes run on the precise kernel of ufomi_gbp (FP64 maps and weights,
+     * per-projection rounding of the running sum to the result type, OpenCL's
+     * normalised repeat rules applied to u / W).  A swept axis-angle-z follows
+     * ufomi_gbp_sweep: an error without per-projection geometry arrays, every
+     * slice the z reconstruction with them. */
     return UFOMI_OK;
+}
+
+/* Whether the reference generates its vectorized kernel: some geometry array
+ * other than axis-angle-z holds one value per projection (:1321-1336). */
+static int gbp_vectorized(const UfomiTask *t)
+{
+    static const int arrays[] = {G_AX, G_AY, G_VX, G_VY, G_VZ, G_DAX, G_DAY, G_DAZ, G_DX, G_DY, G_DZ,
+                                 G_SX, G_SY, G_SZ, G_CX, G_CZ};
+    const int n = (int) (unsigned) t->v[G_NPROJ].num;
+    for (size_t k = 0; k < sizeof(arrays) / sizeof(arrays[0]); k++)
+        if (n > 0 && t->v[arrays[k]].n == n)
+            return 1;
+    return 0;
 }
 
 static int gbp_requisition(UfomiTask *t, UfomiBuffer **in, UfomiRequisition *req, UfomiError *e)
 {
     GbpPriv *p = t->priv;
     const UfomiRequisition *ir = &in[0]->req;
+    /* parameter axis-angle-z with the scalar kernel: `axis_angle_z = region[idz]`
+     * names an undeclared variable, the kernel does not build (:1509-1516) */
+    if ((int) t->v[G_PARAM].num == 2 && !gbp_vectorized(t))
+        return task_error(e, UFOMI_TASK_ERROR_GET_REQUISITION, UFOMI_ERR_UNSUPPORTED,
+                          "Error creating backprojection kernels");
     if (!p->d_stack) {
         gbp_fill_params(t, &p->params);
         int rc = ufomi_gbp_volume_shape(&p->params, (int) ir->dims[0], &p->nx, &p->ny, &p->nz);

@@ -85,7 +85,12 @@ class GbpParams(ctypes.Structure):
     _fields_ = [("num_projections", ctypes.c_uint), ("overall_angle", ctypes.c_double), ("z", ctypes.c_double),
                 ("region", ctypes.c_double * 3), ("x_region", ctypes.c_double * 3),
                 ("y_region", ctypes.c_double * 3), ("store_type", ctypes.c_int), ("addressing_mode", ctypes.c_int),
-                ("gray_map_min", ctypes.c_double), ("gray_map_max", ctypes.c_double)]
+                ("gray_map_min", ctypes.c_double), ("gray_map_max", ctypes.c_double),
+                ("compute_type", ctypes.c_int), ("result_type", ctypes.c_int), ("burst", ctypes.c_uint)]
+
+
+COMPUTE_TYPES = {"float": 0, "double": 1}
+RESULT_TYPES = {"half": 0, "float": 1, "double": 2}
 
 
 class LaminoParams(ctypes.Structure):
@@ -260,7 +265,7 @@ def gbp_projection_rows(geometry, params):
 
 def gbp_params(num_projections, overall_angle=2 * math.pi, region=(0, 0, 0), x_region=(0, 0, 0),
                y_region=(0, 0, 0), store_type="float", addressing_mode="clamp", gray_map_min=0.0,
-               gray_map_max=0.0):
+               gray_map_max=0.0, compute_type="float", result_type="float", burst=0):
     p = GbpParams()
     load().ufomi_gbp_params_init(ctypes.byref(p))
     p.num_projections = num_projections
@@ -272,6 +277,9 @@ def gbp_params(num_projections, overall_angle=2 * math.pi, region=(0, 0, 0), x_r
     p.addressing_mode = ADDRESSING[addressing_mode]
     p.gray_map_min = gray_map_min
     p.gray_map_max = gray_map_max
+    p.compute_type = COMPUTE_TYPES[compute_type]
+    p.result_type = RESULT_TYPES[result_type]
+    p.burst = burst
     return p
 
 
