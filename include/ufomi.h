@@ -124,10 +124,13 @@ int ufomi_ifft_rows(ufomi_ctx *ctx, const float *d_in, int N, int P, int crop, f
  * ufomi_ifft_nd: interleaved complex [m2][m1][m0] in, inverse DFT over the
  * transformed axes normalised by their sizes, real part cropped to
  * crop_width x crop_height (0 = no crop) out, [m2][crop_height][crop_width].
- * Powers of two up to 8192 and other sizes up to 4096 (chirp-z) are built on
- * every axis, one LDS pass per transformed axis; 1-D transforms of longer
- * rows run as two passes of a four-step split (powers of two up to 2^20
- * points) or as Bluestein over that split (other sizes up to 2^19).
+ * Powers of two up to 8192 and other sizes up to 4096 (chirp-z) take one
+ * LDS pass per transformed axis.  Longer axes run as two passes of a
+ * four-step split (powers of two up to 2^20 points) or as Bluestein over that
+ * split (other sizes up to 2^19): along the rows of 1-D transforms directly,
+ * along any axis of 2-D and 3-D transforms after gathering the lines into
+ * rows (and scattering them back); such transforms allocate two intermediates
+ * of the output's size for the call.
  */
 int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, const int n[3], int dimensions, const int m[3],
                  float *d_out);

@@ -7,7 +7,8 @@ Shepp-Logan window and with ramp-fromreal, backprojected by the default
 unmasked branch; 4096 tiles run unsplit).  A second case runs 8 slices of
 512 px, which splits the angles into partial volumes.
 
-Bar: filtered sinogram within 1e-5 of the FP64 ramp chain; f32 volume within
+Bar: filtered sinogram within 1e-6 of the FP64 ramp chain relative to the
+input rows' norm (5e-5 relative to the filtered rows' own); f32 volume within
 1e-5 relative L2 of the oracle's float32 reference semantics
 (backproject.cl:50-94 on the float32-rounded FP64 filter output) and of the
 FP64 oracle; f16 storage within 1e-3 and bf16 within 8e-3 of FP64.
@@ -57,7 +58,10 @@ def test_config1_shapes_f32(ctx, torch_dev, filt_name):
     k = 3  # one slice against both oracles (about 10 s of CPU each)
     s = sino[k].cpu().numpy()
     f64 = oracle.ramp_chain(s, N, oracle.filter_coefficients(2 * N, filt_name), W)
-    assert rel_l2(filt[k], f64) < 1e-5
+    # float32 FFT rounding scales with the input; the ramp removes most of the smooth sinogram's
+    # energy, so against the small filtered rows the same rounding reads about 10x larger
+    assert rel_l2(filt[k], f64) < 5e-5
+    assert np.linalg.norm(filt[k] - f64) / np.linalg.norm(s) < 1e-6
     ref32 = oracle.bp_texture_ref32(f64.astype(np.float32), axis_pos=W / 2.0)
     ref64 = oracle.bp_texture_f64(f64, axis_pos=W / 2.0)
     err32, err64 = rel_l2(vol[k], ref32), rel_l2(vol[k], ref64)
@@ -93,6 +97,7 @@ def test_eight_slices_split_angles(ctx, torch_dev, filt_name):
     coef = oracle.filter_coefficients(2 * N, filt_name)
     for k in range(S):
         f64 = oracle.ramp_chain(s_all[k], N, coef, W)
-        assert rel_l2(filt[k], f64) < 1e-5
+        assert rel_l2(filt[k], f64) < 5e-5
+        assert np.linalg.norm(filt[k] - f64) / np.linalg.norm(s_all[k]) < 1e-6
         assert rel_l2(vol[k], oracle.bp_texture_ref32(f64.astype(np.float32), axis_pos=W / 2.0)) < 1e-5
         assert rel_l2(vol[k], oracle.bp_texture_f64(f64, axis_pos=W / 2.0)) < 1e-5

@@ -119,9 +119,16 @@ def test_unstacked_2d(ctx, torch_dev):
 
 @pytest.mark.parametrize("shape,dims,m", [((2, 40, 3000), 1, (3000, 40, 2)), ((1, 300, 200), 2, (512, 300, 1)),
                                           ((3, 64, 4096), 1, (4096, 64, 3)), ((1, 2, 8192), 1, (8192, 2, 1)),
-                                          ((5, 7, 1), 2, (1, 7, 5))])
+                                          ((5, 7, 1), 2, (1, 7, 5)),
+                                          # axes longer than one LDS pass in 2-D and 3-D transforms
+                                          ((1, 4, 9), 2, (4097, 4, 1)), ((1, 4, 9), 2, (16, 16384, 1)),
+                                          ((1, 4, 9), 2, (16384, 4, 1)), ((3, 5, 7), 3, (16, 8, 9000)),
+                                          ((2, 3, 20), 2, (32, 20000, 3)), ((2, 3, 10), 1, (16384, 5, 2)),
+                                          ((2, 3, 5), 3, (10000, 4, 3)), ((1, 4, 9), 2, (32768, 4, 1))])
 def test_large_and_edge_sizes(ctx, torch_dev, shape, dims, m):
-    """The longest chirp-z (4096 needs an 8192-point convolution), the longest power of two, size 1."""
+    """The longest LDS chirp-z (4096 needs an 8192-point convolution), the longest LDS power of two,
+    size 1, and axes past one LDS pass (gathered into rows, four-step split or Bluestein over it,
+    scattered back) on every axis, up to the task's 32768."""
     x = np.random.default_rng(sum(shape)).standard_normal(shape).astype(np.float32)
     fft = _task("fft", dimensions=dims, size_x=m[0], size_y=m[1], size_z=m[2])
     got = _run(ctx, x[None], fft)[0].view(np.complex64)
@@ -133,39 +140,51 @@ def test_large_and_edge_sizes(ctx, torch_dev, shape, dims, m):
     back = _run(ctx, x[None], _task("fft", dimensions=dims, size_x=m[0], size_y=m[1], size_z=m[2]),
                 _task("ifft", dimensions=dims))[0]
     assert rel_l2(back, pad) < 1e-5
+    if dims >= 2:  # the inverse's cropped last pass
+        cw, ch = max(1, m[0] // 3), max(1, m[1] // 2)
+        crop = _run(ctx, x[None], _task("fft", dimensions=dims, size_x=m[0], size_y=m[1], size_z=m[2]),
+                    _task("ifft", dimensions=dims, crop_width=cw, crop_height=ch))[0]
+        assert rel_l2(crop, pad[:, :ch, :cw]) < 1e-5
 
 
-def test_complex_input_and_unbuilt_sizes(ctx, torch_dev):
-    """A complex item goes through the forward transform unchanged in size; sizes past the
-    built range fail with a clear status."""
+def test_complex_input_and_long_axes(ctx, torch_dev):
+    """A complex item goes through the forward transform unchanged in size, also along an axis
+    longer than one LDS pass."""
     x = np.random.default_rng(2).standard_normal((1, 6, 10)).astype(np.float32)
     spec = _run(ctx, x[None], _task("fft", dimensions=1, size_x=12))[0]
     assert spec.shape == (1, 6, 24)
-    # the spectrum again through fft: complex input, size taken from the interleaved width
-    t = _task("fft", dimensions=1)
     buf_in = spec.view(np.complex64)
-    ref = oracle.fft_nd(buf_in, 1, (12, 6, 1), in_complex=True)
     import torch
     dev = torch.device("cuda:0")
     xin = torch.from_numpy(spec).to(dev)
-    b = ufo_tasks.Buffer()
-    b.data = xin.data_ptr()
-    b.req.n_dims = 3
-    b.req.dims[0], b.req.dims[1], b.req.dims[2] = 24, 6, 1
-    b.layout = ufo_tasks.LAYOUT_COMPLEX
-    b.dtype = 0
-    t.setup(ctx)
-    req = t.get_requisition(b)
-    assert (req.dims[0], req.dims[1], req.dims[2]) == (24, 6, 1)
-    out = torch.empty((1, 6, 24), device=dev)
-    o = ufo_tasks.Buffer()
-    o.data = out.data_ptr()
-    o.req = req
-    t.process(b, o, req)
-    torch.cuda.synchronize()
-    assert rel_l2(out.cpu().numpy().view(np.complex64), ref) < 1e-5
-    with pytest.raises(RuntimeError, match="not built"):
-        _run(ctx, x[None], _task("fft", dimensions=2, size_x=5000))
+
+    def run_complex(t):
+        # the spectrum again through fft: a complex-layout buffer, size taken from the interleaved width
+        b = ufo_tasks.Buffer()
+        b.data = xin.data_ptr()
+        b.req.n_dims = 3
+        b.req.dims[0], b.req.dims[1], b.req.dims[2] = 24, 6, 1
+        b.layout = ufo_tasks.LAYOUT_COMPLEX
+        b.dtype = 0
+        t.setup(ctx)
+        req = t.get_requisition(b)
+        out = torch.empty((req.dims[2], req.dims[1], req.dims[0]), device=dev)
+        o = ufo_tasks.Buffer()
+        o.data = out.data_ptr()
+        o.req = req
+        t.process(b, o, req)
+        torch.cuda.synchronize()
+        return (req.dims[0], req.dims[1], req.dims[2]), out.cpu().numpy().view(np.complex64)
+
+    dims, got = run_complex(_task("fft", dimensions=1))
+    assert dims == (24, 6, 1)
+    assert rel_l2(got, oracle.fft_nd(buf_in, 1, (12, 6, 1), in_complex=True)) < 1e-5
+    dims, got = run_complex(_task("fft", dimensions=2, size_x=5000))
+    assert dims == (10000, 6, 1)
+    assert rel_l2(got, oracle.fft_nd(buf_in, 2, (5000, 6, 1), in_complex=True)) < 1e-5
+    # the same items with the real layout: 24 real samples per row
+    got_r = _run(ctx, spec[None], _task("fft", dimensions=2, size_x=5000))[0]
+    assert rel_l2(got_r.view(np.complex64), oracle.fft_nd(spec, 2, (5000, 6, 1))) < 1e-5
 
 
 @pytest.mark.parametrize("width,size_x", [(10000, 16384), (16384, 16384), (20000, 32768), (9, 32768)])

@@ -64,16 +64,16 @@ def test_round_trip_crop_like_reference(ctx):
     np.testing.assert_almost_equal(back, padded, decimal=5)
 
 
-def test_not_built_paths_raise(ctx):
-    """Transform sizes past the built range (chirp-z above 4096, powers of two above 8192, except along
-    the rows of 1-D transforms) fail loudly."""
-    x = np.ones((1, 4, 9), np.float32)
-    with pytest.raises(RuntimeError, match="not built"):
-        run_chain(ctx, x, make("fft", dimensions=2, size_x=4097, size_y=4))
-    with pytest.raises(RuntimeError, match="not built"):
-        run_chain(ctx, x, make("fft", dimensions=2, size_x=16, size_y=16384))
-    with pytest.raises(RuntimeError, match="not built"):
-        run_chain(ctx, x, make("fft", dimensions=2, size_x=16384, size_y=4))
+def test_sizes_up_to_the_property_range(ctx):
+    """size-x/y up to 32768 (the properties' range, ufo-fft-task.c:513-531) on 2-D transforms, chirp-z
+    and powers of two, against numpy's FP64 transform."""
+    x = np.linspace(-1, 2, 36, dtype=np.float32).reshape(1, 4, 9)
+    for sx, sy in ((4097, 4), (16, 16384), (32768, 4), (20000, 5)):
+        got = run_chain(ctx, x, make("fft", dimensions=2, size_x=sx, size_y=sy))
+        pad = np.zeros((1, sy, sx))
+        pad[:, :4, :9] = x
+        ref = np.fft.fftn(pad, axes=(1, 2))
+        assert rel_l2(got.view(np.complex64), ref) < 1e-5
 
 
 def test_requisition_errors(ctx):

@@ -421,18 +421,23 @@ __global__ __launch_bounds__(kEw) void chirp_post_kernel(const float2 *__restric
 
 unsigned ew_blocks(long total) { return (unsigned) std::min<long>((total + kEw - 1) / kEw, 1L << 16); }
 
+// Bytes of work space split_chirp_on needs for `rows` rows of an M-point transform.
+size_t split_chirp_bytes(int rows, int M)
+{
+    const long L = next_pow2(2L * M - 1);
+    return (((size_t) M + L) * sizeof(float2) + 255) / 256 * 256 + 3 * (size_t) rows * L * sizeof(float2);
+}
+
 template <int DIR, typename InT, bool OUT_REAL>
-int split_chirp(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int M, long keep, long out_pitch,
-                float scale, void *out)
+int split_chirp_on(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int M, long keep,
+                   long out_pitch, float scale, void *out, void *base)
 {
     const int L = (int) next_pow2(2L * M - 1);
     if (rows <= 0 || keep > M)
         return ufomi_fail(UFOMI_ERR_ARG, "invalid transform rows");
     const size_t row_bytes = (size_t) rows * L * sizeof(float2);
     const size_t tab = (((size_t) M + L) * sizeof(float2) + 255) / 256 * 256;
-    void *base = nullptr;
-    int rc = ufomi_scratch(ctx, tab + 3 * row_bytes, &base);
-    if (rc) return rc;
+    int rc;
     float2 *chirp = (float2 *) base, *kern = chirp + M;
     float2 *A = (float2 *) ((char *) base + tab), *B = (float2 *) ((char *) A + row_bytes),
            *T = (float2 *) ((char *) B + row_bytes);
@@ -454,6 +459,195 @@ int split_chirp(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int r
                        keep, out_pitch, scale, out, tot_out);
     UFOMI_CHECK_LAUNCH();
     return UFOMI_OK;
+}
+
+template <int DIR, typename InT, bool OUT_REAL>
+int split_chirp(ufomi_ctx *ctx, const void *in, long in_pitch, long valid, int rows, int M, long keep, long out_pitch,
+                float scale, void *out)
+{
+    void *base = nullptr;
+    int rc = ufomi_scratch(ctx, split_chirp_bytes(rows, M), &base);
+    if (rc) return rc;
+    return split_chirp_on<DIR, InT, OUT_REAL>(ctx, in, in_pitch, valid, rows, M, keep, out_pitch, scale, out, base);
+}
+
+// ---- axes longer than one LDS pass holds, in 2-D and 3-D transforms
+//
+// The lines along the axis are gathered into contiguous rows, transformed by the
+// four-step split (powers of two) or Bluestein over it (other sizes), and
+// scattered back along the axis of the destination.  Line (b, c) runs over the
+// two other axes; element i of it sits at b * sb + c * sc + i * sa.
+struct LineMap {
+    long sa, sb, sc;  // element, line-b and line-c strides (elements)
+};
+
+template <typename InT>
+__global__ __launch_bounds__(kEw) void line_gather_kernel(const InT *__restrict__ in, LineMap m, int nb, int n,
+                                                          float2 *__restrict__ rows, long total)
+{
+    for (long idx = (long) blockIdx.x * kEw + threadIdx.x; idx < total; idx += (long) gridDim.x * kEw) {
+        const long l = idx / n, i = idx - l * n;
+        const long b = l % nb, c = l / nb;
+        rows[idx] = load_as_complex(in, b * m.sb + c * m.sc + i * m.sa);
+    }
+}
+
+template <bool OUT_REAL>
+__global__ __launch_bounds__(kEw) void line_scatter_kernel(const float2 *__restrict__ rows, LineMap m, int nb, int n,
+                                                           void *__restrict__ out, long total)
+{
+    for (long idx = (long) blockIdx.x * kEw + threadIdx.x; idx < total; idx += (long) gridDim.x * kEw) {
+        const long l = idx / n, k = idx - l * n;
+        const long b = l % nb, c = l / nb;
+        const long o = b * m.sb + c * m.sc + k * m.sa;
+        if (OUT_REAL)
+            ((float *) out)[o] = rows[idx].x;
+        else
+            ((float2 *) out)[o] = rows[idx];
+    }
+}
+
+LineMap line_map(int axis, const int X[3])
+{
+    const long s[3] = {1, X[0], (long) X[0] * X[1]};
+    const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+    return LineMap{s[axis], s[o1], s[o2]};
+}
+
+// One axis of `M` points (n_in inputs per line, n_out outputs kept) from src
+// (extents S) into dst (extents D); lines cover the other axes up to E.
+template <int DIR, typename InT, bool OUT_REAL>
+int long_axis(ufomi_ctx *ctx, int axis, const void *src, const int S[3], const int D[3], const int E[3], int n_in,
+              int M, int n_out, float scale, void *dst)
+{
+    const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+    const int nb = E[o1], nc = E[o2];
+    const long lines = (long) nb * nc;
+    if (lines <= 0 || lines >= (1L << 31))
+        return ufomi_fail(UFOMI_ERR_ARG, "transform of %ld lines does not fit one launch", lines);
+    const bool p2 = pow2(M);
+    if (p2 ? (M > kMaxSplit) : (M > kMaxSplit / 2))
+        return ufomi_fail(UFOMI_ERR_UNSUPPORTED, "transform size %d is not built (powers of two up to %ld, "
+                          "other sizes up to %ld)", M, kMaxSplit, kMaxSplit / 2);
+    const size_t in_bytes = ((size_t) lines * n_in * sizeof(float2) + 255) / 256 * 256;
+    const size_t out_bytes = ((size_t) lines * n_out * sizeof(float2) + 255) / 256 * 256;
+    const size_t work = p2 ? (size_t) lines * M * sizeof(float2) : split_chirp_bytes((int) lines, M);
+    void *base = nullptr;
+    int rc = ufomi_scratch(ctx, in_bytes + out_bytes + work, &base);
+    if (rc) return rc;
+    float2 *rows_in = (float2 *) base, *rows_out = (float2 *) ((char *) base + in_bytes);
+    void *wk = (char *) base + in_bytes + out_bytes;
+    const long t_in = lines * n_in, t_out = lines * n_out;
+    hipLaunchKernelGGL(line_gather_kernel<InT>, dim3(ew_blocks(t_in)), dim3(kEw), 0, ctx->stream, (const InT *) src,
+                       line_map(axis, S), nb, n_in, rows_in, t_in);
+    UFOMI_CHECK_LAUNCH();
+    if (p2)
+        rc = split_passes<DIR, float2, false>(ctx, rows_in, n_in, n_in, (int) lines, M, n_out, n_out, scale,
+                                              (float2 *) wk, rows_out);
+    else
+        rc = split_chirp_on<DIR, float2, false>(ctx, rows_in, n_in, n_in, (int) lines, M, n_out, n_out, scale,
+                                                rows_out, wk);
+    if (rc) return rc;
+    hipLaunchKernelGGL(line_scatter_kernel<OUT_REAL>, dim3(ew_blocks(t_out)), dim3(kEw), 0, ctx->stream, rows_out,
+                       line_map(axis, D), nb, n_out, dst, t_out);
+    UFOMI_CHECK_LAUNCH();
+    return UFOMI_OK;
+}
+
+bool fits_lds(int M)
+{
+    bool chirp;
+    return line_length(M, &chirp) <= kMaxL;
+}
+
+// A transform with an axis longer than one LDS pass: axis by axis through two
+// complex intermediates of the library's own, each axis either one LDS pass
+// or a long_axis gather / split transform / scatter.
+//   forward: axes 0 .. dims-1, the last writing every output extent m;
+//   inverse: axes dims-1 .. 0, the last (axis 0) writing real, cropped rows.
+template <int DIR>
+int axis_by_axis(ufomi_ctx *ctx, const float *d_in, bool in_complex, const int n[3], int dimensions, const int m[3],
+                 int cw, int ch, float *d_out)
+{
+    int order[3], np = 0;
+    for (int k = 0; k < dimensions; k++)
+        order[np++] = DIR < 0 ? k : dimensions - 1 - k;
+    // extents before / after each step and the largest intermediate
+    int S[3] = {n[0], n[1], n[2]};
+    size_t inter = 1;
+    int Ss[3][3], Ds[3][3], Es[3][3], nin[3], nout[3];
+    for (int p = 0; p < np; p++) {
+        const int a = order[p];
+        int D[3] = {S[0], S[1], S[2]}, E[3] = {S[0], S[1], S[2]};
+        const bool last = p == np - 1;
+        nin[p] = S[a];
+        if (DIR < 0) {
+            nout[p] = m[a];
+            D[a] = m[a];
+            if (last)
+                for (int k = 0; k < 3; k++) D[k] = m[k];
+        } else {
+            nout[p] = a == 0 ? cw : (a == 1 ? ch : m[2]);
+            D[a] = nout[p];
+            if (a == 0) E[1] = D[1] = ch;
+        }
+        for (int k = 0; k < 3; k++) Ss[p][k] = S[k], Ds[p][k] = D[k], Es[p][k] = E[k];
+        if (!last)
+            inter = std::max(inter, (size_t) D[0] * D[1] * D[2]);
+        for (int k = 0; k < 3; k++) S[k] = D[k];
+    }
+    float2 *buf[2] = {nullptr, nullptr};
+    int rc = UFOMI_OK;
+    if (np > 1) {
+        if ((rc = ufomi_device_alloc(ctx, inter * sizeof(float2), (void **) &buf[0]))) return rc;
+        if (np > 2 && (rc = ufomi_device_alloc(ctx, inter * sizeof(float2), (void **) &buf[1]))) {
+            ufomi_device_free(ctx, buf[0]);
+            return rc;
+        }
+    }
+    if (DIR < 0) {
+        bool padded_batch = false;
+        for (int k = dimensions; k < 3; k++) padded_batch |= m[k] > n[k];
+        if (padded_batch)
+            rc = hipMemsetAsync(d_out, 0, (size_t) m[0] * m[1] * m[2] * sizeof(float2), ctx->stream) == hipSuccess
+                     ? UFOMI_OK : ufomi_fail(UFOMI_ERR_HIP, "hipMemsetAsync failed");
+    }
+    double norm = 1.0;
+    for (int k = 0; k < dimensions; k++) norm *= m[k];
+    const void *src = d_in;
+    for (int p = 0; p < np && !rc; p++) {
+        const int a = order[p];
+        const bool last = p == np - 1, real_in = p == 0 && DIR < 0 && !in_complex, real_out = last && DIR > 0;
+        void *dst = last ? (void *) d_out : (void *) buf[p & 1];
+        const float scale = real_out ? (float) (1.0 / norm) : 1.0f;
+        if (fits_lds(m[a])) {
+            PassSpec ps;
+            if ((rc = make_pass(a, Ss[p], Ds[p], Es[p], nin[p], nout[p], m[a], &ps))) break;
+            ps.g.scale = scale;
+            float2 *t0, *t1, *tables;
+            if ((rc = setup_scratch(ctx, &ps, 1, DIR, 1, &t0, &t1, &tables))) break;
+            if (real_in)
+                rc = launch_pass<DIR, float, false>(ctx->stream, ps, src, dst, tables, ctx->twiddles);
+            else if (real_out)
+                rc = launch_pass<DIR, float2, true>(ctx->stream, ps, src, dst, tables, ctx->twiddles);
+            else
+                rc = launch_pass<DIR, float2, false>(ctx->stream, ps, src, dst, tables, ctx->twiddles);
+        } else if (real_in) {
+            rc = long_axis<DIR, float, false>(ctx, a, src, Ss[p], Ds[p], Es[p], nin[p], m[a], nout[p], scale, dst);
+        } else if (real_out) {
+            rc = long_axis<DIR, float2, true>(ctx, a, src, Ss[p], Ds[p], Es[p], nin[p], m[a], nout[p], scale, dst);
+        } else {
+            rc = long_axis<DIR, float2, false>(ctx, a, src, Ss[p], Ds[p], Es[p], nin[p], m[a], nout[p], scale, dst);
+        }
+        src = dst;
+    }
+    if (buf[0] || buf[1]) {
+        // the intermediates are the library's own: free them once the stream is done with them
+        (void) hipStreamSynchronize(ctx->stream);
+        if (buf[0]) ufomi_device_free(ctx, buf[0]);
+        if (buf[1]) ufomi_device_free(ctx, buf[1]);
+    }
+    return rc;
 }
 
 }  // namespace
@@ -487,6 +681,11 @@ extern "C" int ufomi_fft_nd(ufomi_ctx *ctx, const float *d_in, int in_complex, c
                           : split_transform<-1, float, false>(ctx, d_in, n[0], n[0], rows, m[0], m[0], m[0], 1.0f,
                                                               d_out);
     }
+
+    // an axis longer than one LDS pass: axis by axis through the long-axis path
+    for (int a = 0; a < dimensions; a++)
+        if (!fits_lds(m[a]))
+            return axis_by_axis<-1>(ctx, d_in, in_complex != 0, n, dimensions, m, m[0], m[1], d_out);
 
     // passes from axis 0 up; the last one writes the output extents on every axis
     PassSpec ps[3];
@@ -542,6 +741,10 @@ extern "C" int ufomi_ifft_nd(ufomi_ctx *ctx, const float *d_in, const int *m, in
     if (dimensions == 1 && pow2(m[0]) && m[0] > kMaxL && m[0] <= kMaxSplit && (ch == m[1] || m[2] == 1))
         return split_transform<+1, float2, true>(ctx, d_in, m[0], m[0], ch * m[2], m[0], cw, cw,
                                                  (float) (1.0 / m[0]), d_out);
+
+    for (int a = 0; a < dimensions; a++)
+        if (!fits_lds(m[a]))
+            return axis_by_axis<+1>(ctx, d_in, true, m, dimensions, m, cw, ch, d_out);
 
     // passes from the highest transformed axis down to axis 0, which writes the real, cropped rows
     PassSpec ps[3];
