@@ -51,19 +51,21 @@ def log(msg):
 
 
 def measure_lds_peak(ctx, torch):
-    """Best of 5 runs of the conflict-free ds_read_b128 microbenchmark (single runs vary by ~20%
-    with clocks and neighbours on the node)."""
+    """Median of 7 runs of the conflict-free ds_read_b128 microbenchmark, each long enough (16384
+    iterations on 8192 workgroups) that clock ramps and neighbours on the node average out."""
     sink = torch.empty(8192 * 256, device="cuda")
-    ctx.bench_lds_read(8192, 64, sink)  # warm-up
-    best = 0.0
-    for _ in range(5):
+    ctx.bench_lds_read(8192, 256, sink)  # warm-up
+    rates = []
+    for _ in range(7):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        nbytes = ctx.bench_lds_read(8192, 4096, sink)
+        nbytes = ctx.bench_lds_read(8192, 16384, sink)
         e1.record()
         torch.cuda.synchronize()
-        best = max(best, nbytes / (e0.elapsed_time(e1) * 1e-3))
-    return best
+        rates.append(nbytes / (e0.elapsed_time(e1) * 1e-3))
+    rates.sort()
+    log("LDS read peak runs (TB/s): %s" % " ".join("%.1f" % (v / 1e12) for v in rates))
+    return rates[len(rates) // 2]
 
 
 def measure_hbm_copy(ctx, torch):
@@ -82,14 +84,48 @@ def measure_hbm_copy(ctx, torch):
     return bw
 
 
-def traffic_for(key):
+def traffic_entry(key):
+    """Counter figures of one kernel and workload from profiles/traffic.json: HBM bytes per launch
+    (rocprof FETCH_SIZE and WRITE_SIZE, scaled) and VALU instructions per voxel update (SQ_INSTS_VALU)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
-        table = json.load(f)
-    entry = table.get(key)
-    return None if entry is None else entry["fetch_bytes"] + entry["write_bytes"]
+        return json.load(f).get(key, {})
+
+
+def traffic_for(key):
+    entry = traffic_entry(key)
+    return entry["fetch_bytes"] + entry["write_bytes"] if "fetch_bytes" in entry else None
+
+
+# VALU issue capacity: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles of 2.4 GHz,
+# i.e. 16 lane-operations per SIMD and cycle
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
+
+
+def want_accuracy(args):
+    """The FP64 error probes run with the CPU baseline, and alone for the secondary workloads."""
+    return not args.no_cpu_baseline or getattr(args, "accuracy_only", False)
+
+
+def want_cpu(args):
+    return not args.no_cpu_baseline and not getattr(args, "accuracy_only", False)
+
+
+# The reference's own OpenCL kernels cannot serve as the CPU baseline here: the GPU box's OpenCL
+# platform (AMD APP, clinfo) lists the GPU only and no CPU ICD is installed (probe summary in
+# profiles/host_opencl_probe.md).  The baseline is the float32 C restatement of those kernels.
+CPU_NOTE = ("CPU OpenCL baseline unavailable: the host's only OpenCL platform (AMD APP) lists the GPU alone, "
+            "no CPU ICD (probed with clinfo); reported instead: the float32 C restatement of the reference's "
+            "kernels, OpenMP on the host's cores")
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def oracle_module(threads):
@@ -201,11 +237,12 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
                      "bp_kernel": kname(st)}
                 for st, r in runs.items()}
     cpu = accuracy = None
-    if not args.no_cpu_baseline:
+    if want_accuracy(args):
         threads = min(16, os.cpu_count() or 1)
         n_cpu = min(8 if S > 1 else 1, S)
         some = sino[probe:probe + n_cpu].cpu().numpy()
         del sino, vol
+        torch.cuda.empty_cache()
         oracle = oracle_module(threads)
         # error of every storage against the FP64 oracle (FP64 fft-filter-ifft + FP64 backprojection)
         t0 = time.perf_counter()
@@ -217,20 +254,21 @@ def fbp_parallel(args, ctx, torch, np, ufomi, S, W, P, filt_name, noise, storage
             variants[st]["rel_l2_vs_fp64"] = float(oracle.rel_l2(r["slice"], ref))
             accuracy[st] = variants[st]["rel_l2_vs_fp64"]
         log("FP64 oracle slice %.1f s; rel-L2 %s" % (time.perf_counter() - t0, accuracy))
+    if want_cpu(args):
         log("cpu baseline: oracle on %d slices (%d threads)" % (n_cpu, threads))
         coef_o = oracle.filter_coefficients(2 * N, filt_name)
         reps = 1 if S > 1 else 10
         t0 = time.perf_counter()
         for _ in range(reps):
             for s in some:
-                f32 = oracle.ramp_chain(s, N, coef_o, W).astype(np.float32)
+                f32 = oracle.ramp_chain_f32(s, N, coef_o, W)
                 oracle.bp_texture_ref32(f32, axis_pos=axis)
         dt = time.perf_counter() - t0
         cpu = {"value": round(reps * n_cpu * W * W * P / dt / 1e9, 3), "unit": "GUPS", "cores": threads,
-               "kind": "port",
-               "sample": "%d of %d slices (%dx%d px, %d angles each)%s: FP64 fft-filter-ifft + float32-semantics "
-                         "backprojection of the C oracle, %.1f s" % (n_cpu, S, W, W, P,
-                                                                     " x %d repeats" % reps if reps > 1 else "", dt)}
+               "host_cores": host_cores(), "kind": "port", "note": CPU_NOTE,
+               "sample": "%d of %d slices (%dx%d px, %d angles each)%s: float32 fft-filter-ifft + float32 "
+                         "backprojection (backproject.cl semantics) of the C restatement, %.1f s"
+                         % (n_cpu, S, W, W, P, " x %d repeats" % reps if reps > 1 else "", dt)}
     workload = ("parallel-beam FBP: %d sinogram%s x %d angles x %d px -> %dx%dx%d volume, %s ramp zero-padded "
                 "to %d, %s filtered-sinogram storage" % (S, "s" if S > 1 else "", P, W, S, W, W, filt_name, N, first))
     if len(storages) > 1:
@@ -299,7 +337,7 @@ def general(args, ctx, torch, np, ufomi, kind):
     mode = (1 if zsep else 2) if kind == "cone" else (3 if zsep else 0)
     kernel = "gbp_lds_kernel<%d,2,float>" % mode
     cpu = accuracy = None
-    if not args.no_cpu_baseline:
+    if want_accuracy(args):
         threads = min(16, os.cpu_count() or 1)
         oracle = oracle_module(threads)
         # probe: the central 64 x 64 x 4 voxels and 256 detector rows of the filter
@@ -311,6 +349,7 @@ def general(args, ctx, torch, np, ufomi, kind):
         got_rows = filt.reshape(P * H, W)[rows_idx].cpu().numpy()
         f_np = filt.cpu().numpy()
         del projs, weighted, filt, vol
+        torch.cuda.empty_cache()
         t0 = time.perf_counter()
         ref_rows = oracle.ramp_chain(src_rows, N, oracle.filter_coefficients(2 * N, "ramp-fromreal"), W)
         ref_sub = oracle.gbp(f_np, overall_angle=2 * math.pi, region=(0.0, float(probe_z), 1.0),
@@ -320,6 +359,7 @@ def general(args, ctx, torch, np, ufomi, kind):
                     "filter_rel_l2": float(oracle.rel_l2(got_rows, ref_rows)),
                     "backprojection_rel_l2": float(oracle.rel_l2(got_sub, ref_sub))}
         log("FP64 oracle probe %.1f s: %s" % (time.perf_counter() - t0, accuracy))
+    if want_cpu(args):
         sub = 64 if kind == "cone" else 16
         x_reg = (-128.0, 128.0, 1.0)
         t0 = time.perf_counter()
@@ -327,7 +367,8 @@ def general(args, ctx, torch, np, ufomi, kind):
                    y_region=x_reg, mode="ref32", **geom)
         dt = time.perf_counter() - t0
         n_up = sub * 256 * 256 * P
-        cpu = {"value": round(n_up / dt / 1e9, 4), "unit": "GUPS", "cores": threads, "kind": "port",
+        cpu = {"value": round(n_up / dt / 1e9, 4), "unit": "GUPS", "cores": threads, "host_cores": host_cores(),
+               "kind": "port", "note": CPU_NOTE,
                "sample": "central 256x256x%d voxels, all %d projections, float32-semantics general "
                          "backprojection of the C oracle, %.1f s" % (sub, P, dt)}
     desc = ("cone-beam FDK: %d projections of %dx%d, SOD %.0f px, SDD %.0f px, cosine weight + Ram-Lak (N %d) -> "
@@ -447,13 +488,13 @@ def batch(args, ctx, torch, np, ufomi):
     log("graph %.2f ms per replay; direct launches: filters %.2f ms, persistent backprojection %.2f ms"
         % (graph_ms, filt_ms, bp_ms))
     cpu = accuracy = None
-    if not args.no_cpu_baseline:
+    cost = [rw * rh * P for (W, P, rw, rh) in shapes]
+    if want_accuracy(args):
         threads = min(16, os.cpu_count() or 1)
         oracle = oracle_module(threads)
         b.run()
         torch.cuda.synchronize()
         # FP64 oracle on the two cheapest requests; the CPU baseline on requests in index order up to ~30 G updates
-        cost = [rw * rh * P for (W, P, rw, rh) in shapes]
         cheap = sorted(range(n), key=lambda i: cost[i])[:2]
         accuracy = {"vs": "FP64 C oracle (FP64 ramp-fromreal fft-filter-ifft + FP64 backprojection) of requests %s"
                           % cheap}
@@ -469,6 +510,7 @@ def batch(args, ctx, torch, np, ufomi):
             worst = max(worst, float(oracle.rel_l2(r["slice"].cpu().numpy(), ref)))
         accuracy["max_rel_l2"] = worst
         log("accuracy %s" % accuracy)
+    if want_cpu(args):
         sample, acc_up = [], 0.0
         for i in range(n):
             if acc_up >= 30e9:
@@ -481,12 +523,13 @@ def batch(args, ctx, torch, np, ufomi):
             N = 1
             while N < 2 * W:
                 N *= 2
-            f32 = oracle.ramp_chain(s_np, N, oracle.filter_coefficients(2 * N), W).astype(np.float32)
+            f32 = oracle.ramp_chain_f32(s_np, N, oracle.filter_coefficients(2 * N), W)
             oracle.bp_texture_ref32(f32, roi_x=r["roi_x"], roi_y=r["roi_y"], roi_width=rw, roi_height=rh)
         dt = time.perf_counter() - t0
-        cpu = {"value": round(acc_up / dt / 1e9, 3), "unit": "GUPS", "cores": threads, "kind": "port",
-               "sample": "requests 0..%d of %d (%.3g voxel updates): FP64 fft-filter-ifft + float32-semantics "
-                         "backprojection of the C oracle, %.1f s" % (len(sample) - 1, n, acc_up, dt)}
+        cpu = {"value": round(acc_up / dt / 1e9, 3), "unit": "GUPS", "cores": threads, "host_cores": host_cores(),
+               "kind": "port", "note": CPU_NOTE,
+               "sample": "requests 0..%d of %d (%.3g voxel updates): float32 fft-filter-ifft + float32 "
+                         "backprojection of the C restatement, %.1f s" % (len(sample) - 1, n, acc_up, dt)}
     b.close()
     kernel = "bp_batch_kernel<%d,%d,float>" % (tile, chunk)
     extra = {"graph_ms": round(graph_ms, 3), "filter_kernels_ms": round(filt_ms, 3),
@@ -659,6 +702,8 @@ def main():
                     help="filtered-sinogram storage; 'all' (the fbp2048 default) reports f32 as the value "
                          "and f16 / bf16 beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="fbp2048: leave out the secondary configs (512^2 slice, uneven batch, cone, laminography)")
     ap.add_argument("--bp-config", default="", help="S,A,H: slices per tile, angle chunk, f16 half staging "
                                                      "(ufomi_bp_configure; default automatic)")
     ap.add_argument("--views", type=int, default=120, help="learned: sparse view count (divides 1800)")
@@ -721,33 +766,9 @@ def main():
 
     lds_peak = measure_lds_peak(ctx, torch)
     hbm_copy = measure_hbm_copy(ctx, torch)
-    bp_s = r["bp_ms"] * 1e-3
-    achieved = r["updates"] * r["bytes_per_update"] / bp_s
-    traffic = traffic_for(r["traffic_key"])
-    log("%s: %.1f GUPS kernel, gather %.1f TB/s vs %.1f TB/s LDS peak, HBM copy %.2f TB/s"
-        % (r["kernel"], r["updates"] / bp_s / 1e9, achieved / 1e12, lds_peak / 1e12, hbm_copy / 1e12))
-    roof = {
-        "kernel": r["kernel"],
-        "bound": "lds",
-        "achieved": round(achieved / 1e9, 1),
-        "peak": round(lds_peak / 1e9, 1),
-        "unit": "GB/s",
-        "frac": round(achieved / lds_peak, 4),
-        "traffic": traffic,
-        "algorithmic_bytes_per_update": r["bytes_per_update"],
-        "kernel_ms": round(r["bp_ms"], 3),
-        "kernel_gups": round(r["updates"] / bp_s / 1e9, 1),
-        "peak_source": "measured: ufomi_bench_lds_read (ds_read_b128 on every CU)",
-        "hbm_copy_GBps": round(hbm_copy / 1e9, 1),
-        # the HBM-compulsory fraction beside the on-chip one: inputs and output once per launch
-        "hbm_compulsory_bytes": r["hbm_bytes"],
-        "hbm_compulsory_GBps": round(r["hbm_bytes"] / bp_s / 1e9, 1),
-        "hbm_compulsory_frac_of_8TBps": round(r["hbm_bytes"] / bp_s / 8e12, 5),
-    }
+    roof = roofline_of(r, lds_peak, hbm_copy)
     variants = r["extra"].pop("storage_variants", None)
     roof.update(r["extra"])
-    if traffic:
-        roof["hbm_achieved_GBps"] = round(traffic / bp_s / 1e9, 1)
     result = {
         "metric": r["metric"],
         "value": round(r["updates"] / (r["wall"] / args.steps) / 1e9, 2),
@@ -770,8 +791,83 @@ def main():
         result["accuracy"] = r["accuracy"]
     if r["cpu"] is not None:
         result["cpu_baseline"] = r["cpu"]
+    if args.workload == "fbp2048" and not args.no_other_configs:
+        result["other_configs"] = other_configs(args, ctx, torch, np, ufomi, lds_peak, hbm_copy)
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def roofline_of(r, lds_peak, hbm_copy):
+    """Roofline of a workload's dominant backprojection kernel: the LDS gather rate against the
+    measured ds_read_b128 peak (algorithmic bytes per update x updates / kernel time), the VALU
+    issue rate the counters imply, and the HBM-compulsory and counted HBM rates."""
+    bp_s = r["bp_ms"] * 1e-3
+    gups = r["updates"] / bp_s
+    achieved = r["updates"] * r["bytes_per_update"] / bp_s
+    entry = traffic_entry(r["traffic_key"])
+    traffic = traffic_for(r["traffic_key"])
+    log("%s: %.1f GUPS kernel, gather %.1f TB/s vs %.1f TB/s LDS peak, HBM copy %.2f TB/s"
+        % (r["kernel"], gups / 1e9, achieved / 1e12, lds_peak / 1e12, hbm_copy / 1e12))
+    roof = {
+        "kernel": r["kernel"],
+        "bound": "lds",
+        "achieved": round(achieved / 1e9, 1),
+        "peak": round(lds_peak / 1e9, 1),
+        "unit": "GB/s",
+        "frac": round(achieved / lds_peak, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_update": r["bytes_per_update"],
+        "kernel_ms": round(r["bp_ms"], 3),
+        "kernel_gups": round(gups / 1e9, 1),
+        "peak_source": "measured: ufomi_bench_lds_read (ds_read_b128 on every CU), median of 7 runs",
+        "hbm_copy_GBps": round(hbm_copy / 1e9, 1),
+        # the HBM-compulsory fraction beside the on-chip one: inputs and output once per launch
+        "hbm_compulsory_bytes": r["hbm_bytes"],
+        "hbm_compulsory_GBps": round(r["hbm_bytes"] / bp_s / 1e9, 1),
+        "hbm_compulsory_frac_of_8TBps": round(r["hbm_bytes"] / bp_s / 8e12, 5),
+    }
+    if traffic:
+        roof["hbm_achieved_GBps"] = round(traffic / bp_s / 1e9, 1)
+    valu = entry.get("valu_insts_per_update")
+    if valu:
+        # what binds these gathers: VALU issue.  Instructions per update from SQ_INSTS_VALU (profiles/)
+        roof["valu_insts_per_update"] = valu
+        roof["valu_ceiling_gups"] = round(VALU_LANE_OPS / valu / 1e9, 1)
+        roof["valu_issue_frac"] = round(gups * valu / VALU_LANE_OPS, 4)
+    return roof
+
+
+def other_configs(args, ctx, torch, np, ufomi, lds_peak, hbm_copy):
+    """Configs 0, 2 and 3 after the headline one: a few timed steps each, the kernel's rooflines and the
+    FP64 error probe (no CPU baselines, which `--workload <name>` runs report)."""
+    import copy
+    sub = copy.copy(args)
+    sub.steps, sub.warmup, sub.accuracy_only, sub.no_cpu_baseline = min(args.steps, 5), min(args.warmup, 2), True, True
+    out = {}
+    torch.cuda.empty_cache()
+    for name in ("fbp512", "batch", "cone", "lamino"):
+        log("secondary workload %s" % name)
+        t0 = time.perf_counter()
+        if name == "fbp512":
+            r = fbp_parallel(sub, ctx, torch, np, ufomi, 1, 512, 512, "ramp-fromreal", 0.01, ["f32"])
+        elif name == "batch":
+            r = batch(sub, ctx, torch, np, ufomi)
+        else:
+            r = general(sub, ctx, torch, np, ufomi, name)
+        roof = roofline_of(r, lds_peak, hbm_copy)
+        out[name] = {
+            "workload": r["workload"], "metric": r["metric"],
+            "value": round(r["updates"] / (r["wall"] / sub.steps) / 1e9, 2), "unit": "GUPS",
+            "steps": sub.steps, "ms_per_step": round(r["wall"] / sub.steps * 1e3, 3),
+            "roofline": {k: roof[k] for k in ("kernel", "bound", "achieved", "peak", "unit", "frac", "traffic",
+                                              "kernel_ms", "kernel_gups", "valu_insts_per_update",
+                                              "valu_ceiling_gups", "valu_issue_frac", "hbm_compulsory_frac_of_8TBps")
+                         if k in roof},
+            "accuracy": r.get("accuracy"),
+            "seconds": round(time.perf_counter() - t0, 1),
+        }
+        torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

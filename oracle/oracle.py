@@ -57,6 +57,7 @@ def lib():
         L.orc_filter_task.argtypes = [_f64p, c_int, c_int, _f32p]
         L.orc_ifft_task.argtypes = [_f64p, c_int, c_int, c_int, _f64p]
         L.orc_ramp_chain.argtypes = [_f32p, c_int, c_int, c_int, _f32p, c_int, _f64p]
+        L.orc_ramp_chain_f32.argtypes = [_f32p, c_int, c_int, c_int, _f32p, c_int, _f32p]
         L.orc_bp_lut.argtypes = [c_double, c_double, c_int, _f32p, _f32p]
         L.orc_bp_texture_ref32.argtypes = [_f32p, c_int, c_int, _f32p, _f32p, c_uint, c_float,
                                            c_uint, c_uint, c_int, c_int, _f32p]
@@ -180,6 +181,16 @@ def ramp_chain(sino, n, coef, crop=0):
     crop = crop or n
     out = np.empty((p, crop), dtype=np.float64)
     lib().orc_ramp_chain(sino, w, p, n, np.ascontiguousarray(coef, dtype=np.float32), crop, out)
+    return out
+
+
+def ramp_chain_f32(sino, n, coef, crop=0):
+    """The same chain in float32 throughout (radix-2 transforms, rows in parallel) -> [P][crop] float32."""
+    sino = np.ascontiguousarray(sino, dtype=np.float32)
+    p, w = sino.shape
+    crop = crop or n
+    out = np.empty((p, crop), dtype=np.float32)
+    lib().orc_ramp_chain_f32(sino, w, p, n, np.ascontiguousarray(coef, dtype=np.float32), crop, out)
     return out
 
 

@@ -309,6 +309,73 @@ void orc_ramp_chain(const float *in, int W, int P, int N, const float *coef, int
     free (tmp);
 }
 
+/*
+ * The same chain in float32 throughout (the CPU baseline's restatement; the
+ * reference's oclfft runs float32 transforms): per row, zero padding to N
+ * complex values, an iterative radix-2 forward transform with float twiddles
+ * (FP64 sin/cos rounded once), the coefficient product, the inverse transform,
+ * Re / N cropped.  Rows run in parallel.  Checked against orc_ramp_chain.
+ */
+void orc_ramp_chain_f32(const float *in, int W, int P, int N, const float *coef, int crop, float *out)
+{
+    float *twr = malloc (sizeof (float) * (size_t) (N / 2)), *twi = malloc (sizeof (float) * (size_t) (N / 2));
+    for (int k = 0; k < N / 2; k++) {
+        twr[k] = (float) cos (2.0 * M_PI * k / N);
+        twi[k] = (float) sin (2.0 * M_PI * k / N);
+    }
+    #pragma omp parallel
+    {
+        float *re = malloc (sizeof (float) * (size_t) N), *im = malloc (sizeof (float) * (size_t) N);
+        #pragma omp for schedule(static)
+        for (int r = 0; r < P; r++) {
+            for (int pass = 0; pass < 2; pass++) {
+                const float dir = pass == 0 ? -1.0f : 1.0f;
+                if (pass == 0) {
+                    for (int i = 0; i < N; i++) {
+                        re[i] = i < W ? in[(size_t) r * W + i] : 0.0f;
+                        im[i] = 0.0f;
+                    }
+                } else {
+                    for (int i = 0; i < N; i++) {  /* filter: interleaved coefficients (re, im) of bin i */
+                        re[i] *= coef[2 * i];
+                        im[i] *= coef[2 * i + 1];
+                    }
+                }
+                for (int i = 1, j = 0; i < N; i++) {
+                    int bit = N >> 1;
+                    for (; j & bit; bit >>= 1)
+                        j ^= bit;
+                    j ^= bit;
+                    if (i < j) {
+                        float t = re[i]; re[i] = re[j]; re[j] = t;
+                        t = im[i]; im[i] = im[j]; im[j] = t;
+                    }
+                }
+                for (int len = 2; len <= N; len <<= 1) {
+                    const int half = len >> 1, step = N / len;
+                    for (int i = 0; i < N; i += len) {
+                        for (int k = 0; k < half; k++) {
+                            const float wr = twr[k * step], wi = dir * twi[k * step];
+                            const float xr = re[i + k + half], xi = im[i + k + half];
+                            const float vr = xr * wr - xi * wi, vi = xr * wi + xi * wr;
+                            re[i + k + half] = re[i + k] - vr;
+                            im[i + k + half] = im[i + k] - vi;
+                            re[i + k] += vr;
+                            im[i + k] += vi;
+                        }
+                    }
+                }
+            }
+            for (int i = 0; i < crop; i++)
+                out[(size_t) r * crop + i] = re[i] / (float) N;
+        }
+        free (re);
+        free (im);
+    }
+    free (twr);
+    free (twi);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Parallel-beam backprojection: src/ufo-backproject-task.c, backproject.cl  */
 /* ------------------------------------------------------------------------- */

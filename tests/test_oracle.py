@@ -155,6 +155,18 @@ def test_ramp_chain_vs_numpy():
 
 # ------------------------------------------------------------ backprojection
 
+@pytest.mark.parametrize("filt", ["ramp-fromreal", "shepp-logan", "faris-byer"])
+def test_ramp_chain_f32_vs_f64(filt):
+    # the CPU baseline's float32 chain against the FP64 chain (both restate fft ! filter ! ifft)
+    rng = np.random.default_rng(4)
+    sino = rng.standard_normal((9, 300)).astype(np.float32)
+    coef = oracle.filter_coefficients(1024, filt)
+    f64 = oracle.ramp_chain(sino, 512, coef, 300)
+    f32 = oracle.ramp_chain_f32(sino, 512, coef, 300)
+    assert f32.dtype == np.float32 and f32.shape == (9, 300)
+    assert oracle.rel_l2(f32, f64) < 1e-6
+
+
 def test_bp_vs_brute_force_golden():
     sino = GOLDEN["bp_brute_sino"]
     axis = float(GOLDEN["bp_brute_axis"])
