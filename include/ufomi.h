@@ -190,6 +190,22 @@ int ufomi_bp_parallel(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int 
                       float *d_out, size_t out_slice_pitch);
 
 /*
+ * ufomi_bp_parallel without any device allocation: same arguments, kernels
+ * and results, but the (sin, cos) table (computed on the device in FP64 and
+ * rounded to float, as the LUT of ufomi_bp_parallel) and the partial volumes
+ * of an angle split live in one preallocated array per GPU (64.25 MiB, held
+ * until the process ends), and no host-to-device copy is made.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Limit: P_in <= 32768 (UFOMI_ERR_SIZE beyond); every other
+ * shape ufomi_bp_parallel takes fits.  All calls on one GPU share the array,
+ * so calls that overlap on one GPU (other streams, other threads) race.
+ */
+int ufomi_bp_parallel_noalloc(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in,
+                              int n_slices, size_t sino_slice_pitch, const ufomi_bp_params *params,
+                              float *d_out, size_t out_slice_pitch);
+
+/*
  * `stacked-backproject` (src/ufo-stacked-backproject-task.c:300-430,
  * src/kernels/stacked-backproject.cl): n_slices float32 sinograms [P_in][W]
  * (consecutive) -> slices [roi_h][roi_w] with the `backproject` LUT, axis and

@@ -24,8 +24,9 @@
 #include "ufomi_internal.h"
 
 #include <cmath>
-#include <type_traits>
 #include <cstring>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -506,6 +507,45 @@ __global__ __launch_bounds__(kThreads) void bp_nearest_kernel(
     out[(long) sl * out_pitch + (long) idy * roi_w + idx] = r / (float) P;
 }
 
+// ---- preallocated device memory of ufomi_bp_parallel_noalloc
+//
+// One file-scope array per GPU, held until the process ends: the (sin, cos)
+// table of up to 32768 angles (the range of the num-projections property),
+// then the partial volumes of an angle-split launch.  A split happens only
+// below 1024 workgroups (nb tiles x slice groups), with n_split <= 1024 / nb + 1
+// parts of at most S * 32 * 32 <= 8192 voxels per workgroup, so the partial
+// volumes never exceed 2047 * 8192 floats < 64 MiB.
+constexpr size_t kNoallocTrigMax = 32768;
+constexpr size_t kNoallocTrigBytes = kNoallocTrigMax * sizeof(float2);
+constexpr size_t kNoallocPartBytes = (size_t) 64 << 20;
+constexpr int kNoallocDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoallocTrigBytes + kNoallocPartBytes];
+
+// Host-side table of each device's address of `scratch` (hipGetSymbolAddress).
+unsigned char *g_noalloc_ptr[kNoallocDevices];
+std::mutex g_noalloc_mutex;
+
+// The LUT of ufo-backproject-task.c:174-196 computed on the device: FP64
+// sin/cos of angle_offset + (offset + i) step rounded to float (sin_sign -1
+// stores -sin).  The device's FP64 sin/cos are within an ulp of the host's,
+// so the rounded floats agree except when the FP64 value lies within an ulp
+// of a float rounding boundary.
+__global__ __launch_bounds__(kThreads) void trig_table_kernel(float2 *__restrict__ trig, int P, double angle_offset,
+                                                              double step, unsigned offset, int sin_sign)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= P)
+        return;
+    const double a = angle_offset + (double) (offset + (unsigned) i) * step;
+    const float sv = (float) sin(a);
+    trig[i] = make_float2(sin_sign < 0 ? -sv : sv, (float) cos(a));
+}
+
+int bp_parallel_impl(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in, int n_slices,
+                     size_t sino_pitch, const ufomi_bp_params *prm, float *d_out, size_t out_pitch, int sin_sign,
+                     bool half_staging, unsigned char *fixed);
+
 }  // namespace
 
 // LUT of ufo-backproject-task.c:174-196: FP64 sin/cos of angle_offset + i*step,
@@ -574,6 +614,22 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
                          size_t sino_pitch, const ufomi_bp_params *prm, float *d_out, size_t out_pitch, int sin_sign,
                          bool half_staging)
 {
+    return bp_parallel_impl(ctx, d_sino, dtype, W, P_in, n_slices, sino_pitch, prm, d_out, out_pitch, sin_sign,
+                            half_staging, nullptr);
+}
+
+namespace {
+
+// One body for both entry points.  `fixed` = nullptr: the (sin, cos) table is
+// the context's cached LUT and angle-split partial volumes live in the
+// context's grow-only buffer.  Otherwise `fixed` is this device's address of
+// the file-scope `scratch` array: the table is computed into its first
+// kNoallocTrigBytes by a kernel and the partial volumes follow it, so nothing
+// is allocated or copied from the host.
+int bp_parallel_impl(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in, int n_slices,
+                     size_t sino_pitch, const ufomi_bp_params *prm, float *d_out, size_t out_pitch, int sin_sign,
+                     bool half_staging, unsigned char *fixed)
+{
     int rc = ufomi_enter(ctx);
     if (rc) return rc;
     if (!d_sino || !d_out || !prm || W <= 0 || P_in <= 0 || n_slices <= 0)
@@ -610,7 +666,18 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
         return ufomi_fail(UFOMI_ERR_ARG, "a sinogram of %d x %d exceeds 4 GiB", P_in, W);
 
     float2 *trig = nullptr;
-    if ((rc = ufomi_bp_trig(ctx, prm->angle_offset, step, prm->offset, P_in, sin_sign, &trig))) return rc;
+    if (fixed) {
+        if ((size_t) P_in > kNoallocTrigMax)
+            return ufomi_fail(UFOMI_ERR_SIZE,
+                              "sinogram height %d exceeds the %zu angles of the preallocated backprojection scratch",
+                              P_in, kNoallocTrigMax);
+        trig = (float2 *) fixed;
+        hipLaunchKernelGGL(trig_table_kernel, dim3((P_in + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                           ctx->stream, trig, P_in, prm->angle_offset, step, prm->offset, sin_sign);
+        UFOMI_CHECK_LAUNCH();
+    } else if ((rc = ufomi_bp_trig(ctx, prm->angle_offset, step, prm->offset, P_in, sin_sign, &trig))) {
+        return rc;
+    }
 
     if (prm->mode == UFOMI_BP_NEAREST) {
         if (dtype != UFOMI_F32)
@@ -651,7 +718,14 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
     }
     const long plane = (long) roi_w * roi_h;
     float *part = nullptr;
-    if (n_split > 1) {
+    if (n_split > 1 && fixed) {
+        const size_t bytes = (size_t) n_split * n_slices * plane * sizeof(float);
+        if (bytes > kNoallocPartBytes)
+            return ufomi_fail(UFOMI_ERR_SIZE,
+                              "angle-split partial volumes of %zu bytes exceed the %zu bytes of the preallocated "
+                              "backprojection scratch", bytes, kNoallocPartBytes);
+        part = (float *) (fixed + kNoallocTrigBytes);
+    } else if (n_split > 1) {
         const size_t bytes = (size_t) n_split * n_slices * plane * sizeof(float);
         if (bytes > ctx->bp_part_bytes) {
             if (ctx->bp_part) {
@@ -707,6 +781,44 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
     case UFOMI_BF16: return by_s((const __hip_bfloat16 *) d_sino);
     default: return by_s((const float *) d_sino);
     }
+}
+
+}  // namespace
+
+// Set-up of ufomi_bp_parallel_noalloc, called by ufomi_create with `device`
+// current: this device's address of `scratch`, taken once per device.
+int ufomi_bp_noalloc_setup(int device)
+{
+    if (device < 0 || device >= kNoallocDevices)
+        return ufomi_fail(UFOMI_ERR_ARG, "device %d beyond the %d devices of the preallocated scratch table", device,
+                          kNoallocDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[device])
+        return UFOMI_OK;
+    void *ptr = nullptr;
+    UFOMI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr)
+        return ufomi_fail(UFOMI_ERR_HIP, "hipGetSymbolAddress returned no address for the backprojection scratch");
+    g_noalloc_ptr[device] = (unsigned char *) ptr;
+    return UFOMI_OK;
+}
+
+extern "C" int ufomi_bp_parallel_noalloc(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, int P_in,
+                                         int n_slices, size_t sino_pitch, const ufomi_bp_params *prm, float *d_out,
+                                         size_t out_pitch)
+{
+    if (!ctx) return ufomi_fail(UFOMI_ERR_ARG, "context is NULL");
+    if (ctx->device < 0 || ctx->device >= kNoallocDevices)
+        return ufomi_fail(UFOMI_ERR_ARG, "device %d beyond the preallocated scratch table", ctx->device);
+    unsigned char *fixed;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        fixed = g_noalloc_ptr[ctx->device];
+    }
+    if (!fixed)
+        return ufomi_fail(UFOMI_ERR_ARG, "the preallocated scratch of device %d was not set up (ufomi_create)",
+                          ctx->device);
+    return bp_parallel_impl(ctx, d_sino, dtype, W, P_in, n_slices, sino_pitch, prm, d_out, out_pitch, +1, true, fixed);
 }
 
 // ---- persistent batch kernel launchers

@@ -108,6 +108,12 @@ extern "C" int ufomi_create(int device, ufomi_ctx **out)
         free(ctx);
         return ufomi_fail(UFOMI_ERR_HIP, "twiddle table upload failed");
     }
+    if (int rc = ufomi_bp_noalloc_setup(device)) {
+        (void) hipFree(ctx->flags);
+        (void) hipFree(ctx->twiddles);
+        free(ctx);
+        return rc;
+    }
     *out = ctx;
     return UFOMI_OK;
 }

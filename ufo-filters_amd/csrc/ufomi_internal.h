@@ -80,6 +80,10 @@ int ufomi_bp_parallel_ex(ufomi_ctx *ctx, const void *d_sino, int dtype, int W, i
                          size_t sino_pitch, const ufomi_bp_params *prm, float *d_out, size_t out_pitch, int sin_sign,
                          bool half_staging);
 
+// Takes the calling thread's current device's address of the preallocated
+// scratch of ufomi_bp_parallel_noalloc (once per device; ufomi_create).
+int ufomi_bp_noalloc_setup(int device);
+
 // ---- batched FBP (batch.hip) over the kernels of fft_filter.hip and bp_parallel.hip
 
 // One segment of a segmented ramp-filter launch: `rows` rows of width W at `in`

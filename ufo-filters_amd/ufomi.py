@@ -43,6 +43,7 @@ EXPORTS = [
     "ufomi_unet_create", "ufomi_unet_destroy", "ufomi_unet_num_params", "ufomi_unet_num_tensors", "ufomi_unet_tensor",
     "ufomi_unet_get_params", "ufomi_unet_set_params", "ufomi_unet_forward", "ufomi_unet_gradients",
     "ufomi_unet_train_step", "ufomi_nn_conv3x3", "ufomi_bench_mfma", "ufomi_bp_configure",
+    "ufomi_bp_parallel_noalloc",
 ]
 
 
@@ -125,6 +126,7 @@ def load():
         "ufomi_ifft_rows": [vp, vp, ci, ci, ci, vp], "ufomi_ramp_filter": [vp, vp, ci, ci, ci, vp, ci, vp, ci],
         "ufomi_bp_params_init": [ctypes.POINTER(BpParams)],
         "ufomi_bp_parallel": [vp, vp, ci, ci, ci, ci, csz, ctypes.POINTER(BpParams), vp, csz],
+        "ufomi_bp_parallel_noalloc": [vp, vp, ci, ci, ci, ci, csz, ctypes.POINTER(BpParams), vp, csz],
         "ufomi_bp_stacked": [vp, vp, ci, ci, ci, ctypes.POINTER(BpParams), ci, vp],
         "ufomi_flat_field_correct": [vp, vp, ci, ci, ci, vp, ci, vp, ci, ci, ci, ci, ctypes.c_float, ctypes.c_float,
                                      vp],
@@ -390,6 +392,13 @@ class Context:
         p = bp_params(**props)
         check(self.L.ufomi_bp_parallel(self.h, _ptr(d_sino), dtype, W, P_in, n_slices, sino_pitch,
                                        ctypes.byref(p), _ptr(d_out), out_pitch))
+
+    def bp_parallel_noalloc(self, d_sino, W, P_in, d_out, n_slices=1, dtype=F32, sino_pitch=0, out_pitch=0,
+                            **props):
+        """bp_parallel on the library's preallocated device scratch (no allocation)."""
+        p = bp_params(**props)
+        check(self.L.ufomi_bp_parallel_noalloc(self.h, _ptr(d_sino), dtype, W, P_in, n_slices, sino_pitch,
+                                               ctypes.byref(p), _ptr(d_out), out_pitch))
 
     def flat_field_correct(self, d_data, W, H, n, d_dark, n_dark, d_flat, n_flat, d_out, sinogram_input=False,
                            absorption_correct=False, fix_nan_and_inf=False, dark_scale=1.0, flat_scale=1.0):
