@@ -186,7 +186,10 @@ template <int R, int DIR> __device__ __forceinline__ void dft(float2 *v)
 // entry t[i] = exp(-2 pi i i / M).
 template <int DIR> __device__ __forceinline__ float2 tw_at(const float2 *__restrict__ t, int idx)
 {
-    float2 w = t[idx];
+    // the table pointer is uniform: a 32-bit lane offset from it (no 64-bit address)
+    const uint64_t raw = *(const __attribute__((address_space(1))) uint64_t *) (
+        (const __attribute__((address_space(1))) char *) t + (unsigned) idx * 8u);
+    float2 w = make_float2(__uint_as_float((unsigned) raw), __uint_as_float((unsigned) (raw >> 32)));
     if (DIR > 0) w.y = -w.y;
     return w;
 }

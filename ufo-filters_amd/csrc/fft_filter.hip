@@ -182,6 +182,34 @@ int launch_by_b(int B, K k)
 // remain for N = 4 and 8.
 // ------------------------------------------------------------------------
 
+// Wave-uniform row pointers in SGPRs, so that a row's loads and stores take the
+// scalar-base + 32-bit lane-offset form (no 64-bit address per element).
+template <typename T> __device__ __forceinline__ T *uniform_row(T *p)
+{
+    const uint64_t v = (uint64_t) p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned) v), hi = __builtin_amdgcn_readfirstlane((unsigned) (v >> 32));
+    return (T *) (((uint64_t) hi << 32) | lo);
+}
+using GlbChar = __attribute__((address_space(1))) char;
+// Element i of a row (rows stay below 4 GiB): a 32-bit byte offset from its base.
+template <typename R> __device__ __forceinline__ R glb_ld(const void *base, unsigned i)
+{
+    return *(const __attribute__((address_space(1))) R *) ((const GlbChar *) base + i * (unsigned) sizeof(R));
+}
+template <typename R> __device__ __forceinline__ void glb_st(void *base, unsigned i, R v)
+{
+    *(__attribute__((address_space(1))) R *) ((GlbChar *) base + i * (unsigned) sizeof(R)) = v;
+}
+__device__ __forceinline__ void glb_store(float *p, unsigned i, float v) { glb_st<float>(p, i, v); }
+__device__ __forceinline__ void glb_store(__half *p, unsigned i, float v)
+{
+    glb_st<unsigned short>(p, i, __builtin_bit_cast(unsigned short, (_Float16) v));
+}
+__device__ __forceinline__ void glb_store(__hip_bfloat16 *p, unsigned i, float v)
+{
+    glb_st<unsigned short>(p, i, __builtin_bit_cast(unsigned short, __float2bfloat16(v)));
+}
+
 // Block `blk` of a ramp-filter launch over `rows` rows of width W: G transforms
 // (2 G rows when `pair`, the real-even coefficient case) of the rows' block.
 template <int N, typename OutT>
@@ -200,24 +228,43 @@ __device__ __forceinline__ void ramp16_block(
     const long r0 = pair ? 2 * unit : unit, r1 = r0 + 1;
     const bool v0 = r0 < rows, v1 = pair && r1 < rows;
     const float *a = in + r0 * W, *b = in + r1 * W;
+    OutT *o0 = out + r0 * crop, *o1 = out + r1 * crop;
 
+    // A transform of N >= 1024 points is whole waves (TF >= 64): its rows are
+    // wave-uniform, and elements are addressed by 32-bit offsets from them.
+    constexpr bool kUniform = P::TF >= 64;
+    const float *ua = nullptr, *ub = nullptr;
+    OutT *uo0 = nullptr, *uo1 = nullptr;
+    if constexpr (kUniform) {
+        ua = uniform_row(a);
+        ub = uniform_row(b);
+        uo0 = uniform_row(o0);
+        uo1 = uniform_row(o1);
+    }
     float2 v[P::V];
 #pragma unroll
     for (int s = 0; s < P::V; s++) {
         const int e = fft16::slot_index<N, R0>(t, s);
         float re = 0.0f, im = 0.0f;
         if (e < W) {
-            if (v0) re = a[e];
-            if (v1) im = b[e];
+            if constexpr (kUniform) {
+                if (v0) re = glb_ld<float>(ua, (unsigned) e);
+                if (v1) im = glb_ld<float>(ub, (unsigned) e);
+            } else {
+                if (v0) re = a[e];
+                if (v1) im = b[e];
+            }
         }
         v[s] = make_float2(re, im);
     }
     fft16::transform_forward_order<N, -1>(v, x, t, tw);
 #pragma unroll
     for (int s = 0; s < P::V; s++) {
-        const int e = fft16::slot_index<N, RL>(t, s);
-        const float cr = coef[2 * e];
-        const float ci = pair ? cr : coef[2 * e + 1];
+        const unsigned e = (unsigned) fft16::slot_index<N, RL>(t, s);
+        // (c[2e], c[2e + 1]) in one 8-byte load, no branch on `pair`
+        const uint64_t cc = glb_ld<uint64_t>(coef, e);
+        const float cr = __uint_as_float((unsigned) cc);
+        const float ci = pair ? cr : __uint_as_float((unsigned) (cc >> 32));
         v[s] = make_float2(v[s].x * cr, v[s].y * ci);
     }
     fft16::transform_reverse_order<N, +1>(v, x, t, tw);
@@ -225,8 +272,13 @@ __device__ __forceinline__ void ramp16_block(
     for (int s = 0; s < P::V; s++) {
         const int e = fft16::slot_index<N, R0>(t, s);
         if (e < crop) {
-            if (v0) ufomi_store(out, r0 * crop + e, v[s].x * scale);
-            if (v1) ufomi_store(out, r1 * crop + e, v[s].y * scale);
+            if constexpr (kUniform) {
+                if (v0) glb_store(uo0, (unsigned) e, v[s].x * scale);
+                if (v1) glb_store(uo1, (unsigned) e, v[s].y * scale);
+            } else {
+                if (v0) ufomi_store(o0, e, v[s].x * scale);
+                if (v1) ufomi_store(o1, e, v[s].y * scale);
+            }
         }
     }
 }
