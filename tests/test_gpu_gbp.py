@@ -98,6 +98,19 @@ def test_cone_beam(ctx, torch_dev):
            center_position_x=[12.0], center_position_z=[10.0], region=(-5.0, 5.0, 1.0))
 
 
+@pytest.mark.parametrize("geo", [
+    dict(axis_angle_x=[math.radians(30)]),                                   # laminography
+    dict(),                                                                  # untilted parallel beam
+    dict(source_position_y=[-200.0], detector_position_y=[100.0]),           # cone beam, magnification 1.5
+])
+def test_forty_slices(ctx, torch_dev, geo):
+    # 40 slices: several voxel tiles along z, the last one partial (8 per tile for parallel
+    # beam, 16 for cone beam), on the z-separable maps
+    projs = _smooth_projs(7, 24, 40, 48)
+    _check(ctx, torch_dev, projs, center_position_x=[24.0], center_position_z=[20.0], region=(-20.0, 20.0, 1.0),
+           **geo)
+
+
 @pytest.mark.parametrize("variant", [
     dict(detector_angle_x=[0.1], detector_angle_z=[0.05]),
     dict(detector_position_x=[1.5], detector_position_z=[-0.7]),

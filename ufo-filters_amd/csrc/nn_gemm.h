@@ -91,13 +91,36 @@ __global__ __launch_bounds__(256) void gemm_kernel(Op op)
             }
         }
     }
+    if constexpr (Op::kVecStore) {
+        // finished bf16 values through LDS, then 16-byte stores of 8 consecutive columns
+        __shared__ bf16x8 tile[BM * BN / 8];
+        bf16 *t = reinterpret_cast<bf16 *>(tile);
+        const int mw = (wave % WM) * MT * 16, nw = (wave / WM) * NT * 16;
 #pragma unroll
-    for (int i = 0; i < MT; i++)
+        for (int i = 0; i < MT; i++)
 #pragma unroll
-        for (int j = 0; j < NT; j++)
+            for (int j = 0; j < NT; j++)
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                op.store(m0 + i * 16 + 4 * q + e, n0 + j * 16 + r, acc[i][j][e], blockIdx.z);
+                for (int e = 0; e < 4; e++) {
+                    const long m = m0 + i * 16 + 4 * q + e, n = n0 + j * 16 + r;
+                    t[(mw + i * 16 + 4 * q + e) * BN + nw + j * 16 + r] =
+                        (bf16) (op.valid(m, n) ? op.finish(m, n, acc[i][j][e]) : 0.0f);
+                }
+        __syncthreads();
+        const long mb = (long) blockIdx.x * BM, nb = (long) blockIdx.y * BN;
+        for (int v = threadIdx.x; v < BM * BN / 8; v += 256) {
+            const long m = mb + v / (BN / 8), n = nb + (v % (BN / 8)) * 8;
+            if (op.valid(m, n)) op.store8(m, n, tile[v]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < MT; i++)
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    op.store(m0 + i * 16 + 4 * q + e, n0 + j * 16 + r, acc[i][j][e], blockIdx.z);
+    }
 }
 
 // A tensor of pixels in NHWC order: element (pixel p, channel c) at
@@ -126,6 +149,7 @@ __device__ __forceinline__ Pix pix_of(long p, int lh, int lw)
 // zero padding, K ordered (tap, c); cin a multiple of 32.  With mask, channels
 // n >= mask_from are zeroed where mask[p][n] <= 0 (a ReLU derivative).
 struct Conv3 {
+    static constexpr bool kVecStore = false;
     View<const bf16> in;
     int lh, lw, lcin, cout;
     long npix;
@@ -135,6 +159,7 @@ struct Conv3 {
     View<const bf16> mask;  // mask.ptr nullptr: none
     int mask_from;
     View<bf16> out;
+    View<bf16> pool;        // pool.ptr not nullptr: also the 2x2 max pool of out (LDS kernel only)
 
     struct RowA {
         long p;
@@ -179,13 +204,18 @@ struct Conv3 {
     {
         return c.ok ? ld8(c.w + k) : zero8();
     }
-    __device__ __forceinline__ void store(long m, long n, float v, int) const
+    // bias, ReLU and mask of output (m, n) (m < npix, n < cout)
+    __device__ __forceinline__ float finish(long m, long n, float v) const
     {
-        if (m >= npix || n >= cout) return;
         if (bias) v += bias[n];
         if (relu) v = fmaxf(v, 0.0f);
         if (mask.ptr && n >= mask_from && !((float) *mask.at(m, (int) n) > 0.0f)) v = 0.0f;
-        *out.at(m, (int) n) = (bf16) v;
+        return v;
+    }
+    __device__ __forceinline__ void store(long m, long n, float v, int) const
+    {
+        if (m >= npix || n >= cout) return;
+        *out.at(m, (int) n) = (bf16) finish(m, n, v);
     }
 };
 
@@ -194,6 +224,8 @@ struct Conv3 {
 // pixels, N = 4 cout ((ab, co)), K = cin (multiple of 32).  Output grid is
 // (2h, 2w).
 struct UpFwd {
+    // outputs leave as 8 consecutive channels (cout a multiple of 8) of one child pixel
+    static constexpr bool kVecStore = true;
     View<const bf16> in;
     int lh, lw, lcin, lcout;
     long npix;
@@ -225,14 +257,27 @@ struct UpFwd {
         return r.ok ? ld8(in.at(r.p, (int) k)) : zero8();
     }
     __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const { return c.ok ? ld8(c.w + k) : zero8(); }
-    __device__ __forceinline__ void store(long m, long n, float v, int) const
+    __device__ __forceinline__ bool valid(long m, long n) const { return m < npix && n < (4L << lcout); }
+    __device__ __forceinline__ float finish(long, long n, float v) const
     {
-        if (m >= npix || n >= (4L << lcout)) return;
-        const int ab = (int) (n >> lcout), co = (int) (n & ((1 << lcout) - 1));
+        return v + bias[n & ((1 << lcout) - 1)];
+    }
+    __device__ __forceinline__ long child(long m, int ab) const
+    {
         const long img = m >> (lh + lw);
         const Pix px = pix_of(m, lh, lw);
-        const long child = (((img << (lh + 1)) + 2 * px.y + (ab >> 1)) << (lw + 1)) + 2 * px.x + (ab & 1);
-        *out.at(child, co) = (bf16) (v + bias[co]);
+        return (((img << (lh + 1)) + 2 * px.y + (ab >> 1)) << (lw + 1)) + 2 * px.x + (ab & 1);
+    }
+    __device__ __forceinline__ void store8(long m, long n, bf16x8 v) const
+    {
+        const int ab = (int) (n >> lcout), co = (int) (n & ((1 << lcout) - 1));
+        *reinterpret_cast<bf16x8 *>(out.at(child(m, ab), co)) = v;
+    }
+    __device__ __forceinline__ void store(long m, long n, float v, int) const
+    {
+        if (!valid(m, n)) return;
+        const int ab = (int) (n >> lcout), co = (int) (n & ((1 << lcout) - 1));
+        *out.at(child(m, ab), co) = (bf16) finish(m, n, v);
     }
 };
 
@@ -240,6 +285,7 @@ struct UpFwd {
 // masked by the ReLU of the forward input.  M = input pixels, N = cin,
 // K = 4 cout ((ab, co), cout a multiple of 32).
 struct UpDgrad {
+    static constexpr bool kVecStore = false;
     View<const bf16> g;
     int lh, lw, lcout, cin;
     long npix;
@@ -314,6 +360,7 @@ struct Partial {
 // M = cout, N = 9 cin ((tap, c)), K = pixels in groups of 8 along a row
 // (W a multiple of 8).  Pixel-strided operands: 8 scalar loads per fragment.
 struct Conv3Wgrad {
+    static constexpr bool kVecStore = false;
     View<const bf16> g;
     View<const bf16> in;
     int lh, lw, cin, cout;
@@ -369,6 +416,7 @@ struct Conv3Wgrad {
 // Weight gradient of UpFwd: dW[ab][co][c] = sum_p g[child(p, ab)][co] in[p][c].
 // M = 4 cout ((ab, co)), N = cin, K = input pixels (groups of 8 along a row).
 struct UpWgrad {
+    static constexpr bool kVecStore = false;
     View<const bf16> g;
     View<const bf16> in;
     int lh, lw, lcout, cin;
@@ -432,7 +480,10 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
     constexpr int TX = 64, TY = 4, PX = TX + 2, PY = TY + 2, MT = 4, BN = NT * 16;
     constexpr int NVEC = PY * PX * 4;                   // 16-byte vectors per patch (32 channels)
     constexpr int PER = (NVEC + 255) / 256;
-    __shared__ bf16x8 patch[PY * PX * 4];
+    // the patch, then (after the last chunk) the finished output tile [TY x TX pixels][BN channels]
+    constexpr int SH = NVEC > TY * TX * BN / 8 ? NVEC : TY * TX * BN / 8;
+    __shared__ bf16x8 sh[SH];
+    bf16x8 *patch = sh;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, q = lane >> 4;
     const int W = 1 << op.lw, H = 1 << op.lh;
@@ -515,14 +566,51 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
             __syncthreads();
         }
     }
-    // D[4q + e][r] of tile (i, j): pixel (y0 + wave, x0 + 16 i + 4 q + e), channel n0 + 16 j + r
+    // D[4q + e][r] of tile (i, j): pixel (y0 + wave, x0 + 16 i + 4 q + e), channel n0 + 16 j + r.
+    // Finished values go through LDS (the loop ended with a barrier: the patch is free), so the
+    // output leaves in 16-byte vectors of 8 channels, and the optional 2 x 2 max pool of the
+    // tile (TY = 4 rows: 2 pooled rows) is formed from it without another pass over the output.
+    bf16 *tile = reinterpret_cast<bf16 *>(sh);
 #pragma unroll
     for (int i = 0; i < MT; i++)
 #pragma unroll
         for (int j = 0; j < NT; j++)
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                op.store(pix0 + ((long) wave << op.lw) + i * 16 + 4 * q + e, n0 + j * 16 + r, acc[i][j][e], 0);
+            for (int e = 0; e < 4; e++) {
+                const int px = wave * TX + i * 16 + 4 * q + e, ch = j * 16 + r;
+                const long n = n0 + ch;
+                const float v = n < op.cout ? op.finish(pix0 + ((long) wave << op.lw) + i * 16 + 4 * q + e, n,
+                                                        acc[i][j][e]) : 0.0f;
+                tile[px * BN + ch] = (bf16) v;
+            }
+    __syncthreads();
+    constexpr int VPP = BN / 8;  // 16-byte vectors per pixel
+    for (int v = threadIdx.x; v < TY * TX * VPP; v += 256) {
+        const int px = v / VPP, cg = v % VPP;
+        const long n = n0 + cg * 8;
+        if (n < op.cout)
+            *reinterpret_cast<bf16x8 *>(op.out.at(pix0 + ((long) (px / TX) << op.lw) + px % TX, (int) n)) = sh[v];
+    }
+    if (op.pool.ptr) {
+        // pooled pixel (y0 / 2 + pr, x0 / 2 + pc); window order (0,0), (0,1), (1,0), (1,1), first
+        // maximum kept (a NaN never replaces the running value)
+        const long pbase = (img << (op.lh + op.lw - 2)) + ((long) (y0 / 2) << (op.lw - 1)) + x0 / 2;
+        for (int v = threadIdx.x; v < (TY / 2) * (TX / 2) * VPP; v += 256) {
+            const int cg = v % VPP, pc = (v / VPP) % (TX / 2), pr = v / (VPP * (TX / 2));
+            const long n = n0 + cg * 8;
+            if (n >= op.cout) continue;
+            const int p00 = (2 * pr) * TX + 2 * pc;
+            bf16x8 m = sh[p00 * VPP + cg];
+            const int offs[3] = {1, TX, TX + 1};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const bf16x8 w = sh[(p00 + offs[k]) * VPP + cg];
+#pragma unroll
+                for (int c = 0; c < 8; c++) m[c] = (float) w[c] > (float) m[c] ? w[c] : m[c];
+            }
+            *reinterpret_cast<bf16x8 *>(op.pool.at(pbase + ((long) pr << (op.lw - 1)) + pc, (int) n)) = m;
+        }
+    }
 }
 
 }  // namespace nn

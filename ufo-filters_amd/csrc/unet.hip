@@ -432,9 +432,13 @@ long npix_at(const ufomi_unet *n, int B, int l) { return (long) B * (n->H >> l) 
 View<const bf16> cv(const bf16 *p, int cs, int c0 = 0) { return View<const bf16>{p, cs, c0}; }
 View<bf16> mv(bf16 *p, int cs, int c0 = 0) { return View<bf16>{p, cs, c0}; }
 
-// relu(conv3x3(in) + b) into out (forward), or the plain conv (data gradient).
+int maxpool(hipStream_t s, View<const bf16> in, View<bf16> out, long nout, int lh, int lw, int C);
+
+// relu(conv3x3(in) + b) into out (forward), or the plain conv (data gradient);
+// with pool.ptr, also the 2x2 max pool of out into pool (fused into the LDS kernel).
 int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, int cin, int cout, const bf16 *w,
-          const float *bias, bool relu, View<const bf16> mask, int mask_from, View<bf16> out)
+          const float *bias, bool relu, View<const bf16> mask, int mask_from, View<bf16> out,
+          View<bf16> pool = View<bf16>{nullptr, 0, 0})
 {
     nn::Conv3 op;
     op.in = in;
@@ -449,6 +453,7 @@ int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, i
     op.mask = mask;
     op.mask_from = mask_from;
     op.out = out;
+    op.pool = pool;
     const int lh = n->lh - l, lw = n->lw - l;
     if (lw >= 6 && lh >= 2) {
         // LDS-staged input patches (4 x 64 pixels per workgroup)
@@ -464,7 +469,10 @@ int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, i
         UFOMI_CHECK_LAUNCH();
         return UFOMI_OK;
     }
-    return gemm_fwd(s, op, npix, cout);
+    op.pool = View<bf16>{nullptr, 0, 0};
+    int rc = gemm_fwd(s, op, npix, cout);
+    if (rc || !pool.ptr) return rc;
+    return maxpool(s, View<const bf16>{out.ptr, out.cs, out.c0}, pool, npix / 4, lh, lw, cout);
 }
 
 // Split count for a K of `pixels` over an M x N weight gradient: enough
@@ -581,12 +589,9 @@ int forward(ufomi_unet *n, hipStream_t s, const float *img, int B, float *out)
         }
         const ConvP &c2 = n->enc2[l];
         View<bf16> dst = l < kLevels - 1 ? mv(n->cat[l], 2 * C, C) : mv(n->bottom, C);
+        View<bf16> pooled = l < kLevels - 1 ? mv(n->pool[l], C) : View<bf16>{nullptr, 0, 0};
         if ((rc = conv3(n, s, cv(n->e1[l], C), np, l, C, C, n->w16 + c2.w, n->master + c2.b, true, cv(nullptr, 0), 0,
-                        dst)))
-            return rc;
-        if (l < kLevels - 1 &&
-            (rc = maxpool(s, cv(n->cat[l], 2 * C, C), mv(n->pool[l], C), npix_at(n, B, l + 1), n->lh - l, n->lw - l,
-                          C)))
+                        dst, pooled)))
             return rc;
     }
     for (int l = kLevels - 2; l >= 0; l--) {
