@@ -474,14 +474,14 @@ namespace nn {
 // prefetched into registers.  Wave w computes row w of the tile (4 x NT MFMA
 // tiles).  Needs W >= 64 (a multiple of 64), H a multiple of 4, cin a
 // multiple of 32.
-template <int NT>
+template <int NT, bool LDS_OUT>
 __global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
 {
     constexpr int TX = 64, TY = 4, PX = TX + 2, PY = TY + 2, MT = 4, BN = NT * 16;
     constexpr int NVEC = PY * PX * 4;                   // 16-byte vectors per patch (32 channels)
     constexpr int PER = (NVEC + 255) / 256;
-    // the patch, then (after the last chunk) the finished output tile [TY x TX pixels][BN channels]
-    constexpr int SH = NVEC > TY * TX * BN / 8 ? NVEC : TY * TX * BN / 8;
+    // the patch, then (after the last chunk, LDS_OUT) the finished output tile [TY x TX pixels][BN channels]
+    constexpr int SH = (!LDS_OUT || NVEC > TY * TX * BN / 8) ? NVEC : TY * TX * BN / 8;
     __shared__ bf16x8 sh[SH];
     bf16x8 *patch = sh;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -567,9 +567,20 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
         }
     }
     // D[4q + e][r] of tile (i, j): pixel (y0 + wave, x0 + 16 i + 4 q + e), channel n0 + 16 j + r.
-    // Finished values go through LDS (the loop ended with a barrier: the patch is free), so the
-    // output leaves in 16-byte vectors of 8 channels, and the optional 2 x 2 max pool of the
-    // tile (TY = 4 rows: 2 pooled rows) is formed from it without another pass over the output.
+    if constexpr (!LDS_OUT) {
+#pragma unroll
+        for (int i = 0; i < MT; i++)
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    op.store(pix0 + ((long) wave << op.lw) + i * 16 + 4 * q + e, n0 + j * 16 + r, acc[i][j][e], 0);
+        return;
+    }
+    // LDS_OUT: finished values go through LDS (the loop ended with a barrier: the patch is
+    // free), so the output leaves in 16-byte vectors of 8 channels, and the optional 2 x 2 max
+    // pool of the tile (TY = 4 rows: 2 pooled rows) is formed from it without another pass over
+    // the output.
     bf16 *tile = reinterpret_cast<bf16 *>(sh);
 #pragma unroll
     for (int i = 0; i < MT; i++)
