@@ -71,6 +71,14 @@ def test_noalloc_unsplit_and_nearest(ctx, torch_dev):
     assert rel_l2(got, ref) < 1e-6
 
 
+def test_noalloc_half_staging_split(ctx, torch_dev):
+    # 8 f16 slices of 256 px: half-precision (v, d) staging, 64 tiles split into 4 angle parts
+    s8 = _sino((8, 300, 256), 16)
+    got = _run(ctx, torch_dev, ctx.bp_parallel_noalloc, s8, dtype=ufomi.F16)
+    ref = _run(ctx, torch_dev, ctx.bp_parallel, s8, dtype=ufomi.F16)
+    assert rel_l2(got, ref) < 1e-6
+
+
 def test_noalloc_rejects_too_many_angles(ctx, torch_dev):
     import torch
     # refused on the host before any launch: the table holds 32768 angles

@@ -624,4 +624,158 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
     }
 }
 
+
+// 16 zero bytes in device memory: the source of patch lanes outside the image.
+static __device__ __align__(16) unsigned char g_conv_zero16[16];
+
+// 3x3 convolution (Conv3) on 8-row tiles: a workgroup owns 8 rows x 64 pixels of
+// one image and BN = 16 NT output channels; wave w computes rows 2w and 2w + 1
+// (8 x NT MFMA tiles), so each weight fragment serves twice the MFMAs of the
+// 4-row kernel and the patch halo is 10/8 instead of 6/4 of the rows.  Per
+// 32-channel chunk the 10 x 66 pixel patch goes straight into LDS
+// (global_load_lds_dwordx4: no staging registers; lanes outside the image load
+// g_conv_zero16); weight fragments come from L2, loaded one tap ahead.  With
+// LDS_OUT the finished tile leaves through LDS in 16-byte vectors, with the
+// optional 2 x 2 max pool (4 pooled rows).  Needs W a multiple of 64, H a
+// multiple of 8, cin a multiple of 32.  Two waves per SIMD: the accumulators
+// stay in VGPRs.
+template <int NT, bool LDS_OUT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void conv3_t8_kernel(Conv3 op)
+{
+    constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, MT = 8, BN = NT * 16;
+    constexpr int NVEC = PY * PX * 4;        // 16-byte vectors of a patch (32 channels)
+    constexpr int PINS = (NVEC + 63) / 64;   // wave loads per patch
+    constexpr int OUTV = TY * TX * BN / 8;   // 16-byte vectors of the finished tile
+    constexpr int SH = (!LDS_OUT || PINS * 64 > OUTV) ? PINS * 64 : OUTV;
+    __shared__ bf16x8 sh[SH];
+    bf16x8 *patch = sh;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int W = 1 << op.lw, H = 1 << op.lh;
+    const int tiles_x = W / TX, tiles_y = H / TY;
+    int b = blockIdx.x;
+    const int tx = b % tiles_x;
+    b /= tiles_x;
+    const int ty = b % tiles_y;
+    const long img = b / tiles_y;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const long n0 = (long) blockIdx.y * BN;
+    const long pix0 = (img << (op.lh + op.lw)) + ((long) y0 << op.lw) + x0;  // pixel (y0, x0)
+    const int cin = 1 << op.lcin;
+    typedef __attribute__((address_space(3))) void lds_void;
+    const void *zero16 = g_conv_zero16;
+
+    auto stage = [&](int c0) {
+        for (int j = wave; j < PINS; j += 4) {
+            const int v = j * 64 + lane;
+            const void *src = zero16;
+            if (v < NVEC) {
+                const int g = v & 3, pc = (v >> 2) % PX, pr = (v >> 2) / PX;
+                const int yy = y0 + pr - 1, xx = x0 + pc - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    src = op.in.at(pix0 + (long) (pr - 1) * W + (pc - 1), c0 + 8 * g);
+            }
+            __builtin_amdgcn_global_load_lds(src, (lds_void *) (patch + j * 64), 16, 0, 0);
+        }
+    };
+
+    const bf16 *wrow[NT];
+    bool wok[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const long co = n0 + j * 16 + r;
+        wok[j] = co < op.cout;
+        wrow[j] = op.w + (wok[j] ? co : 0) * (9L << op.lcin);
+    }
+    auto load_b = [&](bf16x8 *bfr, int tap, int c0) {
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            bfr[j] = wok[j] ? ld8(wrow[j] + ((long) tap << op.lcin) + c0 + 8 * q) : zero8();
+    };
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    const int nchunks = cin / 32;
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int c0 = ch * 32;
+        stage(c0);
+        bf16x8 bcur[NT], bnext[NT];
+        load_b(bcur, 0, c0);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // patch of chunk ch in LDS
+#pragma unroll 1
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;  // patch offsets (0..2)
+            if (tap < 8) load_b(bnext, tap + 1, c0);
+            bf16x8 afr[MT];
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+                afr[i] = patch[((2 * wave + i / 4 + dy) * PX + ((i % 4) * 16 + r + dx)) * 4 + q];
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[i], bcur[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < NT; j++) bcur[j] = bnext[j];
+        }
+        __syncthreads();  // every wave done with this patch before the next is staged
+    }
+    // D[4q + e][r] of tile (i, j): pixel (y0 + 2 wave + i / 4, x0 + 16 (i % 4) + 4 q + e), channel n0 + 16 j + r
+    if constexpr (!LDS_OUT) {
+#pragma unroll
+        for (int i = 0; i < MT; i++)
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    op.store(pix0 + ((long) (2 * wave + i / 4) << op.lw) + (i % 4) * 16 + 4 * q + e,
+                             n0 + j * 16 + r, acc[i][j][e], 0);
+        return;
+    }
+    bf16 *tile = reinterpret_cast<bf16 *>(sh);
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = 2 * wave + i / 4, col = (i % 4) * 16 + 4 * q + e, ch = j * 16 + r;
+                const long n = n0 + ch;
+                const float v = n < op.cout ? op.finish(pix0 + ((long) row << op.lw) + col, n, acc[i][j][e]) : 0.0f;
+                tile[(row * TX + col) * BN + ch] = (bf16) v;
+            }
+    __syncthreads();
+    constexpr int VPP = BN / 8;  // 16-byte vectors per pixel
+    for (int v = threadIdx.x; v < OUTV; v += 256) {
+        const int px = v / VPP, cg = v % VPP;
+        const long n = n0 + cg * 8;
+        if (n < op.cout)
+            *reinterpret_cast<bf16x8 *>(op.out.at(pix0 + ((long) (px / TX) << op.lw) + px % TX, (int) n)) = sh[v];
+    }
+    if (op.pool.ptr) {
+        // pooled pixel (y0 / 2 + pr, x0 / 2 + pc); window order (0,0), (0,1), (1,0), (1,1), first
+        // maximum kept (a NaN never replaces the running value)
+        const long pbase = (img << (op.lh + op.lw - 2)) + ((long) (y0 / 2) << (op.lw - 1)) + x0 / 2;
+        for (int v = threadIdx.x; v < (TY / 2) * (TX / 2) * VPP; v += 256) {
+            const int cg = v % VPP, pc = (v / VPP) % (TX / 2), pr = v / (VPP * (TX / 2));
+            const long n = n0 + cg * 8;
+            if (n >= op.cout) continue;
+            const int p00 = (2 * pr) * TX + 2 * pc;
+            bf16x8 m = sh[p00 * VPP + cg];
+            const int offs[3] = {1, TX, TX + 1};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const bf16x8 w = sh[(p00 + offs[k]) * VPP + cg];
+#pragma unroll
+                for (int c = 0; c < 8; c++) m[c] = (float) w[c] > (float) m[c] ? w[c] : m[c];
+            }
+            *reinterpret_cast<bf16x8 *>(op.pool.at(pbase + ((long) pr << (op.lw - 1)) + pc, (int) n)) = m;
+        }
+    }
+}
+
 }  // namespace nn

@@ -456,23 +456,27 @@ int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, i
     op.pool = pool;
     const int lh = n->lh - l, lw = n->lw - l;
     if (lw >= 6 && lh >= 2) {
-        // LDS-staged input patches (4 x 64 pixels per workgroup)
+        // LDS-staged input patches (8 or 4 rows x 64 pixels per workgroup)
         const long tiles = npix >> 8;
         const int NT = cout <= 32 ? 2 : 4;
         const long gy = (cout + 16 * NT - 1) / (16 * NT);
         if (tiles > 0x7fffffffL || gy > 65535)
             return ufomi_fail(UFOMI_ERR_SIZE, "conv grid too large");
         // 32 output channels and pooled outputs leave through LDS in 16-byte vectors; for 64
-        // channels per workgroup (two waves per SIMD) the direct 2-byte stores measured faster
-        if (NT == 2)
-            hipLaunchKernelGGL((nn::conv3_lds_kernel<2, true>), dim3((unsigned) tiles, (unsigned) gy), dim3(256), 0,
-                               s, op);
-        else if (pool.ptr)
-            hipLaunchKernelGGL((nn::conv3_lds_kernel<4, true>), dim3((unsigned) tiles, (unsigned) gy), dim3(256), 0,
-                               s, op);
-        else
-            hipLaunchKernelGGL((nn::conv3_lds_kernel<4, false>), dim3((unsigned) tiles, (unsigned) gy), dim3(256), 0,
-                               s, op);
+        // channels per workgroup the direct 2-byte stores measured faster.  Images of at least
+        // 8 rows take the 8-row tiles.
+        auto launch = [&](auto kern, long nt) {
+            hipLaunchKernelGGL(kern, dim3((unsigned) nt, (unsigned) gy), dim3(256), 0, s, op);
+        };
+        if (lh >= 3) {
+            if (NT == 2) launch(nn::conv3_t8_kernel<2, true>, tiles / 2);
+            else if (pool.ptr) launch(nn::conv3_t8_kernel<4, true>, tiles / 2);
+            else launch(nn::conv3_t8_kernel<4, false>, tiles / 2);
+        } else {
+            if (NT == 2) launch(nn::conv3_lds_kernel<2, true>, tiles);
+            else if (pool.ptr) launch(nn::conv3_lds_kernel<4, true>, tiles);
+            else launch(nn::conv3_lds_kernel<4, false>, tiles);
+        }
         UFOMI_CHECK_LAUNCH();
         return UFOMI_OK;
     }
