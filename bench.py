@@ -751,7 +751,7 @@ def main():
             "config": {"workload": r["workload"]},
             "plain_1800_view_slices_per_s": round(r["plain_sps"], 1),
             "learned_vs_plain": round(r["learned_sps"] / r["plain_sps"], 3),
-            "roofline": dict({"kernel": "U-Net forward (gemm_kernel<...,Conv3|UpFwd> + VALU layers)", "bound": "mfma",
+            "roofline": dict({"kernel": "U-Net forward (conv3_t8_kernel, gemm_kernel<...,UpFwd> + VALU layers)", "bound": "mfma",
                               "achieved": round(r["achieved"] / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
                               "frac": round(r["achieved"] / 2.5e15, 4), "traffic": None}, **r["extra"]),
             "accuracy": r["accuracy"], "training": r["training"],
