@@ -573,7 +573,7 @@ def learned_fbp(args, ctx, torch, np, ufomi):
     dev = torch.device("cuda:0")
     size, full, sparse = 512, 1800, args.views
     n_train, n_test, batch = args.train_pairs, 256, 64
-    steps = args.train_steps if args.train_steps > 0 else max(1, n_train // batch)
+    steps = args.train_steps if args.train_steps > 0 else max(1, args.epochs * (n_train // batch))
     pm = learned.PairMaker(ctx, size, sparse, full, chunk=256)
     t0 = time.perf_counter()
     X = torch.empty((n_train, size, size), device=dev)
@@ -708,7 +708,8 @@ def main():
                                                      "(ufomi_bp_configure; default automatic)")
     ap.add_argument("--views", type=int, default=120, help="learned: sparse view count (divides 1800)")
     ap.add_argument("--train-pairs", type=int, default=20000, help="learned: training pairs")
-    ap.add_argument("--train-steps", type=int, default=0, help="learned: Adam steps (0: one epoch)")
+    ap.add_argument("--train-steps", type=int, default=0, help="learned: Adam steps (0: --epochs epochs)")
+    ap.add_argument("--epochs", type=int, default=5, help="learned: epochs over the training pairs")
     args = ap.parse_args()
 
     if args.gpus != 1:

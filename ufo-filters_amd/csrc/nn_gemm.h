@@ -413,6 +413,61 @@ struct Conv3Wgrad {
     __device__ __forceinline__ void store(long m, long n, float v, int z) const { part.put(m, n, v, z); }
 };
 
+// Weight gradient of Conv3 on channel-major copies: the same GEMM as Conv3Wgrad,
+// but A and B rows are contiguous along pixels, so a fragment (8 pixels of one
+// channel, in one image row: W >= 8) is one 16-byte load.
+//   gT[co][p]                   the output gradient, transposed;
+//   inT[(dx + 1) cin + c][p]    the input transposed and shifted by dx in {-1, 0, 1}
+//                               along the row, zero where x + dx leaves the row.
+// The tap's row shift dy is an offset of dy W pixels, zero where y + dy leaves
+// the image.
+struct Conv3WgradT {
+    static constexpr bool kVecStore = false;
+    const bf16 *gT;
+    const bf16 *inT;
+    int lh, lw, cin, cout;
+    long npix;
+    Partial part;
+
+    struct RowA {
+        const bf16 *p;
+        bool ok;
+    };
+    struct ColB {
+        const bf16 *p;
+        int dy;
+        bool ok;
+    };
+    __device__ __forceinline__ RowA row_a(long m) const
+    {
+        const bool ok = m < cout;
+        return RowA{gT + (ok ? m : 0) * npix, ok};
+    }
+    __device__ __forceinline__ ColB col_b(long n) const
+    {
+        ColB c;
+        c.ok = n < 9L * cin;
+        const int tap = c.ok ? (int) (n / cin) : 4, ch = c.ok ? (int) (n % cin) : 0;
+        c.dy = tap / 3 - 1;
+        const int dx = tap % 3 - 1;
+        c.p = inT + ((long) (dx + 1) * cin + ch) * npix + ((long) c.dy << lw);
+        return c;
+    }
+    __device__ __forceinline__ void k_range(int z, long &k0, long &k1) const { part.range(z, k0, k1); }
+    __device__ __forceinline__ bf16x8 load_a(const RowA &r, long k) const
+    {
+        return (r.ok && k < npix) ? ld8(r.p + k) : zero8();
+    }
+    __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const
+    {
+        if (!c.ok || k >= npix) return zero8();
+        const int yy = (int) ((k >> lw) & ((1 << lh) - 1)) + c.dy;
+        if (yy < 0 || yy >= (1 << lh)) return zero8();
+        return ld8(c.p + k);
+    }
+    __device__ __forceinline__ void store(long m, long n, float v, int z) const { part.put(m, n, v, z); }
+};
+
 // Weight gradient of UpFwd: dW[ab][co][c] = sum_p g[child(p, ab)][co] in[p][c].
 // M = 4 cout ((ab, co)), N = cin, K = input pixels (groups of 8 along a row).
 struct UpWgrad {

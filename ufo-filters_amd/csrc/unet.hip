@@ -299,6 +299,34 @@ __global__ __launch_bounds__(256) void up_transpose_kernel(const bf16 *__restric
     wt[i] = w[((long) ab * cout + co) * cin + c];
 }
 
+// Channel-major copies for the weight gradients: out[(s C + c) npix + p] =
+// in[p + dx_s][c] with dx_s = s - 1 (nshift = 3) or 0 (nshift = 1), zero where
+// x + dx_s leaves the row.  A block moves 64 consecutive pixels (plus one on
+// each side) through LDS: channel-contiguous reads, pixel-contiguous writes.
+constexpr int kTpPix = 64;
+__global__ __launch_bounds__(256) void transpose_shift_kernel(View<const bf16> in, int C, long npix, int lw,
+                                                              int nshift, bf16 *__restrict__ out)
+{
+    extern __shared__ bf16 tile[];  // [kTpPix + 2][C + 2]
+    const int pitch = C + 2;
+    const long p0 = (long) blockIdx.x * kTpPix;
+    for (int e = threadIdx.x; e < (kTpPix + 2) * C; e += 256) {
+        const int px = e / C, c = e % C;
+        const long p = p0 + px - 1;
+        tile[px * pitch + c] = (p >= 0 && p < npix) ? *in.at(p, c) : (bf16) 0.0f;
+    }
+    __syncthreads();
+    const int W = 1 << lw;
+    for (int e = threadIdx.x; e < nshift * C * kTpPix; e += 256) {
+        const int px = e % kTpPix, c = (e / kTpPix) % C, sh = e / (kTpPix * C);
+        const int dx = nshift == 3 ? sh - 1 : 0;
+        const long p = p0 + px;
+        if (p >= npix) continue;
+        const int x = (int) (p & (W - 1)) + dx;
+        out[((long) sh * C + c) * npix + p] = (x >= 0 && x < W) ? tile[(px + 1 + dx) * pitch + c] : (bf16) 0.0f;
+    }
+}
+
 // ------------------------------------------------------------ GEMM launches
 
 template <int MT, int NT, int WM, int WN, typename Op>
@@ -399,6 +427,8 @@ struct ufomi_unet {
          *g_d1[kLevels - 1] = {}, *g_d2[kLevels - 1] = {};
     float *out = nullptr, *gout = nullptr, *ws = nullptr;
     size_t ws_floats = 0;
+    bf16 *wgT = nullptr;  // channel-major copies of a weight gradient's operands (training)
+    size_t wgT_elems = 0;
     std::vector<void *> allocs;
     long step = 0;
 };
@@ -525,17 +555,41 @@ int reduce_into(hipStream_t s, const float *ws, int nz, long len, float *dst)
 int conv3_wgrad(ufomi_unet *n, hipStream_t s, View<const bf16> g, View<const bf16> in, long npix, int l, int cin,
                 int cout, float *dw)
 {
-    nn::Conv3Wgrad op;
-    op.g = g;
-    op.in = in;
-    op.lh = n->lh - l;
-    op.lw = n->lw - l;
-    op.cin = cin;
-    op.cout = cout;
-    op.npix = npix;
     const long M = cout, N = 9L * cin;
     long chunk;
     const int splits = wgrad_splits(n, M, N, npix, &chunk);
+    const int lh = n->lh - l, lw = n->lw - l;
+    const size_t need = (size_t) (cout + 3 * cin) * (size_t) npix;
+    if (lw >= 3 && n->wgT && need <= n->wgT_elems && npix % kTpPix == 0) {
+        // channel-major copies (g, and the input at the three row shifts), then 16-byte fragments
+        bf16 *gT = n->wgT, *inT = n->wgT + (size_t) cout * npix;
+        const unsigned blocks = (unsigned) (npix / kTpPix);
+        hipLaunchKernelGGL(transpose_shift_kernel, dim3(blocks), dim3(256), (size_t) (kTpPix + 2) * (cout + 2) * 2, s,
+                           g, cout, npix, lw, 1, gT);
+        hipLaunchKernelGGL(transpose_shift_kernel, dim3(blocks), dim3(256), (size_t) (kTpPix + 2) * (cin + 2) * 2, s,
+                           in, cin, npix, lw, 3, inT);
+        UFOMI_CHECK_LAUNCH();
+        nn::Conv3WgradT op;
+        op.gT = gT;
+        op.inT = inT;
+        op.lh = lh;
+        op.lw = lw;
+        op.cin = cin;
+        op.cout = cout;
+        op.npix = npix;
+        op.part = nn::Partial{n->ws, M, N, chunk, npix};
+        int rc = gemm_wgrad(s, op, M, N, splits);
+        if (rc) return rc;
+        return reduce_into(s, n->ws, splits, M * N, dw);
+    }
+    nn::Conv3Wgrad op;
+    op.g = g;
+    op.in = in;
+    op.lh = lh;
+    op.lw = lw;
+    op.cin = cin;
+    op.cout = cout;
+    op.npix = npix;
     op.part = nn::Partial{n->ws, M, N, chunk, npix};
     int rc = gemm_wgrad(s, op, M, N, splits);
     if (rc) return rc;
@@ -892,6 +946,22 @@ extern "C" int ufomi_unet_create(ufomi_ctx *ctx, int max_batch, int height, int 
         }
         n->ws_floats = 32u << 20;  // 128 MiB of split-K partial sums
         if (!rc) rc = dalloc_t(n, n->ws_floats, &n->ws);
+        // channel-major operand copies of the largest weight gradient: (cout + 3 cin) x pixels
+        size_t wg = 0;
+        auto need = [&](const ConvP &c, int l) {
+            wg = std::max(wg, (size_t) (c.cout + 3 * c.cin) * (size_t) npix_at(n, max_batch, l));
+        };
+        need(n->first, 0);
+        for (int l = 0; l < kLevels; l++) {
+            if (l > 0) need(n->enc1[l], l);
+            need(n->enc2[l], l);
+        }
+        for (int l = 0; l < kLevels - 1; l++) {
+            need(n->dec1[l], l);
+            need(n->dec2[l], l);
+        }
+        n->wgT_elems = wg;
+        if (!rc) rc = dalloc_t(n, wg, &n->wgT);
     }
     if (rc) {
         free_net(n);
