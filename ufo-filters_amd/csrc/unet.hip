@@ -302,21 +302,45 @@ __global__ __launch_bounds__(256) void up_transpose_kernel(const bf16 *__restric
 // Channel-major copies for the weight gradients: out[(s C + c) npix + p] =
 // in[p + dx_s][c] with dx_s = s - 1 (nshift = 3) or 0 (nshift = 1), zero where
 // x + dx_s leaves the row.  A block moves 64 consecutive pixels (plus one on
-// each side) through LDS: channel-contiguous reads, pixel-contiguous writes.
+// each side) through LDS.  With C a multiple of 8 both sides move 16-byte
+// vectors: 8 channels of a pixel in, 8 pixels of a channel out.
 constexpr int kTpPix = 64;
 __global__ __launch_bounds__(256) void transpose_shift_kernel(View<const bf16> in, int C, long npix, int lw,
                                                               int nshift, bf16 *__restrict__ out)
 {
-    extern __shared__ bf16 tile[];  // [kTpPix + 2][C + 2]
-    const int pitch = C + 2;
+    extern __shared__ bf16 tile[];  // [kTpPix + 2][pitch]
+    const int pitch = C % 8 == 0 ? C + 8 : C + 2;
     const long p0 = (long) blockIdx.x * kTpPix;
+    const int W = 1 << lw;
+    if (C % 8 == 0) {
+        const int groups = C / 8;
+        for (int e = threadIdx.x; e < (kTpPix + 2) * groups; e += 256) {
+            const int px = e / groups, cg = e % groups;
+            const long p = p0 + px - 1;
+            nn::bf16x8 v = (p >= 0 && p < npix) ? nn::ld8(in.at(p, 8 * cg)) : nn::zero8();
+            *reinterpret_cast<nn::bf16x8 *>(tile + px * pitch + 8 * cg) = v;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nshift * C * (kTpPix / 8); e += 256) {
+            const int px8 = e % (kTpPix / 8), c = (e / (kTpPix / 8)) % C, sh = e / ((kTpPix / 8) * C);
+            const int dx = nshift == 3 ? sh - 1 : 0;
+            const long p = p0 + 8 * px8;
+            if (p >= npix) continue;
+            const int x0 = (int) (p & (W - 1)) + dx;
+            nn::bf16x8 v;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                v[j] = (x0 + j >= 0 && x0 + j < W) ? tile[(8 * px8 + j + 1 + dx) * pitch + c] : (bf16) 0.0f;
+            *reinterpret_cast<nn::bf16x8 *>(out + ((long) sh * C + c) * npix + p) = v;
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < (kTpPix + 2) * C; e += 256) {
         const int px = e / C, c = e % C;
         const long p = p0 + px - 1;
         tile[px * pitch + c] = (p >= 0 && p < npix) ? *in.at(p, c) : (bf16) 0.0f;
     }
     __syncthreads();
-    const int W = 1 << lw;
     for (int e = threadIdx.x; e < nshift * C * kTpPix; e += 256) {
         const int px = e % kTpPix, c = (e / kTpPix) % C, sh = e / (kTpPix * C);
         const int dx = nshift == 3 ? sh - 1 : 0;
@@ -564,10 +588,9 @@ int conv3_wgrad(ufomi_unet *n, hipStream_t s, View<const bf16> g, View<const bf1
         // channel-major copies (g, and the input at the three row shifts), then 16-byte fragments
         bf16 *gT = n->wgT, *inT = n->wgT + (size_t) cout * npix;
         const unsigned blocks = (unsigned) (npix / kTpPix);
-        hipLaunchKernelGGL(transpose_shift_kernel, dim3(blocks), dim3(256), (size_t) (kTpPix + 2) * (cout + 2) * 2, s,
-                           g, cout, npix, lw, 1, gT);
-        hipLaunchKernelGGL(transpose_shift_kernel, dim3(blocks), dim3(256), (size_t) (kTpPix + 2) * (cin + 2) * 2, s,
-                           in, cin, npix, lw, 3, inT);
+        auto lds = [](int C) { return (size_t) (kTpPix + 2) * (C % 8 == 0 ? C + 8 : C + 2) * sizeof(bf16); };
+        hipLaunchKernelGGL(transpose_shift_kernel, dim3(blocks), dim3(256), lds(cout), s, g, cout, npix, lw, 1, gT);
+        hipLaunchKernelGGL(transpose_shift_kernel, dim3(blocks), dim3(256), lds(cin), s, in, cin, npix, lw, 3, inT);
         UFOMI_CHECK_LAUNCH();
         nn::Conv3WgradT op;
         op.gT = gT;
