@@ -375,11 +375,14 @@ int gemm_fwd(hipStream_t s, const Op &op, long M, long N)
     return gemm_go<4, 4, 2, 2>(s, op, M, N, 1);
 }
 
-// Weight gradients: 64 x 128 tiles, split-K over pixels.
+// Weight gradients, split-K over pixels: 32 x 256 tiles for 32 output channels,
+// 64 x 256 otherwise (each wave 16 or 64 rows by 64 columns; measured against
+// 64 x 128 tiles: training step 100.4 -> 89.8 ms).
 template <typename Op>
 int gemm_wgrad(hipStream_t s, const Op &op, long M, long N, int splits)
 {
-    return gemm_go<2, 4, 2, 2>(s, op, M, N, splits);
+    if (M <= 32) return gemm_go<2, 4, 1, 4>(s, op, M, N, splits);
+    return gemm_go<4, 4, 1, 4>(s, op, M, N, splits);
 }
 
 int ilog2(long x)
