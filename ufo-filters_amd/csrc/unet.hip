@@ -376,7 +376,7 @@ int gemm_fwd(hipStream_t s, const Op &op, long M, long N)
 }
 
 // Weight gradients, split-K over pixels: 32 x 256 tiles for 32 output channels,
-// 64 x 256 otherwise (each wave 16 or 64 rows by 64 columns; measured against
+// 64 x 256 otherwise (each wave 32 or 64 rows by 64 columns; measured against
 // 64 x 128 tiles: training step 100.4 -> 89.8 ms).
 template <typename Op>
 int gemm_wgrad(hipStream_t s, const Op &op, long M, long N, int splits)
