@@ -113,7 +113,7 @@ def want_cpu(args):
     return not args.no_cpu_baseline and not getattr(args, "accuracy_only", False)
 
 
-# The reference's own OpenCL kernels cannot serve as the CPU baseline here: the GPU box's OpenCL
+# The reference's own OpenCL kernels cannot serve as the CPU baseline here: the host's OpenCL
 # platform (AMD APP, clinfo) lists the GPU only and no CPU ICD is installed (probe summary in
 # profiles/host_opencl_probe.md).  The baseline is the float32 C restatement of those kernels.
 CPU_NOTE = ("CPU OpenCL baseline unavailable: the host's only OpenCL platform (AMD APP) lists the GPU alone, "

@@ -26,7 +26,7 @@ typedef enum {
     UFOMI_ERR_ARG = -1,          /* invalid argument (null pointer, size out of range) */
     UFOMI_ERR_HIP = -2,          /* HIP runtime error */
     UFOMI_ERR_NOT_POW2 = -3,     /* filter width not a power of two (ufo-filter-task.c:295-300) */
-    UFOMI_ERR_UNSUPPORTED = -4,  /* option the reference accepts but this build rejects */
+    UFOMI_ERR_UNSUPPORTED = -4,  /* option the reference accepts that is not implemented here */
     UFOMI_ERR_PROJECTIONS = -5,  /* sinogram taller than num-projections (ufo-backproject-task.c:231-237) */
     UFOMI_ERR_SIZE = -6,         /* fft size-x smaller than input (ufo-fft-task.c:139-143), crop larger than input (ufo-ifft-task.c:130-134) */
     UFOMI_ERR_NO_DEVICE = -7,    /* no HIP device / extension not usable */

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/ufo_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product library never loads
+cpu_baseline leg of bench.py, as the test oracle.  The product library never loads
 it.  Each wrapper names the reference code its C function restates.
 """
 import ctypes

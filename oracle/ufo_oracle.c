@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
  * calls this file.  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may use it, and only as the checker (or the CPU baseline),
+ * cpu_baseline leg may use it, and only as the test oracle (or the CPU baseline),
  * never as the thing measured or shipped.
  *
  * Every function restates a piece of the reference (ufo-filters 0.17.0) and

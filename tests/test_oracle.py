@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/ufo_oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/ufo_oracle.c) before the GPU parity tests trust it.
 
 Each check uses a source independent of the oracle's own formula: numpy.fft
 golden vectors, closed forms, brute force on tiny inputs, invariants

@@ -4,7 +4,7 @@
  * ufo-raw-reader.c (formats, data types, frame layout); the writers that of
  * src/writers/ufo-tiff-writer.c, ufo-raw-writer.c and the conversion of
  * src/writers/ufo-writer.c:55-230.  The TIFF code is written against the
- * TIFF 6.0 baseline and the BigTIFF extension (no libtiff in this build).
+ * TIFF 6.0 baseline and the BigTIFF extension, written without libtiff.
  */
 #include "io.h"
 

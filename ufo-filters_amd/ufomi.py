@@ -113,7 +113,7 @@ def load():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise RuntimeError("libufomi.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        raise RuntimeError("libufomi.so is not built (%s); run make -C ufo-filters_amd libufomi.so" % LIB_PATH)
     L = ctypes.CDLL(LIB_PATH)
     vp, ci, cu, cf, cd, csz = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
     dp = ctypes.POINTER(ctypes.c_double)
