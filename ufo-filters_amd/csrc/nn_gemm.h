@@ -24,6 +24,7 @@ namespace nn {
 
 using bf16 = __bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bf16x8 zero8()
@@ -81,8 +82,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(Op op)
 #pragma unroll
             for (int i = 0; i < MT; i++)
 #pragma unroll
-                for (int j = 0; j < NT; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < NT; j++) {
+                    // kVecStore: B as the row operand, so a lane receives 4 consecutive columns of one row
+                    if constexpr (Op::kVecStore)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+                    else
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                }
             if (more) {
 #pragma unroll
                 for (int i = 0; i < MT; i++) a[i] = an[i];
@@ -92,20 +98,23 @@ __global__ __launch_bounds__(256) void gemm_kernel(Op op)
         }
     }
     if constexpr (Op::kVecStore) {
-        // finished bf16 values through LDS, then 16-byte stores of 8 consecutive columns
+        // finished bf16 values through LDS (D[4q + e][r] of tile (i, j) is column n0 + 16 j + 4 q + e
+        // of row m0 + 16 i + r: one 8-byte write per tile), then 16-byte stores of 8 consecutive
+        // columns
         __shared__ bf16x8 tile[BM * BN / 8];
         bf16 *t = reinterpret_cast<bf16 *>(tile);
         const int mw = (wave % WM) * MT * 16, nw = (wave / WM) * NT * 16;
 #pragma unroll
         for (int i = 0; i < MT; i++)
 #pragma unroll
-            for (int j = 0; j < NT; j++)
+            for (int j = 0; j < NT; j++) {
+                const long m = m0 + i * 16 + r, n = n0 + j * 16 + 4 * q;
+                bf16x4 o;
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const long m = m0 + i * 16 + 4 * q + e, n = n0 + j * 16 + r;
-                    t[(mw + i * 16 + 4 * q + e) * BN + nw + j * 16 + r] =
-                        (bf16) (op.valid(m, n) ? op.finish(m, n, acc[i][j][e]) : 0.0f);
-                }
+                for (int e = 0; e < 4; e++)
+                    o[e] = (bf16) (op.valid(m, n + e) ? op.finish(m, n + e, acc[i][j][e]) : 0.0f);
+                *reinterpret_cast<bf16x4 *>(t + (mw + i * 16 + r) * BN + nw + j * 16 + 4 * q) = o;
+            }
         __syncthreads();
         const long mb = (long) blockIdx.x * BM, nb = (long) blockIdx.y * BN;
         for (int v = threadIdx.x; v < BM * BN / 8; v += 256) {
@@ -160,6 +169,12 @@ struct Conv3 {
     int mask_from;
     View<bf16> out;
     View<bf16> pool;        // pool.ptr not nullptr: also the 2x2 max pool of out (LDS kernel only)
+    // fin_out not nullptr (8-row kernel with LDS_OUT, cout == 32): also the 1x1 output conv
+    // fin_out[p] = fin_img[p] + (sum_c fin_w[c] out[p][c] + fin_b[0]) from the finished tile;
+    // out.ptr may then be nullptr (the tile itself not stored)
+    const bf16 *fin_w;
+    const float *fin_b, *fin_img;
+    float *fin_out;
 
     struct RowA {
         long p;
@@ -216,6 +231,29 @@ struct Conv3 {
     {
         if (m >= npix || n >= cout) return;
         *out.at(m, (int) n) = (bf16) finish(m, n, v);
+    }
+    // finish() of the four output channels n .. n + 3 of pixel m (n a multiple of 4,
+    // n + 3 < cout), rounded to bf16
+    __device__ __forceinline__ bf16x4 finish4(long m, long n, f32x4 v) const
+    {
+        if (bias) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) v[e] += bias[n + e];
+        }
+        if (relu) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.0f);
+        }
+        if (mask.ptr) {
+            const bf16x4 mk = *reinterpret_cast<const bf16x4 *>(mask.at(m, (int) n));
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (n + e >= mask_from && !((float) mk[e] > 0.0f)) v[e] = 0.0f;
+        }
+        bf16x4 o;
+#pragma unroll
+        for (int e = 0; e < 4; e++) o[e] = (bf16) v[e];
+        return o;
     }
 };
 
@@ -684,20 +722,21 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
 static __device__ __align__(16) unsigned char g_conv_zero16[16];
 
 // 3x3 convolution (Conv3) on 8-row tiles: a workgroup owns 8 rows x 64 pixels of
-// one image and BN = 16 NT output channels; wave w computes rows 2w and 2w + 1
-// (8 x NT MFMA tiles), so each weight fragment serves twice the MFMAs of the
-// 4-row kernel and the patch halo is 10/8 instead of 6/4 of the rows.  Per
+// one image and BN = 16 NT output channels; wave w of the 8 / RPW waves computes
+// rows RPW w .. RPW w + RPW - 1 (4 RPW x NT MFMA tiles), and the patch halo is
+// 10/8 instead of the 4-row kernel's 6/4 of the rows.  Per
 // 32-channel chunk the 10 x 66 pixel patch goes straight into LDS
 // (global_load_lds_dwordx4: no staging registers; lanes outside the image load
 // g_conv_zero16); weight fragments come from L2, loaded one tap ahead.  With
 // LDS_OUT the finished tile leaves through LDS in 16-byte vectors, with the
 // optional 2 x 2 max pool (4 pooled rows).  Needs W a multiple of 64, H a
-// multiple of 8, cin a multiple of 32.  Two waves per SIMD: the accumulators
-// stay in VGPRs.
-template <int NT, bool LDS_OUT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void conv3_t8_kernel(Conv3 op)
+// multiple of 8, cin a multiple of 32.  The accumulators stay in VGPRs (two
+// waves per SIMD with RPW = 2, four with RPW = 1).
+template <int NT, bool LDS_OUT, int RPW>
+__global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / RPW))) void conv3_t8_kernel(Conv3 op)
 {
-    constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, MT = 8, BN = NT * 16;
+    constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, MT = 4 * RPW, BN = NT * 16;
+    constexpr int WAVES = TY / RPW, NTH = 64 * WAVES;
     constexpr int NVEC = PY * PX * 4;        // 16-byte vectors of a patch (32 channels)
     constexpr int PINS = (NVEC + 63) / 64;   // wave loads per patch
     constexpr int OUTV = TY * TX * BN / 8;   // 16-byte vectors of the finished tile
@@ -721,11 +760,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
     const void *zero16 = g_conv_zero16;
 
     auto stage = [&](int c0) {
-        for (int j = wave; j < PINS; j += 4) {
+        for (int j = wave; j < PINS; j += WAVES) {
             const int v = j * 64 + lane;
             const void *src = zero16;
             if (v < NVEC) {
-                const int g = v & 3, pc = (v >> 2) % PX, pr = (v >> 2) / PX;
+                const int p = v >> 2, g = v & 3, pc = p % PX, pr = p / PX;
                 const int yy = y0 + pr - 1, xx = x0 + pc - 1;
                 if (yy >= 0 && yy < H && xx >= 0 && xx < W)
                     src = op.in.at(pix0 + (long) (pr - 1) * W + (pc - 1), c0 + 8 * g);
@@ -768,54 +807,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
             bf16x8 afr[MT];
 #pragma unroll
             for (int i = 0; i < MT; i++)
-                afr[i] = patch[((2 * wave + i / 4 + dy) * PX + ((i % 4) * 16 + r + dx)) * 4 + q];
+                afr[i] = patch[((RPW * wave + i / 4 + dy) * PX + (i % 4) * 16 + r + dx) * 4 + q];
+            // weights as the row operand: lane (r, q) receives channels 4q .. 4q + 3 of pixel r
 #pragma unroll
             for (int i = 0; i < MT; i++)
 #pragma unroll
                 for (int j = 0; j < NT; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[i], bcur[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bcur[j], afr[i], acc[i][j], 0, 0, 0);
 #pragma unroll
             for (int j = 0; j < NT; j++) bcur[j] = bnext[j];
         }
         __syncthreads();  // every wave done with this patch before the next is staged
     }
-    // D[4q + e][r] of tile (i, j): pixel (y0 + 2 wave + i / 4, x0 + 16 (i % 4) + 4 q + e), channel n0 + 16 j + r
+    // D[4q + e][r] of tile (i, j): channel n0 + 16 j + 4 q + e of pixel (y0 + RPW wave + i / 4, x0 + 16 (i % 4) + r)
     if constexpr (!LDS_OUT) {
 #pragma unroll
-        for (int i = 0; i < MT; i++)
+        for (int i = 0; i < MT; i++) {
+            const long m = pix0 + ((long) (RPW * wave + i / 4) << op.lw) + (i % 4) * 16 + r;
 #pragma unroll
-            for (int j = 0; j < NT; j++)
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-                    op.store(pix0 + ((long) (2 * wave + i / 4) << op.lw) + (i % 4) * 16 + 4 * q + e,
-                             n0 + j * 16 + r, acc[i][j][e], 0);
+            for (int j = 0; j < NT; j++) {
+                const long n = n0 + j * 16 + 4 * q;
+                if (n < op.cout) *reinterpret_cast<bf16x4 *>(op.out.at(m, (int) n)) = op.finish4(m, n, acc[i][j]);
+            }
+        }
         return;
     }
     bf16 *tile = reinterpret_cast<bf16 *>(sh);
 #pragma unroll
-    for (int i = 0; i < MT; i++)
+    for (int i = 0; i < MT; i++) {
+        const int row = RPW * wave + i / 4, col = (i % 4) * 16 + r;
 #pragma unroll
-        for (int j = 0; j < NT; j++)
+        for (int j = 0; j < NT; j++) {
+            const int ch = j * 16 + 4 * q;
+            const long n = n0 + ch;
+            bf16x4 v;
+            if (n < op.cout) {
+                v = op.finish4(pix0 + ((long) row << op.lw) + col, n, acc[i][j]);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int row = 2 * wave + i / 4, col = (i % 4) * 16 + 4 * q + e, ch = j * 16 + r;
-                const long n = n0 + ch;
-                const float v = n < op.cout ? op.finish(pix0 + ((long) row << op.lw) + col, n, acc[i][j][e]) : 0.0f;
-                tile[(row * TX + col) * BN + ch] = (bf16) v;
+                for (int e = 0; e < 4; e++) v[e] = (bf16) 0.0f;
             }
+            *reinterpret_cast<bf16x4 *>(tile + (row * TX + col) * BN + ch) = v;
+        }
+    }
     __syncthreads();
     constexpr int VPP = BN / 8;  // 16-byte vectors per pixel
-    for (int v = threadIdx.x; v < OUTV; v += 256) {
-        const int px = v / VPP, cg = v % VPP;
-        const long n = n0 + cg * 8;
-        if (n < op.cout)
-            *reinterpret_cast<bf16x8 *>(op.out.at(pix0 + ((long) (px / TX) << op.lw) + px % TX, (int) n)) = sh[v];
+    if (op.out.ptr) {
+        for (int v = threadIdx.x; v < OUTV; v += NTH) {
+            const int px = v / VPP, cg = v % VPP;
+            const long n = n0 + cg * 8;
+            if (n < op.cout)
+                *reinterpret_cast<bf16x8 *>(op.out.at(pix0 + ((long) (px / TX) << op.lw) + px % TX, (int) n)) =
+                    sh[v];
+        }
     }
     if (op.pool.ptr) {
         // pooled pixel (y0 / 2 + pr, x0 / 2 + pc); window order (0,0), (0,1), (1,0), (1,1), first
         // maximum kept (a NaN never replaces the running value)
         const long pbase = (img << (op.lh + op.lw - 2)) + ((long) (y0 / 2) << (op.lw - 1)) + x0 / 2;
-        for (int v = threadIdx.x; v < (TY / 2) * (TX / 2) * VPP; v += 256) {
+        for (int v = threadIdx.x; v < (TY / 2) * (TX / 2) * VPP; v += NTH) {
             const int cg = v % VPP, pc = (v / VPP) % (TX / 2), pr = v / (VPP * (TX / 2));
             const long n = n0 + cg * 8;
             if (n >= op.cout) continue;
@@ -829,6 +879,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 for (int c = 0; c < 8; c++) m[c] = (float) w[c] > (float) m[c] ? w[c] : m[c];
             }
             *reinterpret_cast<bf16x8 *>(op.pool.at(pbase + ((long) pr << (op.lw - 1)) + pc, (int) n)) = m;
+        }
+    }
+    if constexpr (BN == 32) {
+        if (op.fin_out) {
+            // the 1x1 output conv over the tile's 32 channels, in channel order as a separate pass
+            // over the stored tile would sum them
+            for (int px = threadIdx.x; px < TY * TX; px += NTH) {
+                const long p = pix0 + ((long) (px / TX) << op.lw) + px % TX;
+                float a = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const bf16x8 v = sh[px * 4 + i];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) a = fmaf((float) op.fin_w[8 * i + j], (float) v[j], a);
+                }
+                op.fin_out[p] = op.fin_img[p] + (a + op.fin_b[0]);
+            }
         }
     }
 }

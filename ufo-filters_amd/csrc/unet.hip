@@ -8,7 +8,7 @@
 //   bottom = relu(conv3x3(relu(conv3x3(maxpool(skip_2)))))    C_3 = 256
 //   dec_l  = relu(conv3x3(relu(conv3x3([up_l(src) | skip_l]))))   src = bottom or dec_{l+1}
 //   up_l   = 2x2 stride-2 transposed conv C_{l+1} -> C_l (linear) UpFwd GEMM
-//   out    = img + conv1x1(dec_0, 32 -> 1)                    final_kernel (VALU, fp32)
+//   out    = img + conv1x1(dec_0, 32 -> 1)                    fp32, in dec_0's conv epilogue (or final_kernel)
 // The concatenation is never copied: up_l and skip_l are written into the two
 // channel halves of one [pixels][2 C_l] buffer.
 //
@@ -491,13 +491,28 @@ View<bf16> mv(bf16 *p, int cs, int c0 = 0) { return View<bf16>{p, cs, c0}; }
 
 int maxpool(hipStream_t s, View<const bf16> in, View<bf16> out, long nout, int lh, int lw, int C);
 
+// The 1x1 output conv out[p] = img[p] + sum_c w[c] d[p][c] + b[0] of the last
+// 32-channel activation d; keep_d: d itself is stored too.
+struct FinalConv {
+    const bf16 *w;
+    const float *b, *img;
+    float *out;
+    bool keep_d;
+};
+
 // relu(conv3x3(in) + b) into out (forward), or the plain conv (data gradient);
 // with pool.ptr, also the 2x2 max pool of out into pool (fused into the LDS kernel).
+// With fin, *fused tells whether the output conv was fused into the epilogue (8-row
+// kernel, cout == 32); if not, the caller runs it.
 int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, int cin, int cout, const bf16 *w,
           const float *bias, bool relu, View<const bf16> mask, int mask_from, View<bf16> out,
-          View<bf16> pool = View<bf16>{nullptr, 0, 0})
+          View<bf16> pool = View<bf16>{nullptr, 0, 0}, const FinalConv *fin = nullptr, bool *fused = nullptr)
 {
     nn::Conv3 op;
+    op.fin_w = nullptr;
+    op.fin_b = op.fin_img = nullptr;
+    op.fin_out = nullptr;
+    if (fused) *fused = false;
     op.in = in;
     op.lh = n->lh - l;
     op.lw = n->lw - l;
@@ -522,13 +537,21 @@ int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, i
         // 32 output channels and pooled outputs leave through LDS in 16-byte vectors; for 64
         // channels per workgroup the direct 2-byte stores measured faster.  Images of at least
         // 8 rows take the 8-row tiles.
-        auto launch = [&](auto kern, long nt) {
-            hipLaunchKernelGGL(kern, dim3((unsigned) nt, (unsigned) gy), dim3(256), 0, s, op);
+        auto launch = [&](auto kern, long nt, int threads = 256) {
+            hipLaunchKernelGGL(kern, dim3((unsigned) nt, (unsigned) gy), dim3(threads), 0, s, op);
         };
         if (lh >= 3) {
-            if (NT == 2) launch(nn::conv3_t8_kernel<2, true>, tiles / 2);
-            else if (pool.ptr) launch(nn::conv3_t8_kernel<4, true>, tiles / 2);
-            else launch(nn::conv3_t8_kernel<4, false>, tiles / 2);
+            if (fin && cout == 32) {
+                op.fin_w = fin->w;
+                op.fin_b = fin->b;
+                op.fin_img = fin->img;
+                op.fin_out = fin->out;
+                if (!fin->keep_d) op.out.ptr = nullptr;
+                *fused = true;
+            }
+            if (NT == 2) launch(nn::conv3_t8_kernel<2, true, 2>, tiles / 2);
+            else if (pool.ptr) launch(nn::conv3_t8_kernel<4, true, 2>, tiles / 2);
+            else launch(nn::conv3_t8_kernel<4, false, 2>, tiles / 2);
         } else {
             if (NT == 2) launch(nn::conv3_lds_kernel<2, true>, tiles);
             else if (pool.ptr) launch(nn::conv3_lds_kernel<4, true>, tiles);
@@ -662,6 +685,7 @@ int refresh_transforms(ufomi_unet *n, hipStream_t s)
 int forward(ufomi_unet *n, hipStream_t s, const float *img, int B, float *out)
 {
     int rc;
+    bool fused = false;
     const int C0 = kBase;
     {
         const long np = npix_at(n, B, 0);
@@ -704,14 +728,19 @@ int forward(ufomi_unet *n, hipStream_t s, const float *img, int B, float *out)
         if ((rc = conv3(n, s, cv(n->cat[l], 2 * C), np, l, 2 * C, C, n->w16 + a.w, n->master + a.b, true,
                         cv(nullptr, 0), 0, mv(n->d1[l], C))))
             return rc;
+        // level 0: the output conv rides in this conv's epilogue where the 8-row kernel runs
+        // (training keeps dec_0 for the backward pass)
+        const FinalConv fc{n->w16 + n->fin_w, n->master + n->fin_b, img, out, n->training};
         if ((rc = conv3(n, s, cv(n->d1[l], C), np, l, C, C, n->w16 + b.w, n->master + b.b, true, cv(nullptr, 0), 0,
-                        mv(n->d2[l], C))))
+                        mv(n->d2[l], C), View<bf16>{nullptr, 0, 0}, l == 0 ? &fc : nullptr, &fused)))
             return rc;
     }
-    const long np = npix_at(n, B, 0);
-    hipLaunchKernelGGL(final_kernel, dim3(blocks_for(np)), dim3(256), 0, s, n->d2[0], np, n->w16 + n->fin_w,
-                       n->master + n->fin_b, img, out);
-    UFOMI_CHECK_LAUNCH();
+    if (!fused) {
+        const long np = npix_at(n, B, 0);
+        hipLaunchKernelGGL(final_kernel, dim3(blocks_for(np)), dim3(256), 0, s, n->d2[0], np, n->w16 + n->fin_w,
+                           n->master + n->fin_b, img, out);
+        UFOMI_CHECK_LAUNCH();
+    }
     return UFOMI_OK;
 }
 
