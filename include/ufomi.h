@@ -564,7 +564,7 @@ int ufomi_add_noise(ufomi_ctx *ctx, float *d, size_t n, double rel_sigma, uint64
 int ufomi_bench_lds_read(ufomi_ctx *ctx, int blocks, int iters, float *d_sink, double *bytes);
 int ufomi_bench_copy(ufomi_ctx *ctx, const float *d_in, float *d_out, size_t n);
 /* Dense BF16 MFMA: blocks x 256 threads, each wave `iters` x 8 independent
- * v_mfma_f32_16x16x32_bf16; *flops = the work issued (2 flops per MAC). */
+ * v_mfma_f32_32x32x16_bf16; *flops = the work issued (2 flops per MAC). */
 int ufomi_bench_mfma(ufomi_ctx *ctx, int blocks, int iters, float *d_sink, double *flops);
 
 #ifdef __cplusplus

@@ -122,11 +122,12 @@ extern "C" int ufomi_destroy(ufomi_ctx *ctx)
 {
     if (!ctx)
         return UFOMI_OK;
-    hipSetDevice(ctx->device);
+    // teardown: errors here have no one left to report to
+    (void) hipSetDevice(ctx->device);
     if (ctx->stream)
-        hipStreamSynchronize(ctx->stream);
+        (void) hipStreamSynchronize(ctx->stream);
     else
-        hipDeviceSynchronize();
+        (void) hipDeviceSynchronize();
     (void) hipFree(ctx->twiddles);
     (void) hipFree(ctx->flags);
     if (ctx->lut)
@@ -134,9 +135,9 @@ extern "C" int ufomi_destroy(ufomi_ctx *ctx)
     if (ctx->bp_part)
         (void) hipFree(ctx->bp_part);
     if (ctx->scratch)
-        hipFree(ctx->scratch);
+        (void) hipFree(ctx->scratch);
     if (ctx->hscratch)
-        hipHostFree(ctx->hscratch);
+        (void) hipHostFree(ctx->hscratch);
     free(ctx);
     return UFOMI_OK;
 }

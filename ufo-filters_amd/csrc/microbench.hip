@@ -1,7 +1,7 @@
 // Roofline denominators measured on the running GPU: LDS read bandwidth
 // (ds_read_b128, conflict-free, every CU busy), HBM copy bandwidth
 // (streaming float4 copy) and dense BF16 MFMA throughput
-// (v_mfma_f32_16x16x32_bf16 chains held in registers).  Used by bench.py to express kernel throughput as a
+// (v_mfma_f32_32x32x16_bf16 chains held in registers).  Used by bench.py to express kernel throughput as a
 // fraction of what this device sustains.
 #include "ufomi_internal.h"
 
@@ -42,9 +42,12 @@ __global__ __launch_bounds__(256) void copy_kernel(const float4 *__restrict__ in
 }
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-// 8 independent accumulator chains per wave, operands in registers.
+// 8 independent v_mfma_f32_32x32x16_bf16 accumulator chains per wave, operands
+// in registers.  (Fewer independent chains leave the pipe waiting on each
+// chain's previous result: 8 chains of the 16x16x32 form reach about half the
+// rate that 16 chains of it or the 32x32x16 form reach.)
 __global__ __launch_bounds__(256) void mfma_kernel(int iters, float *__restrict__ sink)
 {
     bf16x8_t a, b;
@@ -53,16 +56,18 @@ __global__ __launch_bounds__(256) void mfma_kernel(int iters, float *__restrict_
         a[i] = (__bf16) (1.0f + 0.001f * (float) (threadIdx.x + i));
         b[i] = (__bf16) (0.5f - 0.001f * (float) i);
     }
-    f32x4_t acc[8];
+    f32x16_t acc[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) acc[c] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < 8; c++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[c][e] = 0.0f;
     for (int it = 0; it < iters; it++) {
 #pragma unroll
-        for (int c = 0; c < 8; c++) acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[c], 0, 0, 0);
+        for (int c = 0; c < 8; c++) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[c], 0, 0, 0);
     }
     float s = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 8; c++) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+    for (int c = 0; c < 8; c++) s += acc[c][0] + acc[c][15];
     if (s == -1.0f)  // never true; keeps the chains alive
         sink[blockIdx.x * 256 + threadIdx.x] = s;
 }
@@ -77,8 +82,8 @@ extern "C" int ufomi_bench_mfma(ufomi_ctx *ctx, int blocks, int iters, float *d_
         return ufomi_fail(UFOMI_ERR_ARG, "invalid microbenchmark arguments");
     hipLaunchKernelGGL(mfma_kernel, dim3(blocks), dim3(256), 0, ctx->stream, iters, d_sink);
     UFOMI_CHECK_LAUNCH();
-    if (flops)  // 4 waves x 8 chains x 16 x 16 x 32 x 2 per iteration
-        *flops = (double) blocks * 4.0 * iters * 8.0 * 16384.0;
+    if (flops)  // 4 waves x 8 chains x 32 x 32 x 16 x 2 per iteration
+        *flops = (double) blocks * 4.0 * iters * 8.0 * 32768.0;
     return UFOMI_OK;
 }
 

@@ -726,8 +726,12 @@ static __device__ __align__(16) unsigned char g_conv_zero16[16];
 // rows RPW w .. RPW w + RPW - 1 (4 RPW x NT MFMA tiles), and the patch halo is
 // 10/8 instead of the 4-row kernel's 6/4 of the rows.  Per
 // 32-channel chunk the 10 x 66 pixel patch goes straight into LDS
-// (global_load_lds_dwordx4: no staging registers; lanes outside the image load
-// g_conv_zero16); weight fragments come from L2, loaded one tap ahead.  With
+// (global_load_lds_dwordx4: no staging registers; lanes outside the image or
+// past cout load g_conv_zero16).  With 64 output channels (NT = 4) the chunk's
+// 64 x 9 x 32 weights go into LDS the same way, so the tap loop issues no
+// global loads (78 KB of LDS: two workgroups per CU, as the registers allow);
+// with 32 the weight fragments come from L2 a tap ahead and the 42 KB patch
+// alone lets three workgroups share a CU.  With
 // LDS_OUT the finished tile leaves through LDS in 16-byte vectors, with the
 // optional 2 x 2 max pool (4 pooled rows).  Needs W a multiple of 64, H a
 // multiple of 8, cin a multiple of 32.  The accumulators stay in VGPRs (two
@@ -739,10 +743,15 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
     constexpr int WAVES = TY / RPW, NTH = 64 * WAVES;
     constexpr int NVEC = PY * PX * 4;        // 16-byte vectors of a patch (32 channels)
     constexpr int PINS = (NVEC + 63) / 64;   // wave loads per patch
+    constexpr bool WL = NT >= 4;            // weights staged in LDS
+    constexpr int WV = WL ? BN * 36 : 0;    // 16-byte vectors of a chunk's weights: [BN][9 taps][4]
+    constexpr int WINS = (WV + 63) / 64;     // wave loads per chunk of weights
+    constexpr int STAGED = (PINS + WINS) * 64;
     constexpr int OUTV = TY * TX * BN / 8;   // 16-byte vectors of the finished tile
-    constexpr int SH = (!LDS_OUT || PINS * 64 > OUTV) ? PINS * 64 : OUTV;
+    constexpr int SH = (!LDS_OUT || STAGED > OUTV) ? STAGED : OUTV;
     __shared__ bf16x8 sh[SH];
     bf16x8 *patch = sh;
+    [[maybe_unused]] bf16x8 *wts = sh + PINS * 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, q = lane >> 4;
     const int W = 1 << op.lw, H = 1 << op.lh;
@@ -771,8 +780,20 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
             }
             __builtin_amdgcn_global_load_lds(src, (lds_void *) (patch + j * 64), 16, 0, 0);
         }
+        // weights: vector u = (output channel co, tap, channel group g) of w[n0 + co][tap][c0 + 8 g ..]
+        for (int j = wave; WL && j < WINS; j += WAVES) {
+            const int u = j * 64 + lane;
+            const void *src = zero16;
+            if (u < WV) {
+                const int co = u / 36, rem = u - 36 * co, tap = rem >> 2, g = rem & 3;
+                if (n0 + co < op.cout)
+                    src = op.w + (n0 + co) * (9L << op.lcin) + ((long) tap << op.lcin) + c0 + 8 * g;
+            }
+            __builtin_amdgcn_global_load_lds(src, (lds_void *) (wts + j * 64), 16, 0, 0);
+        }
     };
 
+    // weight fragments from L2 (NT < 4): row n0 + 16 j + r, loaded a tap ahead
     const bf16 *wrow[NT];
     bool wok[NT];
 #pragma unroll
@@ -797,13 +818,21 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
         const int c0 = ch * 32;
         stage(c0);
         bf16x8 bcur[NT], bnext[NT];
-        load_b(bcur, 0, c0);
+        if constexpr (!WL) load_b(bcur, 0, c0);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();  // patch of chunk ch in LDS
+        __syncthreads();  // patch (and weights) of chunk ch in LDS
 #pragma unroll 1
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;  // patch offsets (0..2)
-            if (tap < 8) load_b(bnext, tap + 1, c0);
+            bf16x8 bfr[NT];
+            if constexpr (WL) {
+#pragma unroll
+                for (int j = 0; j < NT; j++) bfr[j] = wts[(16 * j + r) * 36 + tap * 4 + q];
+            } else {
+                if (tap < 8) load_b(bnext, tap + 1, c0);
+#pragma unroll
+                for (int j = 0; j < NT; j++) bfr[j] = bcur[j];
+            }
             bf16x8 afr[MT];
 #pragma unroll
             for (int i = 0; i < MT; i++)
@@ -813,11 +842,13 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
             for (int i = 0; i < MT; i++)
 #pragma unroll
                 for (int j = 0; j < NT; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bcur[j], afr[i], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j], afr[i], acc[i][j], 0, 0, 0);
+            if constexpr (!WL) {
 #pragma unroll
-            for (int j = 0; j < NT; j++) bcur[j] = bnext[j];
+                for (int j = 0; j < NT; j++) bcur[j] = bnext[j];
+            }
         }
-        __syncthreads();  // every wave done with this patch before the next is staged
+        __syncthreads();  // every wave done with this chunk before the next is staged
     }
     // D[4q + e][r] of tile (i, j): channel n0 + 16 j + 4 q + e of pixel (y0 + RPW wave + i / 4, x0 + 16 (i % 4) + r)
     if constexpr (!LDS_OUT) {
