@@ -721,29 +721,28 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Conv3 op)
 // 16 zero bytes in device memory: the source of patch lanes outside the image.
 static __device__ __align__(16) unsigned char g_conv_zero16[16];
 
-// 3x3 convolution (Conv3) on 8-row tiles: a workgroup owns 8 rows x 64 pixels of
-// one image and BN = 16 NT output channels; wave w of the 8 / RPW waves computes
-// rows RPW w .. RPW w + RPW - 1 (4 RPW x NT MFMA tiles), and the patch halo is
-// 10/8 instead of the 4-row kernel's 6/4 of the rows.  Per
-// 32-channel chunk the 10 x 66 pixel patch goes straight into LDS
-// (global_load_lds_dwordx4: no staging registers; lanes outside the image or
-// past cout load g_conv_zero16).  With 64 output channels (NT = 4) the chunk's
-// 64 x 9 x 32 weights go into LDS the same way, so the tap loop issues no
-// global loads (78 KB of LDS: two workgroups per CU, as the registers allow);
-// with 32 the weight fragments come from L2 a tap ahead and the 42 KB patch
-// alone lets three workgroups share a CU.  With
-// LDS_OUT the finished tile leaves through LDS in 16-byte vectors, with the
-// optional 2 x 2 max pool (4 pooled rows).  Needs W a multiple of 64, H a
+// 3x3 convolution (Conv3) on 8-row tiles: a workgroup owns 8 rows x TX pixels
+// (TX = 64 or 32) of one image and BN = 16 NT output channels; wave w of the
+// 8 / RPW waves computes rows RPW w .. RPW w + RPW - 1 (RPW TX / 16 x NT MFMA
+// tiles), and the patch halo is 10/8 instead of the 4-row kernel's 6/4 of the
+// rows.  Per 32-channel chunk the 10 x (TX + 2) pixel patch goes straight into
+// LDS (global_load_lds_dwordx4: no staging registers; lanes outside the image or
+// past cout load g_conv_zero16), and so do the chunk's BN x 9 x 32 weights, so
+// the tap loop issues no global loads: 78 KB of LDS for 64 channels x 64 pixels
+// (two workgroups per CU, as the registers allow), 40 KB for 32 channels x 32
+// pixels (four).  Other shapes take the weight fragments from L2 a tap ahead.
+// With LDS_OUT the finished tile leaves through LDS in 16-byte vectors, with the
+// optional 2 x 2 max pool (4 pooled rows).  Needs W a multiple of TX, H a
 // multiple of 8, cin a multiple of 32.  The accumulators stay in VGPRs (two
 // waves per SIMD with RPW = 2, four with RPW = 1).
-template <int NT, bool LDS_OUT, int RPW>
+template <int NT, bool LDS_OUT, int RPW, int TXP = 64>
 __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / RPW))) void conv3_t8_kernel(Conv3 op)
 {
-    constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, MT = 4 * RPW, BN = NT * 16;
+    constexpr int TX = TXP, CG = TX / 16, TY = 8, PX = TX + 2, PY = TY + 2, MT = CG * RPW, BN = NT * 16;
     constexpr int WAVES = TY / RPW, NTH = 64 * WAVES;
     constexpr int NVEC = PY * PX * 4;        // 16-byte vectors of a patch (32 channels)
     constexpr int PINS = (NVEC + 63) / 64;   // wave loads per patch
-    constexpr bool WL = NT >= 4;            // weights staged in LDS
+    constexpr bool WL = NT >= 4 || TX <= 32;  // weights staged in LDS
     constexpr int WV = WL ? BN * 36 : 0;    // 16-byte vectors of a chunk's weights: [BN][9 taps][4]
     constexpr int WINS = (WV + 63) / 64;     // wave loads per chunk of weights
     constexpr int STAGED = (PINS + WINS) * 64;
@@ -836,7 +835,7 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
             bf16x8 afr[MT];
 #pragma unroll
             for (int i = 0; i < MT; i++)
-                afr[i] = patch[((RPW * wave + i / 4 + dy) * PX + (i % 4) * 16 + r + dx) * 4 + q];
+                afr[i] = patch[((RPW * wave + i / CG + dy) * PX + (i % CG) * 16 + r + dx) * 4 + q];
             // weights as the row operand: lane (r, q) receives channels 4q .. 4q + 3 of pixel r
 #pragma unroll
             for (int i = 0; i < MT; i++)
@@ -854,7 +853,7 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
     if constexpr (!LDS_OUT) {
 #pragma unroll
         for (int i = 0; i < MT; i++) {
-            const long m = pix0 + ((long) (RPW * wave + i / 4) << op.lw) + (i % 4) * 16 + r;
+            const long m = pix0 + ((long) (RPW * wave + i / CG) << op.lw) + (i % CG) * 16 + r;
 #pragma unroll
             for (int j = 0; j < NT; j++) {
                 const long n = n0 + j * 16 + 4 * q;
@@ -866,7 +865,7 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
     bf16 *tile = reinterpret_cast<bf16 *>(sh);
 #pragma unroll
     for (int i = 0; i < MT; i++) {
-        const int row = RPW * wave + i / 4, col = (i % 4) * 16 + r;
+        const int row = RPW * wave + i / CG, col = (i % CG) * 16 + r;
 #pragma unroll
         for (int j = 0; j < NT; j++) {
             const int ch = j * 16 + 4 * q;
