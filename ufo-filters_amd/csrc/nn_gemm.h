@@ -100,10 +100,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(Op op)
     if constexpr (Op::kVecStore) {
         // finished bf16 values through LDS (D[4q + e][r] of tile (i, j) is column n0 + 16 j + 4 q + e
         // of row m0 + 16 i + r: one 8-byte write per tile), then 16-byte stores of 8 consecutive
-        // columns
+        // columns; a tile wholly inside M x N skips the per-element bounds tests, and the stores
+        // address from a per-tile base with 32-bit offsets
         __shared__ bf16x8 tile[BM * BN / 8];
         bf16 *t = reinterpret_cast<bf16 *>(tile);
         const int mw = (wave % WM) * MT * 16, nw = (wave / WM) * NT * 16;
+        const long mb = (long) blockIdx.x * BM, nb = (long) blockIdx.y * BN;
+        const bool full = op.inside(mb + BM - 1, nb + BN - 1);
 #pragma unroll
         for (int i = 0; i < MT; i++)
 #pragma unroll
@@ -112,14 +115,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(Op op)
                 bf16x4 o;
 #pragma unroll
                 for (int e = 0; e < 4; e++)
-                    o[e] = (bf16) (op.valid(m, n + e) ? op.finish(m, n + e, acc[i][j][e]) : 0.0f);
+                    o[e] = (bf16) (full || op.valid(m, n + e) ? op.finish(m, n + e, acc[i][j][e]) : 0.0f);
                 *reinterpret_cast<bf16x4 *>(t + (mw + i * 16 + r) * BN + nw + j * 16 + 4 * q) = o;
             }
         __syncthreads();
-        const long mb = (long) blockIdx.x * BM, nb = (long) blockIdx.y * BN;
+        const typename Op::TileBase tb = op.tile_base(mb);
         for (int v = threadIdx.x; v < BM * BN / 8; v += 256) {
-            const long m = mb + v / (BN / 8), n = nb + (v % (BN / 8)) * 8;
-            if (op.valid(m, n)) op.store8(m, n, tile[v]);
+            const int lm = v / (BN / 8);
+            const long n = nb + (v % (BN / 8)) * 8;
+            if (full || op.valid(mb + lm, n)) op.store8(tb, lm, (int) n, tile[v]);
         }
     } else {
 #pragma unroll
@@ -296,9 +300,23 @@ struct UpFwd {
     }
     __device__ __forceinline__ bf16x8 load_b(const ColB &c, long k) const { return c.ok ? ld8(c.w + k) : zero8(); }
     __device__ __forceinline__ bool valid(long m, long n) const { return m < npix && n < (4L << lcout); }
+    __device__ __forceinline__ bool inside(long m, long n) const { return valid(m, n); }
     __device__ __forceinline__ float finish(long, long n, float v) const
     {
         return v + bias[n & ((1 << lcout) - 1)];
+    }
+    // Output base of the image holding input pixel mb, and mb's (y, x): the children of input
+    // pixel mb + lm are then 32-bit offsets from it (a tile's rows run on into the next image
+    // contiguously; 8 h w cs < 2^32, checked on the host).
+    struct TileBase {
+        bf16 *img0;
+        int y0, x0;
+    };
+    __device__ __forceinline__ TileBase tile_base(long mb) const
+    {
+        const long img = mb >> (lh + lw);
+        const Pix px = pix_of(mb, lh, lw);
+        return TileBase{out.ptr + (img << (lh + lw + 2)) * out.cs + out.c0, px.y, px.x};
     }
     __device__ __forceinline__ long child(long m, int ab) const
     {
@@ -306,10 +324,13 @@ struct UpFwd {
         const Pix px = pix_of(m, lh, lw);
         return (((img << (lh + 1)) + 2 * px.y + (ab >> 1)) << (lw + 1)) + 2 * px.x + (ab & 1);
     }
-    __device__ __forceinline__ void store8(long m, long n, bf16x8 v) const
+    // columns n .. n + 7 of input pixel tb + lm (one child pixel)
+    __device__ __forceinline__ void store8(const TileBase &tb, int lm, int n, bf16x8 v) const
     {
-        const int ab = (int) (n >> lcout), co = (int) (n & ((1 << lcout) - 1));
-        *reinterpret_cast<bf16x8 *>(out.at(child(m, ab), co)) = v;
+        const int ab = n >> lcout, co = n & ((1 << lcout) - 1);
+        const int xx = tb.x0 + lm, x = xx & ((1 << lw) - 1), y = tb.y0 + (xx >> lw);
+        const unsigned c = (unsigned) ((((2 * y + (ab >> 1)) << (lw + 1)) + 2 * x + (ab & 1)));
+        *reinterpret_cast<bf16x8 *>(tb.img0 + c * (unsigned) out.cs + (unsigned) co) = v;
     }
     __device__ __forceinline__ void store(long m, long n, float v, int) const
     {

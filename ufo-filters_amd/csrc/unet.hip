@@ -724,6 +724,9 @@ int forward(ufomi_unet *n, hipStream_t s, const float *img, int B, float *out)
         up.w = n->w16 + n->up[l].w;
         up.bias = n->master + n->up[l].b;
         up.out = mv(n->cat[l], 2 * C, 0);
+        if ((8L << (up.lh + up.lw)) * up.out.cs >= (1L << 32))
+            return ufomi_fail(UFOMI_ERR_SIZE, "transposed conv output of %ld pixels x %d channels exceeds 32-bit offsets",
+                              4L << (up.lh + up.lw), up.out.cs);
         if ((rc = gemm_fwd(s, up, up.npix, 4L * C))) return rc;
         const ConvP &a = n->dec1[l], &b = n->dec2[l];
         if ((rc = conv3(n, s, cv(n->cat[l], 2 * C), np, l, 2 * C, C, n->w16 + a.w, n->master + a.b, true,
