@@ -565,6 +565,25 @@ def unet_flops(size):
     return f + 2 * size * size * 32
 
 
+def unet_hbm_bytes(size):
+    """HBM bytes of one U-Net forward pass on a size^2 image with every activation written once and
+    read once by each consumer (no halo re-reads): the network's compulsory traffic."""
+    P = [(size >> lev) ** 2 for lev in range(4)]
+    C = [32 << lev for lev in range(4)]
+    b = 4 * P[0] + 2 * 32 * P[0] + 2 * P[0]  # first conv: float image in, e1 and bf16 image out
+    for lev in range(4):
+        if lev > 0:
+            b += 2 * C[lev - 1] * P[lev] + 2 * C[lev] * P[lev]  # first conv of the level: pooled in, e1 out
+        b += 2 * 2 * C[lev] * P[lev]  # second conv: e1 in, skip half (or bottom) out
+        if lev < 3:
+            b += 2 * C[lev] * P[lev] // 4  # its 2x2 max pool
+    for lev in range(2, -1, -1):
+        b += 2 * 2 * C[lev] * P[lev + 1] + 2 * C[lev] * P[lev]  # transposed conv: source in, up half out
+        b += 2 * 2 * C[lev] * P[lev] + 2 * C[lev] * P[lev]  # decoder conv a: concatenation in, d1 out
+        b += 2 * C[lev] * P[lev] + (2 * C[lev] * P[lev] if lev > 0 else 8 * P[0])  # conv b (level 0: + image, output)
+    return b
+
+
 def learned_fbp(args, ctx, torch, np, ufomi):
     """Learned sparse-view FBP of 512^2 textured phantoms: train the U-Net on pairs
     (sparse-view FBP, 1800-view FBP), then compare slices/s and RMSE to the phantom of
@@ -658,6 +677,7 @@ def learned_fbp(args, ctx, torch, np, ufomi):
     mfma_peak = mf / (e0.elapsed_time(e1) * 1e-3)
     flops = unet_flops(size) * B
     achieved = flops / (unet_ms * 1e-3)
+    hbm_bytes = unet_hbm_bytes(size) * B
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -678,9 +698,12 @@ def learned_fbp(args, ctx, torch, np, ufomi):
                "sample": "%d slices: C-oracle %d-view FBP + numpy float64 U-Net forward (same weights), %.1f s"
                          % (n_cpu, sparse, dt)}
     inf.close()
-    extra = {"kernel_ms": round(unet_ms, 3), "flops_per_forward": flops, "fbp_sparse_ms": round(fbp_sparse_ms, 3),
-             "mfma_peak_measured_TFLOPs": round(mfma_peak / 1e12, 1)}
+    extra = {"kernel_ms": round(unet_ms, 3), "algorithmic_bytes_per_forward": hbm_bytes,
+             "flops_per_forward": flops, "flop_per_byte": round(flops / hbm_bytes, 1),
+             "mfma_achieved_TFLOPs": round(achieved / 1e12, 1), "mfma_frac_of_2.5PFLOPs": round(achieved / 2.5e15, 4),
+             "mfma_peak_measured_TFLOPs": round(mfma_peak / 1e12, 1), "fbp_sparse_ms": round(fbp_sparse_ms, 3)}
     return dict(learned_sps=learned_sps, plain_sps=plain_sps, wall=wall_l, kernel_ms=unet_ms, achieved=achieved,
+                hbm_achieved=hbm_bytes / (unet_ms * 1e-3),
                 extra=extra, cpu=cpu, accuracy=acc,
                 training={"pairs": n_train, "steps": steps, "batch": batch, "lr": 1e-3, "optimizer": "Adam",
                           "seconds": round(train_s, 1), "data_seconds": round(gen_s, 1),
@@ -752,9 +775,12 @@ def main():
             "config": {"workload": r["workload"]},
             "plain_1800_view_slices_per_s": round(r["plain_sps"], 1),
             "learned_vs_plain": round(r["learned_sps"] / r["plain_sps"], 3),
-            "roofline": dict({"kernel": "U-Net forward (conv3_t8_kernel, gemm_kernel<...,UpFwd> + VALU layers)", "bound": "mfma",
-                              "achieved": round(r["achieved"] / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
-                              "frac": round(r["achieved"] / 2.5e15, 4), "traffic": None}, **r["extra"]),
+            # 18.8 TFLOP over 73.4 GB of compulsory traffic is 256 flop/B, below the 312 flop/B at which
+            # 2.5 PFLOP/s and 8 TB/s meet: the forward pass is bound by HBM, and its MFMA rate is reported beside it
+            "roofline": dict({"kernel": "U-Net forward (conv3_t8_kernel, gemm_kernel<...,UpFwd> + VALU layers)", "bound": "hbm",
+                              "achieved": round(r["hbm_achieved"] / 1e9, 1), "peak": MI355X_HBM_TBPS * 1e3, "unit": "GB/s",
+                              "frac": round(r["hbm_achieved"] / (MI355X_HBM_TBPS * 1e12), 4),
+                              "traffic": traffic_for("U-Net forward learned512 batch256")}, **r["extra"]),
             "accuracy": r["accuracy"], "training": r["training"],
         }
         if r["cpu"] is not None:

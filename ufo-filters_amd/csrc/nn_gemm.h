@@ -776,13 +776,27 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
     const int r = lane & 15, q = lane >> 4;
     const int W = 1 << op.lw, H = 1 << op.lh;
     const int tiles_x = W / TX, tiles_y = H / TY;
-    int b = blockIdx.x;
+    // One flat grid of (pixel tile, channel tile) workgroups.  Workgroup w runs on XCD w % 8:
+    // each XCD takes a contiguous eighth of the pixel tiles (a tile's upper neighbour, whose
+    // patch shares two rows with it, ran on the same XCD shortly before), and the gy channel
+    // tiles of a pixel tile take consecutive slots of one XCD, so all but the first find the
+    // patch in that XCD's L2 (when the pixel tiles are a multiple of 8).
+    const int gy = (op.cout + BN - 1) / BN, nw = (int) gridDim.x;
+    int b, ny;
+    if (((nw / gy) & 7) == 0) {
+        const int xcd = (int) blockIdx.x & 7, slot = (int) blockIdx.x >> 3;
+        ny = slot % gy;
+        b = xcd * (nw / gy / 8) + slot / gy;
+    } else {
+        ny = (int) blockIdx.x % gy;
+        b = (int) blockIdx.x / gy;
+    }
     const int tx = b % tiles_x;
     b /= tiles_x;
     const int ty = b % tiles_y;
     const long img = b / tiles_y;
     const int x0 = tx * TX, y0 = ty * TY;
-    const long n0 = (long) blockIdx.y * BN;
+    const long n0 = (long) ny * BN;
     const long pix0 = (img << (op.lh + op.lw)) + ((long) y0 << op.lw) + x0;  // pixel (y0, x0)
     const int cin = 1 << op.lcin;
     typedef __attribute__((address_space(3))) void lds_void;

@@ -538,8 +538,15 @@ int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, i
         // channels per workgroup the direct 8-byte stores measured faster.  Images of at least
         // 8 rows take the 8-row tiles, 32 pixels wide for 32 output channels (measured: forward
         // 34.2 ms against 34.7 ms with 64-pixel tiles and L2 weight fragments).
-        auto launch = [&](auto kern, long nt, int threads = 256) {
-            hipLaunchKernelGGL(kern, dim3((unsigned) nt, (unsigned) gy), dim3(threads), 0, s, op);
+        auto launch = [&](auto kern, long nt) {
+            hipLaunchKernelGGL(kern, dim3((unsigned) nt, (unsigned) gy), dim3(256), 0, s, op);
+        };
+        // the 8-row kernel takes one flat grid of pixel tiles x channel tiles (at most `tiles` of
+        // them, the 4-row count)
+        if (lh >= 3 && tiles * gy > 0x7fffffffL)
+            return ufomi_fail(UFOMI_ERR_SIZE, "conv grid too large");
+        auto launch8 = [&](auto kern, long nt) {
+            hipLaunchKernelGGL(kern, dim3((unsigned) (nt * gy)), dim3(256), 0, s, op);
         };
         if (lh >= 3) {
             if (fin && cout == 32) {
@@ -550,9 +557,9 @@ int conv3(ufomi_unet *n, hipStream_t s, View<const bf16> in, long npix, int l, i
                 if (!fin->keep_d) op.out.ptr = nullptr;
                 *fused = true;
             }
-            if (NT == 2) launch(nn::conv3_t8_kernel<2, true, 2, 32>, tiles);
-            else if (pool.ptr) launch(nn::conv3_t8_kernel<4, true, 2>, tiles / 2);
-            else launch(nn::conv3_t8_kernel<4, false, 2>, tiles / 2);
+            if (NT == 2) launch8(nn::conv3_t8_kernel<2, true, 2, 32>, tiles);
+            else if (pool.ptr) launch8(nn::conv3_t8_kernel<4, true, 2>, tiles / 2);
+            else launch8(nn::conv3_t8_kernel<4, false, 2>, tiles / 2);
         } else {
             if (NT == 2) launch(nn::conv3_lds_kernel<2, true>, tiles);
             else if (pool.ptr) launch(nn::conv3_lds_kernel<4, true>, tiles);
