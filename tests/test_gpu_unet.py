@@ -27,9 +27,14 @@ def _bf16_tensor(a, torch_dev):
 
 
 @pytest.mark.parametrize("cin,cout,H,W", [(32, 32, 16, 32), (64, 64, 16, 32), (32, 128, 16, 32), (128, 64, 16, 32),
-                                          (32, 32, 8, 128), (64, 64, 4, 64), (32, 128, 16, 64), (256, 64, 8, 64)])
+                                          (32, 32, 8, 128), (64, 64, 4, 64), (32, 128, 16, 64), (256, 64, 8, 64),
+                                          (64, 48, 16, 64), (32, 8, 8, 64), (64, 192, 8, 64), (32, 64, 64, 64),
+                                          (32, 128, 64, 64)])
 def test_conv3x3_mfma_vs_oracle(ctx, torch_dev, cin, cout, H, W):
-    """W >= 64 runs the LDS-staged kernel (4 x 64 pixel patches), narrower images the generic GEMM."""
+    """W >= 64 runs the LDS-staged kernels (8-row tiles from 8 rows on, 4-row below), narrower
+    images the generic GEMM.  cout 48 and 8 leave channel tiles partly past cout; cout 192 is three
+    channel tiles of one pixel tile; 64 x 64 images of two are 16 pixel tiles, which the grid
+    spreads over the XCDs in contiguous ranges (with 128 output channels, two channel tiles each)."""
     import torch
     rng = np.random.default_rng(cin + cout)
     B = 2
