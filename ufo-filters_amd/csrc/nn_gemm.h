@@ -802,28 +802,49 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
     typedef __attribute__((address_space(3))) void lds_void;
     const void *zero16 = g_conv_zero16;
 
-    auto stage = [&](int c0) {
-        for (int j = wave; j < PINS; j += WAVES) {
-            const int v = j * 64 + lane;
-            const void *src = zero16;
-            if (v < NVEC) {
-                const int p = v >> 2, g = v & 3, pc = p % PX, pr = p / PX;
-                const int yy = y0 + pr - 1, xx = x0 + pc - 1;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                    src = op.in.at(pix0 + (long) (pr - 1) * W + (pc - 1), c0 + 8 * g);
-            }
-            __builtin_amdgcn_global_load_lds(src, (lds_void *) (patch + j * 64), 16, 0, 0);
+    // The staged vectors' sources differ between chunks only by c0: their element offsets (or
+    // kNone for a zero vector) are formed once.  Patch vector v = (pixel p, channel group g);
+    // weight vector u = (output channel co, tap, channel group g) of w[n0 + co][tap][c0 + 8 g ..].
+    constexpr int PJ = (PINS + WAVES - 1) / WAVES, WJ = (WINS + WAVES - 1) / WAVES, kNone = -1;
+    const bf16 *ibase = op.in.ptr + pix0 * op.in.cs + op.in.c0;
+    const bf16 *wbase = op.w + n0 * (9L << op.lcin);
+    int poff[PJ];
+    [[maybe_unused]] int woff[WJ > 0 ? WJ : 1];
+#pragma unroll
+    for (int k = 0; k < PJ; k++) {
+        const int v = (wave + WAVES * k) * 64 + lane;
+        poff[k] = kNone;
+        if (v < NVEC) {
+            const int p = v >> 2, g = v & 3, pc = p % PX, pr = p / PX;
+            const int yy = y0 + pr - 1, xx = x0 + pc - 1;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) poff[k] = ((pr - 1) * W + (pc - 1)) * op.in.cs + 8 * g;
         }
-        // weights: vector u = (output channel co, tap, channel group g) of w[n0 + co][tap][c0 + 8 g ..]
-        for (int j = wave; WL && j < WINS; j += WAVES) {
-            const int u = j * 64 + lane;
-            const void *src = zero16;
-            if (u < WV) {
-                const int co = u / 36, rem = u - 36 * co, tap = rem >> 2, g = rem & 3;
-                if (n0 + co < op.cout)
-                    src = op.w + (n0 + co) * (9L << op.lcin) + ((long) tap << op.lcin) + c0 + 8 * g;
+    }
+#pragma unroll
+    for (int k = 0; k < WJ; k++) {
+        const int u = (wave + WAVES * k) * 64 + lane;
+        woff[k] = kNone;
+        if (u < WV) {
+            const int co = u / 36, rem = u - 36 * co, tap = rem >> 2, g = rem & 3;
+            if (n0 + co < op.cout) woff[k] = co * (9 << op.lcin) + (tap << op.lcin) + 8 * g;
+        }
+    }
+    auto stage = [&](int c0) {
+#pragma unroll
+        for (int k = 0; k < PJ; k++) {
+            const int j = wave + WAVES * k;
+            if (j < PINS) {
+                const void *src = poff[k] != kNone ? (const void *) (ibase + poff[k] + c0) : zero16;
+                __builtin_amdgcn_global_load_lds(src, (lds_void *) (patch + j * 64), 16, 0, 0);
             }
-            __builtin_amdgcn_global_load_lds(src, (lds_void *) (wts + j * 64), 16, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < WJ; k++) {
+            const int j = wave + WAVES * k;
+            if (WL && j < WINS) {
+                const void *src = woff[k] != kNone ? (const void *) (wbase + woff[k] + c0) : zero16;
+                __builtin_amdgcn_global_load_lds(src, (lds_void *) (wts + j * 64), 16, 0, 0);
+            }
         }
     };
 
@@ -855,9 +876,13 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
         if constexpr (!WL) load_b(bcur, 0, c0);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();  // patch (and weights) of chunk ch in LDS
+        // rows of taps: dy in a loop, dx unrolled (the patch and weight offsets of a row are
+        // immediates from one base)
 #pragma unroll 1
-        for (int tap = 0; tap < 9; tap++) {
-            const int dy = tap / 3, dx = tap % 3;  // patch offsets (0..2)
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+            const int tap = 3 * dy + dx;  // patch offsets (dy, dx), 0..2
             bf16x8 bfr[NT];
             if constexpr (WL) {
 #pragma unroll
