@@ -108,16 +108,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(Op op)
         const long mb = (long) blockIdx.x * BM, nb = (long) blockIdx.y * BN;
         const bool full = op.inside(mb + BM - 1, nb + BN - 1);
 #pragma unroll
-        for (int i = 0; i < MT; i++)
+        for (int j = 0; j < NT; j++) {
+            // the 4 columns' bias, loaded once and unconditionally (loads under the bounds
+            // test would each wait on their own)
+            const long n = n0 + j * 16 + 4 * q;
+            const f32x4 bj = op.bias4(n);
 #pragma unroll
-            for (int j = 0; j < NT; j++) {
-                const long m = m0 + i * 16 + r, n = n0 + j * 16 + 4 * q;
+            for (int i = 0; i < MT; i++) {
+                const long m = m0 + i * 16 + r;
                 bf16x4 o;
 #pragma unroll
                 for (int e = 0; e < 4; e++)
-                    o[e] = (bf16) (full || op.valid(m, n + e) ? op.finish(m, n + e, acc[i][j][e]) : 0.0f);
+                    o[e] = (bf16) (full || op.valid(m, n + e) ? acc[i][j][e] + bj[e] : 0.0f);
                 *reinterpret_cast<bf16x4 *>(t + (mw + i * 16 + r) * BN + nw + j * 16 + 4 * q) = o;
             }
+        }
         __syncthreads();
         const typename Op::TileBase tb = op.tile_base(mb);
         for (int v = threadIdx.x; v < BM * BN / 8; v += 256) {
@@ -236,14 +241,18 @@ struct Conv3 {
         if (m >= npix || n >= cout) return;
         *out.at(m, (int) n) = (bf16) finish(m, n, v);
     }
-    // finish() of the four output channels n .. n + 3 of pixel m (n a multiple of 4,
-    // n + 3 < cout), rounded to bf16
-    __device__ __forceinline__ bf16x4 finish4(long m, long n, f32x4 v) const
+    // bias of output channels n .. n + 3 (n a multiple of 4, n + 3 < cout; zeros without bias)
+    __device__ __forceinline__ f32x4 bias4(long n) const
     {
-        if (bias) {
+        if (!bias) return f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        return f32x4{bias[n], bias[n + 1], bias[n + 2], bias[n + 3]};
+    }
+    // finish() of the four output channels n .. n + 3 of pixel m (n a multiple of 4,
+    // n + 3 < cout) with their bias b (bias4(n)), rounded to bf16
+    __device__ __forceinline__ bf16x4 finish4(long m, long n, f32x4 v, f32x4 b) const
+    {
 #pragma unroll
-            for (int e = 0; e < 4; e++) v[e] += bias[n + e];
-        }
+        for (int e = 0; e < 4; e++) v[e] += b[e];
         if (relu) {
 #pragma unroll
             for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.0f);
@@ -304,6 +313,12 @@ struct UpFwd {
     __device__ __forceinline__ float finish(long, long n, float v) const
     {
         return v + bias[n & ((1 << lcout) - 1)];
+    }
+    // bias of columns n .. n + 3 (n a multiple of 4; any n, as finish reads it)
+    __device__ __forceinline__ f32x4 bias4(long n) const
+    {
+        const float *b = bias + (n & ((1 << lcout) - 1));
+        return f32x4{b[0], b[1], b[2], b[3]};
     }
     // Output base of the image holding input pixel mb, and mb's (y, x): the children of input
     // pixel mb + lm are then 32-bit offsets from it (a tile's rows run on into the next image
@@ -910,6 +925,13 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
         __syncthreads();  // every wave done with this chunk before the next is staged
     }
     // D[4q + e][r] of tile (i, j): channel n0 + 16 j + 4 q + e of pixel (y0 + RPW wave + i / 4, x0 + 16 (i % 4) + r)
+    // the bias of this lane's channels, loaded once and unconditionally (clamped into cout)
+    f32x4 bq[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const long n = n0 + j * 16 + 4 * q;
+        bq[j] = op.bias4(n < op.cout ? n : 0);
+    }
     if constexpr (!LDS_OUT) {
 #pragma unroll
         for (int i = 0; i < MT; i++) {
@@ -917,7 +939,8 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
 #pragma unroll
             for (int j = 0; j < NT; j++) {
                 const long n = n0 + j * 16 + 4 * q;
-                if (n < op.cout) *reinterpret_cast<bf16x4 *>(op.out.at(m, (int) n)) = op.finish4(m, n, acc[i][j]);
+                if (n < op.cout)
+                    *reinterpret_cast<bf16x4 *>(op.out.at(m, (int) n)) = op.finish4(m, n, acc[i][j], bq[j]);
             }
         }
         return;
@@ -932,7 +955,7 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
             const long n = n0 + ch;
             bf16x4 v;
             if (n < op.cout) {
-                v = op.finish4(pix0 + ((long) row << op.lw) + col, n, acc[i][j]);
+                v = op.finish4(pix0 + ((long) row << op.lw) + col, n, acc[i][j], bq[j]);
             } else {
 #pragma unroll
                 for (int e = 0; e < 4; e++) v[e] = (bf16) 0.0f;
