@@ -241,6 +241,12 @@ struct Conv3 {
         if (m >= npix || n >= cout) return;
         *out.at(m, (int) n) = (bf16) finish(m, n, v);
     }
+    // mask values at (m, n .. n + 3) (m < npix, n a multiple of 4, n + 3 < cout), read
+    // unconditionally so that a tile's loads issue together
+    __device__ __forceinline__ bf16x4 mask4(long m, long n) const
+    {
+        return *reinterpret_cast<const bf16x4 *>(mask.at(m, (int) n));
+    }
     // bias of output channels n .. n + 3 (n a multiple of 4, n + 3 < cout; zeros without bias)
     __device__ __forceinline__ f32x4 bias4(long n) const
     {
@@ -249,7 +255,8 @@ struct Conv3 {
     }
     // finish() of the four output channels n .. n + 3 of pixel m (n a multiple of 4,
     // n + 3 < cout) with their bias b (bias4(n)), rounded to bf16
-    __device__ __forceinline__ bf16x4 finish4(long m, long n, f32x4 v, f32x4 b) const
+    // and, with a mask, its values mk at (m, n .. n + 3) (mask4)
+    __device__ __forceinline__ bf16x4 finish4(long n, f32x4 v, f32x4 b, bf16x4 mk) const
     {
 #pragma unroll
         for (int e = 0; e < 4; e++) v[e] += b[e];
@@ -258,7 +265,6 @@ struct Conv3 {
             for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.0f);
         }
         if (mask.ptr) {
-            const bf16x4 mk = *reinterpret_cast<const bf16x4 *>(mask.at(m, (int) n));
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 if (n + e >= mask_from && !((float) mk[e] > 0.0f)) v[e] = 0.0f;
@@ -932,15 +938,27 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
         const long n = n0 + j * 16 + 4 * q;
         bq[j] = op.bias4(n < op.cout ? n : 0);
     }
+    // a pixel row's masks (data gradients), all NT loads issued before their use
+    auto masks = [&](long m, bf16x4 *mk) {
+        if (op.mask.ptr) {
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                const long n = n0 + j * 16 + 4 * q;
+                mk[j] = op.mask4(m, n < op.cout ? n : 0);
+            }
+        }
+    };
     if constexpr (!LDS_OUT) {
 #pragma unroll
         for (int i = 0; i < MT; i++) {
             const long m = pix0 + ((long) (RPW * wave + i / CG) << op.lw) + (i % CG) * 16 + r;
+            bf16x4 mk[NT];
+            masks(m, mk);
 #pragma unroll
             for (int j = 0; j < NT; j++) {
                 const long n = n0 + j * 16 + 4 * q;
                 if (n < op.cout)
-                    *reinterpret_cast<bf16x4 *>(op.out.at(m, (int) n)) = op.finish4(m, n, acc[i][j], bq[j]);
+                    *reinterpret_cast<bf16x4 *>(op.out.at(m, (int) n)) = op.finish4(n, acc[i][j], bq[j], mk[j]);
             }
         }
         return;
@@ -949,13 +967,15 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
 #pragma unroll
     for (int i = 0; i < MT; i++) {
         const int row = RPW * wave + i / CG, col = (i % CG) * 16 + r;
+        bf16x4 mk[NT];
+        masks(pix0 + ((long) row << op.lw) + col, mk);
 #pragma unroll
         for (int j = 0; j < NT; j++) {
             const int ch = j * 16 + 4 * q;
             const long n = n0 + ch;
             bf16x4 v;
             if (n < op.cout) {
-                v = op.finish4(pix0 + ((long) row << op.lw) + col, n, acc[i][j], bq[j]);
+                v = op.finish4(n, acc[i][j], bq[j], mk[j]);
             } else {
 #pragma unroll
                 for (int e = 0; e < 4; e++) v[e] = (bf16) 0.0f;
