@@ -37,25 +37,38 @@ __device__ __forceinline__ float bfr(float v) { return (float) (bf16) v; }
 // -------------------------------------------------------------- VALU layers
 
 // e1 = relu(conv3x3(bf16(img)) + b), 1 -> 32 channels; also stores bf16(img).
+// One pixel per thread.  The 9 taps load from clamped addresses without a
+// branch (zero padding by select), so the loads issue together instead of each
+// waiting under its own bounds test; the 64-byte outputs of a workgroup's 256
+// pixels go out through LDS as contiguous 16-byte stores.
 __global__ __launch_bounds__(256) void first_conv_kernel(const float *__restrict__ img, long npix, int lh, int lw,
                                                          const bf16 *__restrict__ w, const float *__restrict__ b,
                                                          bf16 *__restrict__ img16, bf16 *__restrict__ out)
 {
     __shared__ float ws[kBase * 9 + kBase];
-    for (int i = threadIdx.x; i < kBase * 9; i += 256) ws[i] = (float) w[i];
-    for (int i = threadIdx.x; i < kBase; i += 256) ws[kBase * 9 + i] = b[i];
-    __syncthreads();
-    const long p = (long) blockIdx.x * 256 + threadIdx.x;
-    if (p >= npix) return;
-    const nn::Pix px = nn::pix_of(p, lh, lw);
-    float x[9];
+    __shared__ nn::bf16x8 tile[256 * kBase / 8];
+    const long p0 = (long) blockIdx.x * 256, p = p0 + threadIdx.x;
+    const long pc = p < npix ? p : npix - 1;  // a pixel past the end computes pixel npix - 1, unstored
+    const nn::Pix px = nn::pix_of(pc, lh, lw);
+    const float *row0 = img + (pc - px.x - ((long) px.y << lw));  // this image
+    const int H = 1 << lh, W = 1 << lw;
+    float v[9];
+    bool in[9];
 #pragma unroll
     for (int t = 0; t < 9; t++) {
         const int yy = px.y + t / 3 - 1, xx = px.x + t % 3 - 1;
-        x[t] = (yy >= 0 && yy < (1 << lh) && xx >= 0 && xx < (1 << lw))
-                   ? bfr(img[p + (long) (t / 3 - 1) * (1L << lw) + (t % 3 - 1)]) : 0.0f;
+        in[t] = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const int yc = min(max(yy, 0), H - 1), xc = min(max(xx, 0), W - 1);
+        v[t] = row0[((long) yc << lw) + xc];
     }
-    img16[p] = (bf16) img[p];
+    // the weights, while the taps are in flight
+    for (int i = threadIdx.x; i < kBase * 9; i += 256) ws[i] = (float) w[i];
+    for (int i = threadIdx.x; i < kBase; i += 256) ws[kBase * 9 + i] = b[i];
+    float x[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) x[t] = in[t] ? (float) (bf16) v[t] : 0.0f;
+    if (p < npix) img16[p] = (bf16) v[4];  // tap 4 is the pixel itself
+    __syncthreads();  // ws filled
     nn::bf16x8 o[kBase / 8];
 #pragma unroll
     for (int c = 0; c < kBase; c++) {
@@ -64,9 +77,16 @@ __global__ __launch_bounds__(256) void first_conv_kernel(const float *__restrict
         for (int t = 0; t < 9; t++) a = fmaf(ws[c * 9 + t], x[t], a);
         o[c / 8][c % 8] = (bf16) fmaxf(a + ws[kBase * 9 + c], 0.0f);
     }
-    nn::bf16x8 *dst = reinterpret_cast<nn::bf16x8 *>(out + p * kBase);
 #pragma unroll
-    for (int i = 0; i < kBase / 8; i++) dst[i] = o[i];
+    for (int i = 0; i < kBase / 8; i++) tile[threadIdx.x * (kBase / 8) + i] = o[i];
+    __syncthreads();
+    nn::bf16x8 *dst = reinterpret_cast<nn::bf16x8 *>(out + p0 * kBase);
+    const long nvec = (npix - p0 < 256 ? npix - p0 : 256) * (kBase / 8);
+#pragma unroll
+    for (int k = 0; k < kBase / 8; k++) {
+        const int vi = threadIdx.x + 256 * k;
+        if (vi < nvec) dst[vi] = tile[vi];
+    }
 }
 
 // out = img + sum_c w[c] d[p][c] + b (fp32).
