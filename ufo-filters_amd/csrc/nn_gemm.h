@@ -869,7 +869,8 @@ __global__ __launch_bounds__(512 / RPW) __attribute__((amdgpu_waves_per_eu(4 / R
         }
     };
 
-    // weight fragments from L2 (NT < 4): row n0 + 16 j + r, loaded a tap ahead
+    // weight fragments from L2 when they are not staged (32 channels on 64-pixel tiles): row
+    // n0 + 16 j + r, loaded a tap ahead
     const bf16 *wrow[NT];
     bool wok[NT];
 #pragma unroll
