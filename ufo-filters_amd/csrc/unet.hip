@@ -32,8 +32,6 @@ namespace {
 constexpr int kLevels = 4;
 constexpr int kBase = 32;
 
-__device__ __forceinline__ float bfr(float v) { return (float) (bf16) v; }
-
 // -------------------------------------------------------------- VALU layers
 
 // e1 = relu(conv3x3(bf16(img)) + b), 1 -> 32 channels; also stores bf16(img).
