@@ -224,6 +224,9 @@ constexpr int kBoxW = 64, kBoxH = 48;  // texels per staged footprint
 // of 70, 74 and 76 floats measured the same or slower, 80 much slower).
 constexpr int kBoxS = kBoxW + 8;
 constexpr int kChunk = 32;             // projections per footprint chunk
+// Word 3 of a raw (untyped, unswizzled) buffer resource on gfx9-family GPUs: 32-bit data format,
+// so loads past num_records return 0.
+constexpr int kRawBufferFlags = 0x00020000;
 constexpr int kRowsPerThread = kBoxH / 4;
 
 struct BoxGeom {
@@ -352,30 +355,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     const int sc = tid & 63, sr0 = __builtin_amdgcn_readfirstlane(tid >> 6);
     float st[kRowsPerThread];
     const long plane = (long) W * H;
+    // A row of a box is one load per lane with no VALU work: the row's address is scalar and
+    // the lane adds its column offset.  With a zero border, each row is a buffer resource
+    // whose range is the detector row (empty for a row off the detector), and a column off the
+    // detector reads at the range's end, so the hardware returns 0 for either; otherwise rows
+    // and columns are clamped.  Texels past the box's width or height are never sampled.
     auto stage_load = [&](int p, const BoxGeom &g) {
-        const Glb<float> *img = (const Glb<float> *) (proj + (long) p * plane);
-        int i = g.i_lo + sc;
-        bool ci = sc < g.w;
-        if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
-            ci = ci && i >= 0 && i < W;
-        } else {
-            i = min(max(i, 0), W - 1);
-        }
-        const unsigned ib = (unsigned) i * 4u;
+        const char *img = (const char *) (proj + (long) p * plane);
+        const int i = g.i_lo + sc;
+        const unsigned ib = ADDR == UFOMI_ADDR_CLAMP ? (unsigned) (i >= 0 && i < W ? i : W) * 4u
+                                                     : (unsigned) min(max(i, 0), W - 1) * 4u;
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; k++) {
             const int r = sr0 + 4 * k;
-            int j = g.j_lo + r;
-            bool rok = r < g.h;
+            if (r >= g.h) continue;
+            const int j = g.j_lo + r;
+            // byte offsets inside one projection (< 4 GiB, checked on the host)
             if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
-                rok = rok && j >= 0 && j < H;
+                const bool rin = j >= 0 && j < H;
+                const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *) (img + (rin ? (unsigned) j * (unsigned) W * 4u : 0u)), (short) 0, rin ? W * 4 : 0,
+                    kRawBufferFlags);
+                st[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(row, ib, 0, 0));
             } else {
-                j = min(max(j, 0), H - 1);
+                const Glb<char> *row = (const Glb<char> *) img + (unsigned) min(max(j, 0), H - 1) * (unsigned) W * 4u;
+                st[k] = *(const Glb<float> *) (row + ib);
             }
-            // byte offsets inside one projection (< 4 GiB, checked on the host):
-            // a scalar row offset plus the lane's column
-            const unsigned rowb = (unsigned) j * (unsigned) W * 4u;
-            st[k] = (rok && ci) ? *(const Glb<float> *) ((const Glb<char> *) img + (rowb + ib)) : 0.0f;
         }
     };
     auto commit = [&](int buf, const BoxGeom &g) {
