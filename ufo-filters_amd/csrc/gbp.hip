@@ -355,32 +355,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
     const int sc = tid & 63, sr0 = __builtin_amdgcn_readfirstlane(tid >> 6);
     float st[kRowsPerThread];
     const long plane = (long) W * H;
-    // A row of a box is one load per lane with no VALU work: the row's address is scalar and
-    // the lane adds its column offset.  With a zero border, each row is a buffer resource
-    // whose range is the detector row (empty for a row off the detector), and a column off the
-    // detector reads at the range's end, so the hardware returns 0 for either; otherwise rows
-    // and columns are clamped.  Texels past the box's width or height are never sampled.
+    // A row of a box is one load per lane with no VALU work: each row is a buffer resource
+    // over the detector row (scalar), and the lane adds its column offset.  A lane whose column
+    // lies past the box's width, or off the detector with a zero border, reads at the range's
+    // end, and a row off the detector with a zero border has an empty range: the hardware
+    // returns 0 for those without a memory access.  Other addressing modes clamp rows and
+    // columns.  Texels past the box's width or height are never sampled.
     auto stage_load = [&](int p, const BoxGeom &g) {
         const char *img = (const char *) (proj + (long) p * plane);
-        const int i = g.i_lo + sc;
-        const unsigned ib = ADDR == UFOMI_ADDR_CLAMP ? (unsigned) (i >= 0 && i < W ? i : W) * 4u
-                                                     : (unsigned) min(max(i, 0), W - 1) * 4u;
+        int i = g.i_lo + sc;
+        bool ci = sc < g.w;
+        if constexpr (ADDR == UFOMI_ADDR_CLAMP)
+            ci = ci && i >= 0 && i < W;
+        else
+            i = min(max(i, 0), W - 1);
+        const unsigned ib = (unsigned) (ci ? i : W) * 4u;
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; k++) {
             const int r = sr0 + 4 * k;
             if (r >= g.h) continue;
-            const int j = g.j_lo + r;
-            // byte offsets inside one projection (< 4 GiB, checked on the host)
+            int j = g.j_lo + r;
+            bool rin = true;
             if constexpr (ADDR == UFOMI_ADDR_CLAMP) {
-                const bool rin = j >= 0 && j < H;
-                const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
-                    (void *) (img + (rin ? (unsigned) j * (unsigned) W * 4u : 0u)), (short) 0, rin ? W * 4 : 0,
-                    kRawBufferFlags);
-                st[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(row, ib, 0, 0));
+                rin = j >= 0 && j < H;
+                j = rin ? j : 0;
             } else {
-                const Glb<char> *row = (const Glb<char> *) img + (unsigned) min(max(j, 0), H - 1) * (unsigned) W * 4u;
-                st[k] = *(const Glb<float> *) (row + ib);
+                j = min(max(j, 0), H - 1);
             }
+            // byte offsets inside one projection (< 4 GiB, checked on the host)
+            const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+                (void *) (img + (unsigned) j * (unsigned) W * 4u), (short) 0, rin ? W * 4 : 0, kRawBufferFlags);
+            st[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(row, ib, 0, 0));
         }
     };
     auto commit = [&](int buf, const BoxGeom &g) {
