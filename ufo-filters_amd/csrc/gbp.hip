@@ -415,6 +415,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
         }
     }
     __syncthreads();
+    bool any_global = false;  // workgroup-uniform, like the boxes
     for (int p = 0; p < P; p++) {
         if (p % kChunk == 0 && p + kChunk < P + kChunk - 1)
             footprints(p + kChunk, ((p / kChunk) + 1) & 1);  // read a chunk later
@@ -491,36 +492,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
                 }
             }
         } else {
-            // Global-memory path for this projection (footprint too large or
-            // behind the source).  One voxel per iteration of a rolled loop and
-            // a select chain into the accumulators, so this rare path does not
-            // inflate the register budget of the LDS path.
-            const float *img = proj + (long) p * plane;
-#pragma unroll 1
-            for (int q = 0; q < KA * KB * TZ; q++) {
-                const int a = q / (KB * TZ), b = (q / TZ) % KB, k = q % TZ;
-                const float x = a ? xv[KA - 1] : xv[0], y = b ? yv[KB - 1] : yv[0];
-                const float zz = zs[min(Z0 + k, nz - 1)];
-                float u, v, d;
-                map_point<CONE>(mu, mv, md, x, y, zz, u, v, d);
-                Col<ADDR> col;
-                col.set(u, W);
-                float val = fetch<ADDR>(img, W, H, col, v - 0.5f);
-                if constexpr (CONE) {
-                    const float cf = 1.0f / fmaf(mc.x, x, fmaf(mc.y, y, fmaf(mc.z, zz, mc.w)));
-                    val *= cf * cf;
-                }
-#pragma unroll
-                for (int aa = 0; aa < KA; aa++)
-#pragma unroll
-                    for (int bb = 0; bb < KB; bb++)
-#pragma unroll
-                        for (int kk = 0; kk < TZ; kk++)
-                            acc[aa][bb][kk] += (q == (aa * KB + bb) * TZ + kk) ? val : 0.0f;
-            }
+            any_global = true;
         }
         if (next && gn.w > 0) commit((p + 1) & 1, gn);
         __syncthreads();  // box p + 1 staged; box p free for p + 2
+    }
+
+    // Projections whose footprint does not fit a box (too large, or the tile reaches behind
+    // the source) take the global-memory path in a pass of their own: with it out of the main
+    // loop, the accumulators stay in their registers across that loop's iterations (a branch
+    // there that also updated them made the compiler copy them twice per projection).
+    if (any_global) {
+        for (int c0 = 0; c0 < P; c0 += kChunk) {
+            footprints(c0, 0);  // the main loop's last barrier freed both footprint buffers
+            __syncthreads();
+            for (int p = c0; p < min(c0 + kChunk, P); p++) {
+                if (uniform(bgs[0][p - c0]).w > 0) continue;
+                const float4 mu = mats[4 * p + 0], mv = mats[4 * p + 1], md = mats[4 * p + 2], mc = mats[4 * p + 3];
+                // One voxel per iteration of a rolled loop and a select chain into the
+                // accumulators, so this rare path does not inflate the register budget.
+                const float *img = proj + (long) p * plane;
+#pragma unroll 1
+                for (int q = 0; q < KA * KB * TZ; q++) {
+                    const int a = q / (KB * TZ), b = (q / TZ) % KB, k = q % TZ;
+                    const float x = a ? xv[KA - 1] : xv[0], y = b ? yv[KB - 1] : yv[0];
+                    const float zz = zs[min(Z0 + k, nz - 1)];
+                    float u, v, d;
+                    map_point<CONE>(mu, mv, md, x, y, zz, u, v, d);
+                    Col<ADDR> col;
+                    col.set(u, W);
+                    float val = fetch<ADDR>(img, W, H, col, v - 0.5f);
+                    if constexpr (CONE) {
+                        const float cf = 1.0f / fmaf(mc.x, x, fmaf(mc.y, y, fmaf(mc.z, zz, mc.w)));
+                        val *= cf * cf;
+                    }
+#pragma unroll
+                    for (int aa = 0; aa < KA; aa++)
+#pragma unroll
+                        for (int bb = 0; bb < KB; bb++)
+#pragma unroll
+                            for (int kk = 0; kk < TZ; kk++)
+                                acc[aa][bb][kk] += (q == (aa * KB + bb) * TZ + kk) ? val : 0.0f;
+                }
+            }
+            __syncthreads();  // bgs[0] read before the next chunk's footprints
+        }
     }
 
 #pragma unroll
