@@ -389,10 +389,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
         }
     };
     auto commit = [&](int buf, const BoxGeom &g) {
+        // one address per projection; the rows are immediate offsets from it
+        float *dst = &box[buf][sr0 * kBoxS + sc];
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; k++) {
-            const int r = sr0 + 4 * k;
-            if (r < g.h) box[buf][r * kBoxS + sc] = st[k];
+            if (sr0 + 4 * k < g.h) dst[4 * k * kBoxS] = st[k];
         }
     };
 
